@@ -1,0 +1,157 @@
+/*
+ * wolfgang_amd C ABI: the drop-in boundary for the packed symplectic Pauli hot
+ * path on MI355X (gfx950).
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (WG_OK or one of the WG_ERR_* categories) and leaves a message for
+ * wg_last_error() on failure (thread-local). The categories map to the
+ * reference's exception types: WG_ERR_INVALID <-> std::invalid_argument
+ * (Python ValueError), WG_ERR_RUNTIME <-> std::runtime_error (RuntimeError),
+ * WG_ERR_TYPE <-> TypeError, WG_ERR_OVERFLOW <-> std::overflow_error.
+ *
+ * Operators are device-resident handles mirroring the reference's
+ * DevicePauliSum::Impl (reference src/hip/device_pauli_sum.hip.hpp:19-29):
+ * SoA packed words x[t*W + w], z[t*W + w] (uint64; qubit q is bit q%64 of word
+ * q/64, W = ceil(n/64)) and coefficients as interleaved complex128 (re, im).
+ *
+ * Unless an entry point says otherwise it runs on the handle's device, on the
+ * given stream (NULL = the device's null stream), and synchronizes that stream
+ * before returning, as the reference's HIP backend does
+ * (reference docs/architecture/rocm_backend.md:441-448).
+ */
+#ifndef WOLFGANG_AMD_H
+#define WOLFGANG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wg_pauli_sum wg_pauli_sum;
+typedef struct wg_comm_matrix wg_comm_matrix;
+
+enum {
+  WG_OK = 0,
+  WG_ERR_INVALID = 1,
+  WG_ERR_RUNTIME = 2,
+  WG_ERR_TYPE = 3,
+  WG_ERR_OVERFLOW = 4
+};
+
+enum { WG_DTYPE_COMPLEX64 = 0, WG_DTYPE_COMPLEX128 = 1 };
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* wg_last_error(void);
+
+/* ABI revision of this header. */
+int wg_abi_version(void);
+
+/* Device discovery; replaces DevicePauliSum::hip_status / hip_available
+ * (reference include/wolfgang/device_pauli_sum.hpp:111-112,
+ * src/hip/device_pauli_sum.hip.cpp:171-221). A missing device is not an error:
+ * count is 0 and wg_last_error() holds the skip reason. */
+int wg_device_count(int* count);
+int wg_device_info(int ordinal, char* name, size_t name_len, char* gfx_target, size_t gfx_len,
+                   size_t* total_memory_bytes, int* compute_units);
+int wg_runtime_versions(int* runtime_version, int* driver_version);
+
+/* DevicePauliSum::from_host (reference include/wolfgang/device_pauli_sum.hpp:104-108;
+ * src/hip/device_pauli_sum.hip.cpp:116-157). coeffs holds 2*num_terms doubles. */
+int wg_pauli_sum_create(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* coeffs, int device, wg_pauli_sum** out);
+
+/* DevicePauliSum::to_host (reference src/hip/device_pauli_sum.hip.cpp:239-266).
+ * x and z hold num_terms*words words, coeffs 2*num_terms doubles. */
+int wg_pauli_sum_to_host(const wg_pauli_sum* op, uint64_t* x, uint64_t* z, double* coeffs);
+
+/* num_qubits/num_terms/words/device accessors
+ * (reference include/wolfgang/device_pauli_sum.hpp:152-155). */
+int wg_pauli_sum_info(const wg_pauli_sum* op, size_t* num_qubits, size_t* num_terms,
+                      size_t* words, int* device);
+
+/* Device addresses of the packed buffers (read-only views, owned by op). */
+int wg_pauli_sum_device_buffers(const wg_pauli_sum* op, const uint64_t** x, const uint64_t** z,
+                                const double** coeffs);
+
+/* Frees the device buffers on the owning device (reference
+ * src/hip/device_pauli_sum.hip.cpp:86-99). NULL is accepted. */
+void wg_pauli_sum_destroy(wg_pauli_sum* op);
+
+/* DevicePauliSum::simplify (reference include/wolfgang/device_pauli_sum.hpp:117;
+ * semantics of src/simplify.cpp:147-430): duplicate (x, z) keys summed in
+ * construction order, |sum| <= atol + rtol*max|c| dropped, canonical
+ * lexicographic (x0, z0, x1, z1, ...) output order. */
+int wg_simplify(const wg_pauli_sum* op, double atol, double rtol, void* stream,
+                wg_pauli_sum** out);
+
+/* DevicePauliSum::matmul (reference include/wolfgang/device_pauli_sum.hpp:148-151;
+ * semantics of src/multiply.cpp:73-402): lhs @ rhs with rhs acting first,
+ * products in lhs-outer / rhs-inner order, then optionally simplified with
+ * atol = 1e-12. max_intermediate_terms is checked before any allocation. */
+int wg_matmul(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, int simplify_output,
+              size_t max_intermediate_terms, void* stream, wg_pauli_sum** out);
+
+/* DevicePauliSum::commutes_with_into (reference
+ * include/wolfgang/device_pauli_sum.hpp:137-140; src/hip/commutation_hip.hip.cpp:72-135):
+ * row-major lhs x rhs flags, 1 = commute, into a caller-owned HOST buffer of
+ * exactly out_len = L*R bytes. */
+int wg_commutes_with_host(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, uint8_t* out,
+                          size_t out_len, size_t max_entries, void* stream);
+
+/* DeviceCommutationMatrix::empty (reference
+ * include/wolfgang/device_commutation_matrix.hpp:206-214). The matrix is stored
+ * as packed bits: row i occupies row_words = ceil(cols/64) uint64 words, bit j%64
+ * of word j/64 is entry (i, j); bits past cols are zero. */
+int wg_comm_matrix_create(size_t rows, size_t cols, int device, wg_comm_matrix** out);
+void wg_comm_matrix_destroy(wg_comm_matrix* m);
+int wg_comm_matrix_info(const wg_comm_matrix* m, size_t* rows, size_t* cols, size_t* row_words,
+                        int* device);
+int wg_comm_matrix_device_bits(const wg_comm_matrix* m, const uint64_t** bits);
+
+/* DevicePauliSum::commutes_with_device_into (reference
+ * include/wolfgang/device_pauli_sum.hpp:144-147; src/hip/commutation_hip.hip.cpp:137-209). */
+int wg_commutes_with_device(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, wg_comm_matrix* out,
+                            size_t max_entries, void* stream);
+
+/* DeviceCommutationMatrix::to_host as dense uint8 (reference
+ * src/hip/device_commutation_matrix.hip.cpp:225-262): out_len = rows*cols. */
+int wg_comm_matrix_to_host_u8(const wg_comm_matrix* m, uint8_t* out, size_t out_len, void* stream);
+
+/* Packed words to host: out_words = rows*row_words. */
+int wg_comm_matrix_to_host_packed(const wg_comm_matrix* m, uint64_t* out, size_t out_words,
+                                  void* stream);
+
+/* count_commuting / count_commuting_rows / count_commuting_cols and
+ * conflict_degrees (reference include/wolfgang/device_commutation_matrix.hpp:216-218;
+ * bindings/python/pauli_sum_py.cpp:598-661). axis -1: total into out[0];
+ * axis 1: per-row counts (out_len = rows); axis 0: per-column counts
+ * (out_len = cols). conflicts != 0 counts anticommuting entries instead. */
+int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint64_t* out,
+                         size_t out_len, void* stream);
+
+/* expectation_statevector_{complex128,complex64,device_pointer}
+ * (reference include/wolfgang/device_pauli_sum.hpp:118-125;
+ * src/expectation.cpp:37-155 semantics): E = sum_k c_k <psi|P_k|psi>.
+ * psi_on_device != 0: psi is a device address on op's device (no copy).
+ * out2 receives (Re E, Im E). */
+int wg_expectation(const wg_pauli_sum* op, const void* psi, int dtype, size_t length,
+                   int psi_on_device, void* stream, double* out2);
+
+/* Forward plus adjoint in one call, all pointers device-resident complex128 on
+ * op's device (no reference counterpart: the new adjoint):
+ *   out2            <- E = sum_k c_k <P_k>
+ *   term_values     <- <psi|P_k|psi> per term (num_terms complex128; real for
+ *                      Hermitian P_k), i.e. dE/dc_k          (may be NULL)
+ *   sigma           <- H psi (length complex128), i.e. dE/dpsi*  (may be NULL)
+ *   sigma_dagger    <- H^dagger psi (coefficients conjugated)    (may be NULL) */
+int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t length,
+                           double* term_values, double* sigma, double* sigma_dagger,
+                           void* stream, double* out2);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WOLFGANG_AMD_H */

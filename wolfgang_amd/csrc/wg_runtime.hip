@@ -1,0 +1,375 @@
+// Runtime: device discovery, operator handles, host<->device transfer, the
+// scratch arena and a device-wide exclusive scan used by simplify.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "wg_internal.hpp"
+
+namespace wg {
+namespace {
+
+thread_local std::string t_last_error;
+
+std::mutex g_arena_map_mutex;
+
+constexpr std::size_t kAlign = 256;
+
+}  // namespace
+
+void set_last_error(const std::string& m) { t_last_error = m; }
+void clear_last_error() { t_last_error.clear(); }
+
+void validate_device(int device) {
+  int count = 0;
+  const hipError_t r = hipGetDeviceCount(&count);
+  if (r != hipSuccess) {
+    (void)hipGetLastError();
+    if (r == hipErrorNoDevice) fail_runtime("no HIP device is available");
+    fail_runtime(std::string("HIP runtime library is unavailable or failed during device "
+                             "discovery: ") + hipGetErrorString(r));
+  }
+  if (count == 0) fail_runtime("no HIP device is available");
+  if (device < 0 || device >= count) fail_invalid("HIP device ordinal is out of range");
+}
+
+void require_same_device(const char* operation, const wg_pauli_sum* a, const wg_pauli_sum* b) {
+  if (a->device == b->device) return;
+  fail_invalid(std::string(operation) +
+               " requires operands on the same accelerator backend and same device "
+               "(operation=" + operation + ", left_backend=hip, left_device=" +
+               std::to_string(a->device) + ", right_backend=hip, right_device=" +
+               std::to_string(b->device) + ", failure_stage=pre_allocation)");
+}
+
+struct Scratch::Arena {
+  std::mutex mutex;
+  void* base = nullptr;
+  std::size_t capacity = 0;
+};
+
+namespace {
+// Arenas live until process exit (one per device that was used).
+Scratch::Arena* arena_for(int device) {
+  static auto* arenas = new std::map<int, Scratch::Arena*>();
+  std::lock_guard<std::mutex> lk(g_arena_map_mutex);
+  auto it = arenas->find(device);
+  if (it != arenas->end()) return it->second;
+  auto* a = new Scratch::Arena();
+  (*arenas)[device] = a;
+  return a;
+}
+}  // namespace
+
+Scratch::Scratch(int device) : arena_(arena_for(device)), lock_(arena_->mutex) {}
+
+Scratch::~Scratch() {
+  if (!overflow_.empty()) {
+    // Work using the overflow buffers has completed (entry points synchronize
+    // before returning); grow the arena to this call's high-water mark.
+    for (void* p : overflow_) device_free(p);
+    device_free(arena_->base);
+    arena_->base = nullptr;
+    arena_->capacity = 0;
+    void* p = nullptr;
+    const std::size_t want = used_ + used_ / 4;
+    if (hipMalloc(&p, want) == hipSuccess) {
+      arena_->base = p;
+      arena_->capacity = want;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+}
+
+void* Scratch::take_bytes(std::size_t bytes) {
+  const std::size_t rounded = (bytes + kAlign - 1) / kAlign * kAlign;
+  if (rounded == 0) return nullptr;
+  const std::size_t offset = used_;
+  used_ += rounded;
+  if (overflow_.empty() && used_ <= arena_->capacity) {
+    return static_cast<char*>(arena_->base) + offset;
+  }
+  void* p = nullptr;
+  check_hip(hipMalloc(&p, rounded), "allocate scratch buffer");
+  overflow_.push_back(p);
+  return p;
+}
+
+wg_pauli_sum* new_pauli_sum(std::size_t n, std::size_t terms, int device) {
+  auto* op = new wg_pauli_sum();
+  op->num_qubits = n;
+  op->words = words_for(n);
+  op->num_terms = terms;
+  op->device = device;
+  try {
+    const std::size_t packed = checked_mul(terms, op->words, "HIP device packed words");
+    op->x = device_alloc<u64>(packed);
+    op->z = device_alloc<u64>(packed);
+    op->coeffs = device_alloc<double2>(terms);
+  } catch (...) {
+    delete op;
+    throw;
+  }
+  return op;
+}
+
+// ------------------------------------------------------------------ scan
+
+namespace {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr std::size_t kScanTile = kScanBlock * kScanItems;
+
+__device__ inline u32 wave_inclusive_scan(u32 v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Exclusive block scan of one value per thread; returns the block total.
+__device__ inline u32 block_exclusive_scan(u32 v, u32* lds_waves, u32* total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const u32 inc = wave_inclusive_scan(v);
+  if (lane == 63) lds_waves[wave] = inc;
+  __syncthreads();
+  u32 before = 0, sum = 0;
+  for (int w = 0; w < kScanBlock / 64; ++w) {
+    const u32 s = lds_waves[w];
+    if (w < wave) before += s;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_reduce_kernel(const u32* in, std::size_t n,
+                                                                 u32* block_sums) {
+  __shared__ u32 lds[kScanBlock / 64];
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kScanTile;
+  u32 s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    const std::size_t idx = base + static_cast<std::size_t>(i) * kScanBlock + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  u32 total;
+  (void)block_exclusive_scan(s, lds, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(const u32* in, std::size_t n,
+                                                                const u32* block_offsets,
+                                                                u32* out, u32* total_out) {
+  __shared__ u32 lds[kScanBlock / 64];
+  // Thread t owns the contiguous run [base + t*kScanItems, +kScanItems).
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kScanTile +
+                           static_cast<std::size_t>(threadIdx.x) * kScanItems;
+  u32 vals[kScanItems];
+  u32 s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const std::size_t idx = base + i;
+    vals[i] = idx < n ? in[idx] : 0;
+    s += vals[i];
+  }
+  u32 total;
+  u32 run = block_exclusive_scan(s, lds, &total);
+  const u32 off = block_offsets ? block_offsets[blockIdx.x] : 0;
+  run += off;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const std::size_t idx = base + i;
+    if (idx < n) out[idx] = run;
+    run += vals[i];
+  }
+  if (total_out != nullptr && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    *total_out = off + total;
+  }
+}
+
+}  // namespace
+
+void exclusive_scan_u32(const u32* in, u32* out, std::size_t n, u32* total_dev, Scratch& scratch,
+                        hipStream_t stream) {
+  if (n == 0) {
+    if (total_dev) check_hip(hipMemsetAsync(total_dev, 0, sizeof(u32), stream), "memset");
+    return;
+  }
+  const std::size_t blocks = (n + kScanTile - 1) / kScanTile;
+  if (blocks == 1) {
+    scan_apply_kernel<<<1, kScanBlock, 0, stream>>>(in, n, nullptr, out, total_dev);
+    check_launch("scan_apply");
+    return;
+  }
+  u32* sums = scratch.take<u32>(blocks);
+  u32* offsets = scratch.take<u32>(blocks);
+  scan_reduce_kernel<<<grid_for(blocks, 1), kScanBlock, 0, stream>>>(in, n, sums);
+  check_launch("scan_reduce");
+  exclusive_scan_u32(sums, offsets, blocks, nullptr, scratch, stream);
+  scan_apply_kernel<<<grid_for(blocks, 1), kScanBlock, 0, stream>>>(in, n, offsets, out,
+                                                                    total_dev);
+  check_launch("scan_apply");
+}
+
+}  // namespace wg
+
+wg_pauli_sum::~wg_pauli_sum() {
+  if (x == nullptr && z == nullptr && coeffs == nullptr) return;
+  int prev = -1;
+  const hipError_t r = hipGetDevice(&prev);
+  (void)hipSetDevice(device);
+  wg::device_free(x);
+  wg::device_free(z);
+  wg::device_free(coeffs);
+  if (r == hipSuccess && prev >= 0 && prev != device) (void)hipSetDevice(prev);
+}
+
+wg_comm_matrix::~wg_comm_matrix() {
+  if (bits == nullptr) return;
+  int prev = -1;
+  const hipError_t r = hipGetDevice(&prev);
+  (void)hipSetDevice(device);
+  wg::device_free(bits);
+  if (r == hipSuccess && prev >= 0 && prev != device) (void)hipSetDevice(prev);
+}
+
+namespace wg {
+const char* last_error_cstr() { return t_last_error.c_str(); }
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+const char* wg_last_error(void) { return last_error_cstr(); }
+
+int wg_abi_version(void) { return 1; }
+
+int wg_device_count(int* count) {
+  return guard([&] {
+    *count = 0;
+    int c = 0;
+    const hipError_t r = hipGetDeviceCount(&c);
+    if (r != hipSuccess) {
+      (void)hipGetLastError();
+      // Absence is reported, not raised (reference src/hip/device_pauli_sum.hip.cpp:36-48).
+      set_last_error(r == hipErrorNoDevice
+                         ? std::string("no HIP device is available")
+                         : std::string("HIP runtime library is unavailable or failed during "
+                                       "device discovery: ") + hipGetErrorString(r));
+      return;
+    }
+    *count = c;
+    if (c == 0) set_last_error("no HIP device is available");
+  });
+}
+
+int wg_device_info(int ordinal, char* name, size_t name_len, char* gfx_target, size_t gfx_len,
+                   size_t* total_memory_bytes, int* compute_units) {
+  return guard([&] {
+    validate_device(ordinal);
+    hipDeviceProp_t prop{};
+    check_hip(hipGetDeviceProperties(&prop, ordinal), "get device properties");
+    if (name && name_len) {
+      std::strncpy(name, prop.name, name_len - 1);
+      name[name_len - 1] = '\0';
+    }
+    if (gfx_target && gfx_len) {
+      std::strncpy(gfx_target, prop.gcnArchName, gfx_len - 1);
+      gfx_target[gfx_len - 1] = '\0';
+    }
+    if (total_memory_bytes) *total_memory_bytes = prop.totalGlobalMem;
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+  });
+}
+
+int wg_runtime_versions(int* runtime_version, int* driver_version) {
+  return guard([&] {
+    *runtime_version = 0;
+    *driver_version = 0;
+    (void)hipRuntimeGetVersion(runtime_version);
+    (void)hipDriverGetVersion(driver_version);
+    (void)hipGetLastError();
+  });
+}
+
+int wg_pauli_sum_create(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* coeffs, int device, wg_pauli_sum** out) {
+  return guard([&] {
+    *out = nullptr;
+    validate_device(device);
+    const std::size_t W = words_for(num_qubits);
+    const std::size_t packed = checked_mul(num_terms, W, "HIP device packed words");
+    if (packed != 0 && (x == nullptr || z == nullptr)) fail_invalid("packed mask pointers must be non-null");
+    if (num_terms != 0 && coeffs == nullptr) fail_invalid("coefficient pointer must be non-null");
+    DeviceScope scope(device);
+    std::unique_ptr<wg_pauli_sum> op(new_pauli_sum(num_qubits, num_terms, device));
+    if (packed) {
+      check_hip(hipMemcpy(op->x, x, packed * sizeof(u64), hipMemcpyHostToDevice),
+                "copy host buffer to device");
+      check_hip(hipMemcpy(op->z, z, packed * sizeof(u64), hipMemcpyHostToDevice),
+                "copy host buffer to device");
+    }
+    if (num_terms) {
+      check_hip(hipMemcpy(op->coeffs, coeffs, num_terms * sizeof(double2), hipMemcpyHostToDevice),
+                "copy host buffer to device");
+    }
+    *out = op.release();
+  });
+}
+
+int wg_pauli_sum_to_host(const wg_pauli_sum* op, uint64_t* x, uint64_t* z, double* coeffs) {
+  return guard([&] {
+    require_op(op);
+    DeviceScope scope(op->device);
+    const std::size_t packed = op->num_terms * op->words;
+    if (packed) {
+      check_hip(hipMemcpy(x, op->x, packed * sizeof(u64), hipMemcpyDeviceToHost),
+                "copy device buffer to host");
+      check_hip(hipMemcpy(z, op->z, packed * sizeof(u64), hipMemcpyDeviceToHost),
+                "copy device buffer to host");
+    }
+    if (op->num_terms) {
+      check_hip(hipMemcpy(coeffs, op->coeffs, op->num_terms * sizeof(double2),
+                          hipMemcpyDeviceToHost),
+                "copy device buffer to host");
+    }
+  });
+}
+
+int wg_pauli_sum_info(const wg_pauli_sum* op, size_t* num_qubits, size_t* num_terms, size_t* words,
+                      int* device) {
+  return guard([&] {
+    require_op(op);
+    if (num_qubits) *num_qubits = op->num_qubits;
+    if (num_terms) *num_terms = op->num_terms;
+    if (words) *words = op->words;
+    if (device) *device = op->device;
+  });
+}
+
+int wg_pauli_sum_device_buffers(const wg_pauli_sum* op, const uint64_t** x, const uint64_t** z,
+                                const double** coeffs) {
+  return guard([&] {
+    require_op(op);
+    if (x) *x = op->x;
+    if (z) *z = op->z;
+    if (coeffs) *coeffs = reinterpret_cast<const double*>(op->coeffs);
+  });
+}
+
+void wg_pauli_sum_destroy(wg_pauli_sum* op) { delete op; }
+
+}  // extern "C"
