@@ -150,6 +150,15 @@ int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t len
                            double* term_values, double* sigma, double* sigma_dagger,
                            void* stream, double* out2);
 
+/* Kernel timing with HIP events (no reference counterpart; measurement only).
+ * While enabled, the library brackets each launch of its hot kernels
+ * ("expect_direct", "commute_packed", "product", "radix_scatter", ...) with an
+ * event pair on the launch stream; wg_profile_get resolves the pending pairs and
+ * returns the accumulated milliseconds and launch count for one kernel name. */
+int wg_profile_enable(int enabled);
+int wg_profile_reset(void);
+int wg_profile_get(const char* kernel, double* total_ms, uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,115 @@
+"""GPU parity for statevector expectation and its adjoint (FP64; 1e-12 relative).
+
+Tolerance: |E_gpu - E_ref| <= 1e-12 * scale with scale = sum_k |c_k| * ||psi||^2,
+the bound on |E| from |<P_k>| <= ||psi||^2; E_ref is the oracle's compensated
+(double-double) sum, so the tolerance measures the device error alone. Per-term
+values are checked against the oracle's compensated per-term sums with the same
+relative bound (scale ||psi||^2)."""
+
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import kat_datasets as kd
+from conftest import packed
+from wolfgang_amd import PauliSum
+from wolfgang_amd.workloads import chemistry_masks, random_pauli_masks, random_statevector
+
+pytestmark = pytest.mark.gpu
+KATS = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_kats.json").read_text())
+RTOL = 1e-12
+
+
+def scale(op, psi):
+    return float(np.abs(op.coeffs()).sum() * np.vdot(psi, psi).real)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 7, 12, 16])
+@pytest.mark.parametrize("gen", ["random", "chemistry", "diagonal"])
+def test_expectation_matches_oracle(gpu, orc, n, gen):
+    if gen == "chemistry" and n < 2:
+        pytest.skip("chemistry generator needs two qubits")
+    if gen == "random":
+        op = random_pauli_masks(n, 300, 0.4, 17 * n + 1)
+    elif gen == "chemistry":
+        op = chemistry_masks(n, 300, 17 * n + 2)
+    else:
+        r = random_pauli_masks(n, 120, 0.5, 17 * n + 3)
+        op = PauliSum.from_packed(n, np.zeros_like(r.x_words()), r.z_words() | r.x_words(), r.coeffs())
+    psi = random_statevector(n, 99 + n)
+    want = orc.expectation_compensated(*packed(op), psi)
+    got = op.to_device().expectation_statevector(psi)
+    assert abs(got - want) <= RTOL * scale(op, psi)
+    # Also the plain reference-order oracle (sequential FP64 sum).
+    assert abs(got - orc.expectation(*packed(op), psi)) <= 4 * RTOL * scale(op, psi)
+    # Deterministic: identical bits on repeat.
+    dop = op.to_device()
+    assert dop.expectation_statevector(psi) == dop.expectation_statevector(psi)
+
+
+def test_expectation_complex64_and_device_tensor(gpu, orc):
+    import torch
+
+    n = 10
+    op = random_pauli_masks(n, 256, 0.4, 5)
+    psi = random_statevector(n, 6)
+    dop = op.to_device()
+    want64 = orc.expectation(*packed(op), psi.astype(np.complex64))
+    got64 = dop.expectation_statevector(psi.astype(np.complex64))
+    assert abs(got64 - want64) <= RTOL * scale(op, psi)
+    t = torch.from_numpy(psi).to("cuda:0")
+    assert abs(dop.expectation_statevector(t) - dop.expectation_statevector(psi)) == 0
+    t64 = torch.from_numpy(psi.astype(np.complex64)).to("cuda:0")
+    assert dop.expectation_statevector(t64) == got64
+
+
+@pytest.mark.parametrize("case", KATS["expectation"], ids=lambda c: c["name"])
+def test_expectation_reference_results(gpu, case):
+    labels, coeffs, psi = kd.campaign6_expectation(case)
+    got = PauliSum.from_labels(labels, coeffs).to_device().expectation_statevector(psi)
+    # Published values are MI300X device results; reference tolerance 1e-11.
+    assert got.real == pytest.approx(case["result_real"], abs=1e-11)
+    assert got.imag == pytest.approx(case["result_imag"], abs=1e-11)
+
+
+def test_expectation_edges_and_errors(gpu):
+    psi = random_statevector(3, 1)
+    assert PauliSum.empty(3).to_device().expectation_statevector(psi) == 0
+    ident = PauliSum.from_labels(["III"], [2.5])
+    assert ident.to_device().expectation_statevector(psi) == pytest.approx(2.5, abs=1e-15)
+    d = PauliSum.from_labels(["XYZ"]).to_device()
+    with pytest.raises(ValueError, match="len\\(psi\\) == 2 \\*\\* num_qubits"):
+        d.expectation_statevector(psi[:4])
+    with pytest.raises(TypeError, match="complex64 or complex128"):
+        d.expectation_statevector(psi.real.copy())
+    with pytest.raises(ValueError, match="1-dimensional"):
+        d.expectation_statevector(psi.reshape(2, 4))
+    with pytest.raises(TypeError, match="C-contiguous"):
+        d.expectation_statevector(np.concatenate([psi, psi])[::2])
+
+
+@pytest.mark.parametrize("n", [1, 6, 11])
+def test_adjoint_sigma_and_term_values(gpu, orc, n):
+    import torch
+
+    op = random_pauli_masks(n, 200, 0.4, 3 * n)
+    psi = random_statevector(n, 4 * n)
+    dop = op.to_device()
+    t = torch.from_numpy(psi).to("cuda:0")
+    sigma = torch.empty_like(t)
+    sigma_dag = torch.empty_like(t)
+    tv = torch.empty(op.num_terms, dtype=torch.complex128, device="cuda:0")
+    e = dop.expectation_adjoint(t.data_ptr(), t.numel(), tv.data_ptr(), sigma.data_ptr(), sigma_dag.data_ptr())
+    want_sigma = orc.apply(*packed(op), psi)
+    cdag = packed(op)[:3] + (op.coeffs().conj(),)
+    want_dag = orc.apply(*cdag, psi)
+    s = scale(op, psi)
+    np.testing.assert_allclose(sigma.cpu().numpy(), want_sigma, rtol=0, atol=RTOL * s)
+    np.testing.assert_allclose(sigma_dag.cpu().numpy(), want_dag, rtol=0, atol=RTOL * s)
+    want_tv = orc.term_values(n, op.x_words(), op.z_words(), psi)
+    np.testing.assert_allclose(tv.cpu().numpy().real, want_tv, rtol=0, atol=RTOL)
+    assert abs(e - orc.expectation_compensated(*packed(op), psi)) <= RTOL * s
+    # <psi|sigma> = E
+    assert abs(np.vdot(psi, sigma.cpu().numpy()) - e) <= RTOL * s

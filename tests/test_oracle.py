@@ -1,0 +1,246 @@
+"""Pins the CPU oracle before it is trusted: reference-published known answers,
+the reference's own unit fixtures, the compiled reference where it exists, and
+closed forms (dense matrices, brute force)."""
+
+import functools
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import kat_datasets as kd
+from conftest import packed
+from wolfgang_amd import PauliSum
+from wolfgang_amd.workloads import random_pauli_masks, random_statevector, chemistry_masks
+
+KATS = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_kats.json").read_text())
+
+I2 = np.eye(2, dtype=complex)
+PX = np.array([[0, 1], [1, 0]], dtype=complex)
+PY = np.array([[0, -1j], [1j, 0]], dtype=complex)
+PZ = np.array([[1, 0], [0, -1]], dtype=complex)
+MAT = {"I": I2, "X": PX, "Y": PY, "Z": PZ}
+
+
+def label_matrix(label):
+    # Right-most character is qubit 0 = least significant index bit.
+    return functools.reduce(np.kron, [MAT[ch] for ch in label])
+
+
+def op_matrix(op):
+    labels, coeffs = op.to_labels()
+    n = op.num_qubits
+    out = np.zeros((2**n, 2**n), dtype=complex)
+    for label, c in zip(labels, coeffs):
+        out += c * label_matrix(label)
+    return out
+
+
+def to_op(p):
+    return PauliSum.from_packed(*p)
+
+
+# ------------------------------------------------------ reference known answers
+
+@pytest.mark.parametrize("case", KATS["commutation"], ids=lambda c: c["name"])
+def test_commutation_counts_match_reference_digests(orc, case):
+    lhs, rhs = kd.commutation_operands(case)
+    L, R = PauliSum.from_labels(lhs), PauliSum.from_labels(rhs)
+    m = orc.commutes_with(case["num_qubits"], (L.x_words(), L.z_words()), (R.x_words(), R.z_words()),
+                          variant=2, threads=8)
+    assert int(m.sum()) == case["commuting_count"]
+    assert m.size - int(m.sum()) == case["conflict_count"]
+
+
+@pytest.mark.parametrize("case", KATS["simplify"], ids=lambda c: c["name"])
+def test_simplify_matches_reference_canonical_hash(orc, case):
+    labels, coeffs = kd.simplify_operator_labels(case)
+    op = PauliSum.from_labels(labels, coeffs)
+    out = to_op(orc.simplify(*packed(op)))
+    out_labels, out_coeffs = out.to_labels()
+    assert len(out_labels) == case["output_terms"]
+    assert kd.label_hash(out_labels, out_coeffs) == case["canonical_label_hash"]
+    assert float(np.abs(out_coeffs).sum()) == pytest.approx(case["coefficient_l1"], rel=1e-12)
+
+
+@pytest.mark.parametrize("case", KATS["expectation"], ids=lambda c: c["name"])
+def test_expectation_matches_reference_results(orc, case):
+    labels, coeffs, psi = kd.campaign6_expectation(case)
+    op = PauliSum.from_labels(labels, coeffs)
+    assert kd.label_hash(*op.to_labels()) == case["operator_label_hash"]
+    e = orc.expectation(*packed(op), psi)
+    # The published values are device results (reference bench_rocm_kernels.py:1958).
+    assert e.real == pytest.approx(case["result_real"], abs=1e-11)
+    assert e.imag == pytest.approx(case["result_imag"], abs=1e-11)
+
+
+@pytest.mark.parametrize("case", KATS["matmul"], ids=lambda c: c["name"])
+def test_matmul_matches_reference_output_terms(orc, case):
+    lhs = PauliSum.from_labels(*kd.campaign6_matmul_operand(case, "lhs"))
+    rhs = PauliSum.from_labels(*kd.campaign6_matmul_operand(case, "rhs"))
+    n, x, z, c = orc.matmul(case["num_qubits"], packed(lhs)[1:], packed(rhs)[1:], simplify=case["simplify"])
+    assert c.shape[0] == case["output_terms"]
+    assert float(np.abs(c).sum()) == pytest.approx(case["coefficient_l1"], rel=1e-11)
+
+
+# ------------------------------------------------------ reference unit fixtures
+
+@pytest.mark.parametrize(
+    ("lhs", "rhs", "label", "phase"),
+    [("X", "Y", "Z", 1j), ("Y", "X", "Z", -1j), ("Y", "Z", "X", 1j),
+     ("Z", "Y", "X", -1j), ("Z", "X", "Y", 1j), ("X", "Z", "Y", -1j)],
+)
+def test_phase_table(orc, lhs, rhs, label, phase):
+    # reference tests/test_phase5_arithmetic.py:108-127
+    a, b = PauliSum.from_labels([lhs], [2.0]), PauliSum.from_labels([rhs], [3.0])
+    out = to_op(orc.matmul(1, packed(a)[1:], packed(b)[1:]))
+    labels, coeffs = out.to_labels()
+    assert labels == [label] and coeffs[0] == 6.0 * phase
+
+
+def test_nested_loop_order_and_duplicate_fold(orc):
+    # reference tests/test_phase5_arithmetic.py:130-145
+    lhs = PauliSum.from_labels(["X", "Z"], [1.0, 2.0])
+    rhs = PauliSum.from_labels(["Y", "I"], [3.0, 4.0])
+    labels, coeffs = to_op(orc.matmul(1, packed(lhs)[1:], packed(rhs)[1:], simplify=False)).to_labels()
+    assert labels == ["Z", "X", "X", "Z"]
+    np.testing.assert_array_equal(coeffs, [3j, 4, -6j, 8])
+    lhs = PauliSum.from_labels(["X", "X"], [1.0, 2.0])
+    rhs = PauliSum.from_labels(["Y"], [3.0])
+    labels, coeffs = to_op(orc.matmul(1, packed(lhs)[1:], packed(rhs)[1:])).to_labels()
+    assert labels == ["Z"] and coeffs[0] == 9j
+
+
+def test_simplify_unit_fixtures(orc):
+    # reference tests/test_phase4_simplify.py:42-86
+    op = PauliSum.from_labels(["Z", "X", "X", "I", "Z"], [1.0, 2.0, 3.0, 4.0, -0.25])
+    labels, coeffs = to_op(orc.simplify(*packed(op), 0.0, 0.0)).to_labels()
+    assert labels == ["I", "Z", "X"]
+    np.testing.assert_array_equal(coeffs, [4.0, 0.75, 5.0])
+    op = PauliSum.from_labels(["X", "X", "Z", "Y"], [0.4, 0.6, 10.0, 1.0 + 1.0e-9])
+    labels, coeffs = to_op(orc.simplify(*packed(op), 0.0, 0.1)).to_labels()
+    assert labels == ["Z", "Y"]
+    op = PauliSum.from_labels(["XX", "ZZ"], [1.0e-13, 0.0])
+    out = to_op(orc.simplify(*packed(op)))
+    assert out.num_terms == 0 and out.num_qubits == 2
+    with pytest.raises(ValueError, match="tolerances must be non-negative"):
+        orc.simplify(*packed(PauliSum.from_labels(["X"], [1.0])), -1e-12, 0.0)
+
+    def label_for(n, d):
+        chars = ["I"] * n
+        for q, p in d.items():
+            chars[n - 1 - q] = p
+        return "".join(chars)
+
+    n = 130
+    q64z, q64x, q0z, q0x = (label_for(n, {64: "Z"}), label_for(n, {64: "X"}),
+                            label_for(n, {0: "Z"}), label_for(n, {0: "X"}))
+    op = PauliSum.from_labels([q0x, q64x, q0z, q64z], [4.0, 2.0, 3.0, 1.0])
+    labels, coeffs = to_op(orc.simplify(*packed(op), 0.0, 0.0)).to_labels()
+    assert labels == [q64z, q64x, q0z, q0x]
+    np.testing.assert_array_equal(coeffs, [1.0, 2.0, 3.0, 4.0])
+
+
+def test_commutation_fixture_and_symmetry(orc):
+    # reference tests/test_phase6_commutation_grouping.py:55-88, 115-129
+    lhs = PauliSum.from_labels(["X", "Z"], [1.0, 2.0])
+    rhs = PauliSum.from_labels(["X", "Y", "I"], [1.0, 2.0, 3.0])
+    m = orc.commutes_with(1, (lhs.x_words(), lhs.z_words()), (rhs.x_words(), rhs.z_words()))
+    np.testing.assert_array_equal(m, [[1, 0, 1], [0, 0, 1]])
+    op = random_pauli_masks(70, 40, 0.4, 1979)
+    xz = (op.x_words(), op.z_words())
+    m = orc.commutes_with(70, xz, xz)
+    np.testing.assert_array_equal(m, m.T)
+    assert m.diagonal().all()
+
+
+# ------------------------------------------------------ compiled reference
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 130])
+def test_oracle_bitwise_equals_compiled_reference(orc, ref_lib, n):
+    for seed in range(3):
+        a = random_pauli_masks(n, 37, 0.3, 100 * n + seed)
+        b = random_pauli_masks(n, 29, 0.3, 100 * n + seed + 50)
+        for simplify in (False, True):
+            got = orc.matmul(n, packed(a)[1:], packed(b)[1:], simplify=simplify)
+            want = ref_lib.matmul(n, packed(a)[1:], packed(b)[1:], simplify=simplify)
+            for g, w in zip(got[1:], want[1:]):
+                np.testing.assert_array_equal(g.view(np.uint64), w.view(np.uint64))
+        got = orc.commutes_with(n, (a.x_words(), a.z_words()), (b.x_words(), b.z_words()))
+        want = ref_lib.commutes_with(n, (a.x_words(), a.z_words()), (b.x_words(), b.z_words()))
+        np.testing.assert_array_equal(got, want)
+    # Duplicate-heavy: the hash-accumulation paths with real coefficients
+    # (signed zeros in the imaginary parts make the seed observable).
+    pool = random_pauli_masks(n, 4, 0.5, 7 + n)
+    idx = np.arange(4096) % 4
+    dup = PauliSum.from_packed(n, pool.x_words()[idx], pool.z_words()[idx],
+                               (np.arange(4096) % 13 - 6).astype(complex) * np.where(idx == 1, -1, 1))
+    got = orc.simplify(*packed(dup), 0.0, 0.0)
+    want = ref_lib.simplify(*packed(dup), 0.0, 0.0)
+    for g, w in zip(got[1:], want[1:]):
+        np.testing.assert_array_equal(g.view(np.uint64), w.view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [3, 9])
+def test_oracle_expectation_bitwise_equals_compiled_reference(orc, ref_lib, n):
+    op = random_pauli_masks(n, 50, 0.5, 11 * n)
+    psi = random_statevector(n, 12 * n)
+    assert orc.expectation(*packed(op), psi) == ref_lib.expectation(*packed(op), psi)
+    assert orc.expectation(*packed(op), psi.astype(np.complex64)) == ref_lib.expectation(
+        *packed(op), psi.astype(np.complex64))
+    diag = PauliSum.from_packed(n, np.zeros((80, 1), np.uint64), op.z_words()[:1].repeat(80, 0) ^ np.arange(80, dtype=np.uint64)[:, None] % (1 << n), np.ones(80))
+    assert orc.expectation(*packed(diag), psi) == ref_lib.expectation(*packed(diag), psi)
+
+
+# ------------------------------------------------------ closed forms
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_matmul_against_dense_matrices(orc, n):
+    # reference tests/test_phase5_arithmetic.py:209-216
+    a = random_pauli_masks(n, 5, 0.7, 200 + n)
+    b = random_pauli_masks(n, 4, 0.7, 300 + n)
+    prod = to_op(orc.matmul(n, packed(a)[1:], packed(b)[1:]))
+    np.testing.assert_allclose(op_matrix(prod), op_matrix(a) @ op_matrix(b), rtol=1e-12, atol=1e-12)
+
+
+def test_commutation_against_dense_matrices(orc):
+    n = 3
+    a = random_pauli_masks(n, 12, 0.7, 41)
+    b = random_pauli_masks(n, 9, 0.7, 42)
+    m = orc.commutes_with(n, (a.x_words(), a.z_words()), (b.x_words(), b.z_words()))
+    la, _ = a.to_labels()
+    lb, _ = b.to_labels()
+    for i, p in enumerate(la):
+        for j, q in enumerate(lb):
+            P, Q = label_matrix(p), label_matrix(q)
+            assert m[i, j] == int(np.allclose(P @ Q, Q @ P))
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8])
+def test_expectation_sigma_and_terms_against_dense(orc, n):
+    # reference tests/test_phase8_expectation.py:63-77 (dense oracle, n <= 8)
+    op = random_pauli_masks(n, 30, 0.6, 500 + n)
+    psi = random_statevector(n, 600 + n)
+    H = op_matrix(op)
+    want = np.vdot(psi, H @ psi)
+    assert abs(orc.expectation(*packed(op), psi) - want) < 1e-11
+    assert abs(orc.expectation_compensated(*packed(op), psi) - want) < 1e-12
+    np.testing.assert_allclose(orc.apply(*packed(op), psi), H @ psi, atol=1e-12)
+    labels, _ = op.to_labels()
+    tv = orc.term_values(n, op.x_words(), op.z_words(), psi)
+    for k, label in enumerate(labels):
+        assert tv[k] == pytest.approx(np.vdot(psi, label_matrix(label) @ psi).real, abs=1e-12)
+
+
+def test_chemistry_generator_is_hermitian_and_shaped():
+    op = chemistry_masks(10, 400, 3)
+    labels, coeffs = op.to_labels()
+    assert np.all(coeffs.imag == 0)
+    for label in labels:
+        stripped = label.strip("I")
+        if "X" in stripped or "Y" in stripped:
+            assert stripped[0] == stripped[-1] and stripped[0] in "XY"
+            assert set(stripped[1:-1]) <= {"Z"}
+        else:
+            assert stripped.count("Z") in (1, 2) and set(stripped) <= {"Z", "I"}

@@ -1,0 +1,403 @@
+// Commutation matrix as a packed bit matrix.
+//
+// Entry (i, j) = 1 iff the parity of popcount((lx_i & rz_j) ^ (lz_i & rx_j))
+// over all W words is even (reference src/detail/commutation.hpp:10-46,
+// src/simd/commute_kernels_scalar.cpp:29-90). Row i of the output holds
+// row_words = ceil(R/64) uint64 words; bit j%64 of word j/64 is entry (i, j).
+//
+// Kernel shape (gfx950, wave64): a block owns 64 lhs rows (one per lane) and
+// kWordsPerBlock output words (64 rhs terms each). Every rhs term a wave visits
+// is wave-uniform, so its masks are fetched through scalar loads and enter the
+// VALU as SGPR operands; the per-pair cost is a handful of 32-bit AND/XOR ops,
+// one popcount and one bit insert. Results are staged through LDS so each
+// row's run of output words is written with contiguous stores.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "wg_internal.hpp"
+
+namespace wg {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kWordsPerWave = 4;
+constexpr int kWordsPerBlock = kWaves * kWordsPerWave;
+
+template <int WC>
+__device__ inline u32 conflict_fold(const u64* __restrict__ rx, const u64* __restrict__ rz,
+                                    std::size_t col, std::size_t W, const u64* lxr,
+                                    const u64* lzr) {
+  u64 acc = 0;
+  if constexpr (WC > 0) {
+#pragma unroll
+    for (int w = 0; w < WC; ++w) {
+      acc ^= (lxr[w] & rz[col * WC + w]) ^ (lzr[w] & rx[col * WC + w]);
+    }
+  } else {
+    for (std::size_t w = 0; w < W; ++w) {
+      acc ^= (lxr[w] & rz[col * W + w]) ^ (lzr[w] & rx[col * W + w]);
+    }
+  }
+  return static_cast<u32>(acc) ^ static_cast<u32>(acc >> 32);
+}
+
+// WC = compile-time word count (1, 2, 4) or 0 for a runtime count. For WC == 0
+// the lane's row words are re-read from global memory (cached) per pair.
+template <int WC>
+__global__ __launch_bounds__(kBlock) void commute_packed_kernel(
+    const u64* __restrict__ lx, const u64* __restrict__ lz, const u64* __restrict__ rx,
+    const u64* __restrict__ rz, std::size_t L, std::size_t R, std::size_t W,
+    std::size_t row_words, std::size_t word_blocks, u64* __restrict__ out) {
+  __shared__ u64 stage[64][kWordsPerBlock + 1];
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const std::size_t wb = blockIdx.x % word_blocks;
+  const std::size_t rb = blockIdx.x / word_blocks;
+  const std::size_t row = rb * 64 + lane;
+  const std::size_t safe_row = row < L ? row : (L - 1);
+
+  constexpr int kRegWords = WC > 0 ? WC : 1;
+  u64 lxr[kRegWords], lzr[kRegWords];
+  if constexpr (WC > 0) {
+#pragma unroll
+    for (int w = 0; w < WC; ++w) {
+      lxr[w] = lx[safe_row * WC + w];
+      lzr[w] = lz[safe_row * WC + w];
+    }
+  }
+
+  for (int k = 0; k < kWordsPerWave; ++k) {
+    const std::size_t word = wb * kWordsPerBlock + wave * kWordsPerWave + k;
+    u64 bits = 0;
+    if (word < row_words) {
+      const std::size_t c0 = word * 64;
+      const std::size_t ncols = (R - c0) < 64 ? (R - c0) : 64;
+      if constexpr (WC > 0) {
+        if (ncols == 64) {
+          u32 lo = 0, hi = 0;
+#pragma unroll
+          for (int b = 0; b < 32; ++b) {
+            const u32 f = conflict_fold<WC>(rx, rz, c0 + b, W, lxr, lzr);
+            lo |= ((__popc(f) & 1U) ^ 1U) << b;
+          }
+#pragma unroll
+          for (int b = 0; b < 32; ++b) {
+            const u32 f = conflict_fold<WC>(rx, rz, c0 + 32 + b, W, lxr, lzr);
+            hi |= ((__popc(f) & 1U) ^ 1U) << b;
+          }
+          bits = (static_cast<u64>(hi) << 32) | lo;
+        } else {
+          for (std::size_t b = 0; b < ncols; ++b) {
+            const u32 f = conflict_fold<WC>(rx, rz, c0 + b, W, lxr, lzr);
+            bits |= static_cast<u64>((__popc(f) & 1U) ^ 1U) << b;
+          }
+        }
+      } else {
+        for (std::size_t b = 0; b < ncols; ++b) {
+          u64 acc = 0;
+          for (std::size_t w = 0; w < W; ++w) {
+            acc ^= (lx[safe_row * W + w] & rz[(c0 + b) * W + w]) ^
+                   (lz[safe_row * W + w] & rx[(c0 + b) * W + w]);
+          }
+          const u32 f = static_cast<u32>(acc) ^ static_cast<u32>(acc >> 32);
+          bits |= static_cast<u64>((__popc(f) & 1U) ^ 1U) << b;
+        }
+      }
+    }
+    stage[lane][wave * kWordsPerWave + k] = bits;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * kWordsPerBlock; i += kBlock) {
+    const int r = i / kWordsPerBlock;
+    const int w = i % kWordsPerBlock;
+    const std::size_t orow = rb * 64 + r;
+    const std::size_t oword = wb * kWordsPerBlock + w;
+    if (orow < L && oword < row_words) out[orow * row_words + oword] = stage[r][w];
+  }
+}
+
+// Dense uint8 view: out[i*R + j] = bit (i, j).
+__global__ __launch_bounds__(256) void unpack_u8_kernel(const u64* __restrict__ bits,
+                                                        std::size_t rows, std::size_t cols,
+                                                        std::size_t row_words,
+                                                        std::uint8_t* __restrict__ out) {
+  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  const std::size_t total = rows * cols;
+  for (std::size_t i = e; i < total; i += stride) {
+    const std::size_t r = i / cols, c = i % cols;
+    out[i] = static_cast<std::uint8_t>((bits[r * row_words + c / 64] >> (c % 64)) & 1U);
+  }
+}
+
+// One wave per row: popcount of the row's words; optional per-row output and
+// a total accumulated with integer atomics (order-independent, so exact).
+__global__ __launch_bounds__(256) void count_rows_kernel(const u64* __restrict__ bits,
+                                                         std::size_t rows, std::size_t row_words,
+                                                         unsigned long long* row_counts,
+                                                         unsigned long long* total) {
+  const std::size_t wave_global =
+      (static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+  const unsigned lane = threadIdx.x & 63;
+  if (wave_global >= rows) return;
+  unsigned long long c = 0;
+  for (std::size_t w = lane; w < row_words; w += 64) c += __popcll(bits[wave_global * row_words + w]);
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (lane == 0) {
+    if (row_counts) row_counts[wave_global] = c;
+    if (total) atomicAdd(total, c);
+  }
+}
+
+// Column counts: thread per column over a chunk of rows; the 64 lanes of a wave
+// share one packed word per row (broadcast load).
+constexpr std::size_t kColRowChunk = 2048;
+__global__ __launch_bounds__(256) void count_cols_kernel(const u64* __restrict__ bits,
+                                                         std::size_t rows, std::size_t cols,
+                                                         std::size_t row_words,
+                                                         std::size_t col_blocks,
+                                                         unsigned long long* col_counts) {
+  const std::size_t cb = blockIdx.x % col_blocks;
+  const std::size_t chunk = blockIdx.x / col_blocks;
+  const std::size_t col = cb * blockDim.x + threadIdx.x;
+  if (col >= cols) return;
+  const std::size_t r0 = chunk * kColRowChunk;
+  const std::size_t r1 = (r0 + kColRowChunk) < rows ? (r0 + kColRowChunk) : rows;
+  const std::size_t word = col / 64;
+  const unsigned bit = col % 64;
+  unsigned long long c = 0;
+  for (std::size_t r = r0; r < r1; ++r) c += (bits[r * row_words + word] >> bit) & 1U;
+  if (c) atomicAdd(&col_counts[col], c);
+}
+
+void check_entries(std::size_t L, std::size_t R, std::size_t max_entries) {
+  // Reference src/detail/checked_arithmetic.hpp:49-63.
+  if (L != 0 && R > SIZE_MAX / L) {
+    fail_invalid("commutation matrix entry count overflows size_t");
+  }
+  if (L * R > max_entries) {
+    fail_invalid("commutation matrix entry count exceeds max_commutation_matrix_entries");
+  }
+}
+
+void launch_commute(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, u64* out,
+                    std::size_t row_words, hipStream_t stream) {
+  const std::size_t L = lhs->num_terms, R = rhs->num_terms, W = lhs->words;
+  if (L == 0 || R == 0) return;
+  const std::size_t word_blocks = (row_words + kWordsPerBlock - 1) / kWordsPerBlock;
+  const std::size_t row_blocks = (L + 63) / 64;
+  const unsigned grid = grid_for(checked_mul(word_blocks, row_blocks, "commutation grid"), 1);
+  ProfileScope prof("commute_packed", stream);
+  switch (W) {
+    case 0: {
+      // Zero-qubit operators: every pair commutes.
+      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
+                                                            0, row_words, word_blocks, out);
+      break;
+    }
+    case 1:
+      commute_packed_kernel<1><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
+                                                            1, row_words, word_blocks, out);
+      break;
+    case 2:
+      commute_packed_kernel<2><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
+                                                            2, row_words, word_blocks, out);
+      break;
+    case 4:
+      commute_packed_kernel<4><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
+                                                            4, row_words, word_blocks, out);
+      break;
+    default:
+      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
+                                                            W, row_words, word_blocks, out);
+      break;
+  }
+  check_launch("commute_packed");
+}
+
+void validate_operands(const char* operation, const wg_pauli_sum* lhs, const wg_pauli_sum* rhs) {
+  require_op(lhs);
+  require_op(rhs);
+  require_same_device(operation, lhs, rhs);
+  if (lhs->num_qubits != rhs->num_qubits) {
+    fail_invalid("PauliSum commutes_with requires the same num_qubits");
+  }
+}
+
+}  // namespace
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+int wg_commutes_with_host(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, uint8_t* out,
+                          size_t out_len, size_t max_entries, void* stream_ptr) {
+  return guard([&] {
+    validate_operands("HIP commutes_with", lhs, rhs);
+    const std::size_t L = lhs->num_terms, R = rhs->num_terms;
+    check_entries(L, R, max_entries);
+    if (out_len != L * R) {
+      fail_invalid("HIP commutes_with output buffer size does not match entry count");
+    }
+    if (L * R == 0) return;
+    if (out == nullptr) fail_invalid("commutes_with output pointer must be non-null");
+    DeviceScope scope(lhs->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    Scratch scratch(lhs->device);
+    const std::size_t row_words = (R + 63) / 64;
+    u64* bits = scratch.take<u64>(checked_mul(L, row_words, "commutation matrix"));
+    std::uint8_t* dense = scratch.take<std::uint8_t>(L * R);
+    launch_commute(lhs, rhs, bits, row_words, stream);
+    unpack_u8_kernel<<<grid_for(L * R, 256 * 8), 256, 0, stream>>>(bits, L, R, row_words, dense);
+    check_launch("unpack_u8");
+    check_hip(hipMemcpyAsync(out, dense, L * R, hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+  });
+}
+
+int wg_comm_matrix_create(size_t rows, size_t cols, int device, wg_comm_matrix** out) {
+  return guard([&] {
+    *out = nullptr;
+    validate_device(device);
+    (void)checked_mul(rows, cols, "HIP device commutation matrix entries");
+    std::unique_ptr<wg_comm_matrix> m(new wg_comm_matrix());
+    m->rows = rows;
+    m->cols = cols;
+    m->row_words = (cols + 63) / 64;
+    m->device = device;
+    DeviceScope scope(device);
+    const std::size_t words = checked_mul(rows, m->row_words, "HIP device commutation matrix");
+    m->bits = device_alloc<u64>(words);
+    if (words) check_hip(hipMemset(m->bits, 0, words * sizeof(u64)), "clear device buffer");
+    *out = m.release();
+  });
+}
+
+void wg_comm_matrix_destroy(wg_comm_matrix* m) { delete m; }
+
+int wg_comm_matrix_info(const wg_comm_matrix* m, size_t* rows, size_t* cols, size_t* row_words,
+                        int* device) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    if (rows) *rows = m->rows;
+    if (cols) *cols = m->cols;
+    if (row_words) *row_words = m->row_words;
+    if (device) *device = m->device;
+  });
+}
+
+int wg_comm_matrix_device_bits(const wg_comm_matrix* m, const uint64_t** bits) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    *bits = m->bits;
+  });
+}
+
+int wg_commutes_with_device(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, wg_comm_matrix* out,
+                            size_t max_entries, void* stream_ptr) {
+  return guard([&] {
+    require_op(lhs);
+    require_op(rhs);
+    if (out == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    require_same_device("HIP commutes_with_device", lhs, rhs);
+    if (out->device != lhs->device) {
+      fail_invalid(std::string("HIP commutes_with_device output requires operands on the same "
+                               "accelerator backend and same device (operation=HIP "
+                               "commutes_with_device output, left_backend=hip, left_device=") +
+                   std::to_string(lhs->device) + ", right_backend=hip, right_device=" +
+                   std::to_string(out->device) + ", failure_stage=pre_allocation)");
+    }
+    if (lhs->num_qubits != rhs->num_qubits) {
+      fail_invalid("PauliSum commutes_with requires the same num_qubits");
+    }
+    const std::size_t L = lhs->num_terms, R = rhs->num_terms;
+    check_entries(L, R, max_entries);
+    if (out->rows != L || out->cols != R) {
+      fail_invalid("HIP commutes_with_device output shape does not match operand term counts");
+    }
+    if (L * R == 0) return;
+    DeviceScope scope(lhs->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    launch_commute(lhs, rhs, out->bits, out->row_words, stream);
+    sync(stream);
+  });
+}
+
+int wg_comm_matrix_to_host_u8(const wg_comm_matrix* m, uint8_t* out, size_t out_len,
+                              void* stream_ptr) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    const std::size_t entries = m->rows * m->cols;
+    if (out_len != entries) fail_invalid("output buffer size does not match matrix entry count");
+    if (entries == 0) return;
+    DeviceScope scope(m->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    Scratch scratch(m->device);
+    std::uint8_t* dense = scratch.take<std::uint8_t>(entries);
+    unpack_u8_kernel<<<grid_for(entries, 256 * 8), 256, 0, stream>>>(m->bits, m->rows, m->cols,
+                                                                    m->row_words, dense);
+    check_launch("unpack_u8");
+    check_hip(hipMemcpyAsync(out, dense, entries, hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+  });
+}
+
+int wg_comm_matrix_to_host_packed(const wg_comm_matrix* m, uint64_t* out, size_t out_words,
+                                  void* stream_ptr) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    const std::size_t words = m->rows * m->row_words;
+    if (out_words != words) fail_invalid("output buffer size does not match packed word count");
+    if (words == 0) return;
+    DeviceScope scope(m->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    check_hip(hipMemcpyAsync(out, m->bits, words * sizeof(u64), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+  });
+}
+
+int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint64_t* out,
+                         size_t out_len, void* stream_ptr) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    if (axis != -1 && axis != 0 && axis != 1) fail_invalid("axis must be None, 0, or 1");
+    const std::size_t want = axis == -1 ? 1 : (axis == 1 ? m->rows : m->cols);
+    if (out_len != want) fail_invalid("count output length does not match the requested axis");
+    for (std::size_t i = 0; i < want; ++i) out[i] = 0;
+    if (m->rows == 0 || m->cols == 0) return;
+    DeviceScope scope(m->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    Scratch scratch(m->device);
+    unsigned long long* dev = scratch.take<unsigned long long>(want);
+    check_hip(hipMemsetAsync(dev, 0, want * sizeof(unsigned long long), stream), "clear counts");
+    if (axis == 0) {
+      const std::size_t col_blocks = (m->cols + 255) / 256;
+      const std::size_t chunks = (m->rows + kColRowChunk - 1) / kColRowChunk;
+      count_cols_kernel<<<grid_for(col_blocks * chunks, 1), 256, 0, stream>>>(
+          m->bits, m->rows, m->cols, m->row_words, col_blocks, dev);
+      check_launch("count_cols");
+    } else {
+      count_rows_kernel<<<grid_for(m->rows * 64, 256), 256, 0, stream>>>(
+          m->bits, m->rows, m->row_words, axis == 1 ? dev : nullptr, axis == -1 ? dev : nullptr);
+      check_launch("count_rows");
+    }
+    check_hip(hipMemcpyAsync(out, dev, want * sizeof(u64), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+    if (conflicts) {
+      // Anticommuting entries = line length - commuting entries.
+      const std::size_t line = axis == -1 ? m->rows * m->cols : (axis == 1 ? m->cols : m->rows);
+      for (std::size_t i = 0; i < want; ++i) out[i] = line - out[i];
+    }
+  });
+}
+
+}  // extern "C"

@@ -199,6 +199,20 @@ inline void require_op(const wg_pauli_sum* op) {
 // Same backend + same device check (reference src/accelerator_status.cpp:126-143).
 void require_same_device(const char* operation, const wg_pauli_sum* a, const wg_pauli_sum* b);
 
+// Event-pair timing of named kernel launches (enabled by wg_profile_enable).
+class ProfileScope {
+ public:
+  ProfileScope(const char* name, hipStream_t stream);
+  ~ProfileScope();
+  ProfileScope(const ProfileScope&) = delete;
+  ProfileScope& operator=(const ProfileScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t stream_;
+  hipEvent_t start_ = nullptr;
+};
+
 // Device-wide helpers (wg_runtime.hip).
 void exclusive_scan_u32(const u32* in, u32* out, std::size_t n, u32* total_dev, Scratch& scratch,
                         hipStream_t stream);

@@ -247,6 +247,63 @@ wg_comm_matrix::~wg_comm_matrix() {
 
 namespace wg {
 const char* last_error_cstr() { return t_last_error.c_str(); }
+
+namespace {
+struct PendingTiming {
+  std::string name;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+struct ProfileTotals {
+  double ms = 0.0;
+  std::uint64_t launches = 0;
+};
+std::mutex g_profile_mutex;
+bool g_profile_enabled = false;
+std::vector<PendingTiming>& pending_timings() {
+  static auto* v = new std::vector<PendingTiming>();
+  return *v;
+}
+std::map<std::string, ProfileTotals>& profile_totals() {
+  static auto* m = new std::map<std::string, ProfileTotals>();
+  return *m;
+}
+void resolve_pending_locked() {
+  for (auto& p : pending_timings()) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(p.stop) == hipSuccess && hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      auto& t = profile_totals()[p.name];
+      t.ms += ms;
+      t.launches += 1;
+    }
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  (void)hipGetLastError();
+  pending_timings().clear();
+}
+}  // namespace
+
+ProfileScope::ProfileScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
+  std::lock_guard<std::mutex> lk(g_profile_mutex);
+  if (!g_profile_enabled) return;
+  if (hipEventCreate(&start_) != hipSuccess || hipEventRecord(start_, stream_) != hipSuccess) {
+    (void)hipGetLastError();
+    start_ = nullptr;
+  }
+}
+
+ProfileScope::~ProfileScope() {
+  if (start_ == nullptr) return;
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream_) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(start_);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_profile_mutex);
+  pending_timings().push_back({name_, start_, stop});
+}
 }  // namespace wg
 
 using namespace wg;
@@ -371,5 +428,30 @@ int wg_pauli_sum_device_buffers(const wg_pauli_sum* op, const uint64_t** x, cons
 }
 
 void wg_pauli_sum_destroy(wg_pauli_sum* op) { delete op; }
+
+int wg_profile_enable(int enabled) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_profile_mutex);
+    g_profile_enabled = enabled != 0;
+  });
+}
+
+int wg_profile_reset(void) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_profile_mutex);
+    resolve_pending_locked();
+    profile_totals().clear();
+  });
+}
+
+int wg_profile_get(const char* kernel, double* total_ms, uint64_t* launches) {
+  return guard([&] {
+    std::lock_guard<std::mutex> lk(g_profile_mutex);
+    resolve_pending_locked();
+    auto it = profile_totals().find(kernel ? kernel : "");
+    *total_ms = it == profile_totals().end() ? 0.0 : it->second.ms;
+    *launches = it == profile_totals().end() ? 0 : it->second.launches;
+  });
+}
 
 }  // extern "C"

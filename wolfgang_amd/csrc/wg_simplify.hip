@@ -1,0 +1,492 @@
+// Sort-reduce simplify on the device.
+//
+// Semantics (reference src/simplify.cpp:147-430): terms with equal packed keys
+// (x, z) are summed in construction order, sums with |s| <= atol + rtol*max|c|
+// are dropped, survivors come out in canonical lexicographic order over
+// (x0, z0, x1, z1, ...) compared as unsigned words.
+//
+// Pipeline (all on one stream):
+//   1. key statistics: bitwise OR / AND of every key word over all terms, so
+//      that 8-bit digits that are constant across the operator are skipped;
+//   2. sort records: the key words (packed into one uint64 as x<<32|z when
+//      every mask fits in 32 bits) plus a uint32 construction index;
+//   3. LSD radix sort, 8-bit digits, least significant digit first. Each pass
+//      is histogram -> exclusive scan -> stable scatter. Stability within a
+//      4096-key tile comes from wave-level digit matching (ballots) and
+//      per-wave digit counts in LDS, so equal keys keep construction order;
+//   4. segment heads, one thread per segment summing its coefficients in
+//      sorted (= construction) order, tolerance test with hypot;
+//   5. survivor compaction by scan and a final scatter into a new operator.
+//
+// The accumulation seed follows the reference exactly: its hash path starts
+// from the first coefficient and its sort path from 0.0, which differ only for
+// a component whose every contribution is -0.0. The device sums from the first
+// coefficient and, when the reference would have taken the sort path, adds
+// +0.0 (which maps -0.0 to +0.0 and leaves every other value unchanged).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_set>
+#include <vector>
+
+#include "wg_internal.hpp"
+#include "wg_simplify.hpp"
+
+namespace wg {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr std::size_t kTile = static_cast<std::size_t>(kBlock) * kItems;
+
+// ---------------------------------------------------------------- key stats
+
+__global__ __launch_bounds__(256) void key_stats_kernel(const u64* __restrict__ x,
+                                                        const u64* __restrict__ z, std::size_t T,
+                                                        std::size_t W,
+                                                        unsigned long long* or_out,
+                                                        unsigned long long* and_out) {
+  // or_out / and_out hold 2W words: [x0, z0, x1, z1, ...].
+  for (std::size_t k = 0; k < 2 * W; ++k) {
+    const u64* src = (k & 1) ? z : x;
+    const std::size_t w = k >> 1;
+    u64 o = 0, a = ~0ULL;
+    for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+         t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+      const u64 v = src[t * W + w];
+      o |= v;
+      a &= v;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+      o |= __shfl_xor(o, d, 64);
+      a &= __shfl_xor(a, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicOr(&or_out[k], static_cast<unsigned long long>(o));
+      atomicAnd(&and_out[k], static_cast<unsigned long long>(a));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void max_abs_kernel(const double2* __restrict__ c,
+                                                      std::size_t T, double* block_max) {
+  __shared__ double lds[4];
+  double m = 0.0;
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    m = fmax(m, hypot(c[t].x, c[t].y));
+  }
+  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    block_max[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  }
+}
+
+// ---------------------------------------------------------------- records
+
+__global__ __launch_bounds__(256) void build_records_kernel(const u64* __restrict__ x,
+                                                            const u64* __restrict__ z,
+                                                            std::size_t T, std::size_t W, int KW,
+                                                            u64* __restrict__ keys,
+                                                            u32* __restrict__ idx) {
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (W == 0) {
+      keys[t] = 0;
+    } else if (KW == 1) {
+      keys[t] = (x[t] << 32) | z[t];
+    } else {
+      for (std::size_t w = 0; w < W; ++w) {
+        keys[t * KW + 2 * w] = x[t * W + w];
+        keys[t * KW + 2 * w + 1] = z[t * W + w];
+      }
+    }
+    idx[t] = static_cast<u32>(t);
+  }
+}
+
+// ---------------------------------------------------------------- radix pass
+
+__global__ __launch_bounds__(kBlock) void radix_hist_kernel(const u64* __restrict__ keys,
+                                                            std::size_t T, int KW, int word,
+                                                            int shift, std::size_t nblocks,
+                                                            u32* __restrict__ hist) {
+  __shared__ u32 counts[256];
+  counts[threadIdx.x] = 0;
+  __syncthreads();
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kTile;
+  for (int r = 0; r < kItems; ++r) {
+    const std::size_t i = base + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
+    if (i < T) {
+      const u32 d = static_cast<u32>(keys[i * KW + word] >> shift) & 0xFFU;
+      atomicAdd(&counts[d], 1U);
+    }
+  }
+  __syncthreads();
+  hist[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x] = counts[threadIdx.x];
+}
+
+__device__ inline u64 match_digit(u32 d) {
+  // Lanes of this wave holding the same 9-bit value d.
+  u64 peers = ~0ULL;
+#pragma unroll
+  for (int b = 0; b < 9; ++b) {
+    const bool bit = (d >> b) & 1U;
+    const u64 m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
+    const u64* __restrict__ keys_in, const u32* __restrict__ idx_in, u64* __restrict__ keys_out,
+    u32* __restrict__ idx_out, std::size_t T, int KW, int word, int shift, std::size_t nblocks,
+    const u32* __restrict__ offsets) {
+  __shared__ u32 base[256];
+  __shared__ u32 counts[kBlock / 64][256];
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned wave = threadIdx.x >> 6;
+  base[threadIdx.x] = offsets[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x];
+  for (int w = 0; w < kBlock / 64; ++w) counts[w][threadIdx.x] = 0;
+  __syncthreads();
+  const u64 lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  const std::size_t tile = static_cast<std::size_t>(blockIdx.x) * kTile;
+  for (int r = 0; r < kItems; ++r) {
+    const std::size_t i = tile + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
+    const bool valid = i < T;
+    const u32 d = valid ? (static_cast<u32>(keys_in[i * KW + word] >> shift) & 0xFFU) : 256U;
+    const u64 peers = match_digit(d);
+    const u32 rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) counts[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      u32 pos = base[d] + rank;
+      for (unsigned w = 0; w < wave; ++w) pos += counts[w][d];
+      for (int k = 0; k < KW; ++k) keys_out[static_cast<std::size_t>(pos) * KW + k] = keys_in[i * KW + k];
+      idx_out[pos] = idx_in[i];
+    }
+    __syncthreads();
+    u32 add = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      add += counts[w][threadIdx.x];
+      counts[w][threadIdx.x] = 0;
+    }
+    base[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- segments
+
+__device__ inline bool key_equal(const u64* keys, int KW, std::size_t a, std::size_t b) {
+  for (int k = 0; k < KW; ++k) {
+    if (keys[a * KW + k] != keys[b * KW + k]) return false;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void head_flags_kernel(const u64* __restrict__ keys,
+                                                         std::size_t T, int KW,
+                                                         u32* __restrict__ flags) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    flags[i] = (i == 0 || !key_equal(keys, KW, i, i - 1)) ? 1U : 0U;
+  }
+}
+
+__global__ __launch_bounds__(256) void segment_starts_kernel(const u32* __restrict__ flags,
+                                                             const u32* __restrict__ seg_pos,
+                                                             std::size_t T,
+                                                             u32* __restrict__ seg_start) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (flags[i]) seg_start[seg_pos[i]] = static_cast<u32>(i);
+    if (i == T - 1) seg_start[seg_pos[i] + flags[i]] = static_cast<u32>(T);
+  }
+}
+
+__global__ __launch_bounds__(256) void segment_sum_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx,
+    const double2* __restrict__ c, int zero_seed, double threshold, double2* __restrict__ sums,
+    u32* __restrict__ keep) {
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const u32 b = seg_start[s], e = seg_start[s + 1];
+    double2 acc = c[idx[b]];
+    for (u32 j = b + 1; j < e; ++j) {
+      const double2 v = c[idx[j]];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    if (zero_seed) {
+      acc.x += 0.0;
+      acc.y += 0.0;
+    }
+    sums[s] = acc;
+    keep[s] = hypot(acc.x, acc.y) > threshold ? 1U : 0U;
+  }
+}
+
+__global__ __launch_bounds__(256) void compact_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    const u32* __restrict__ out_pos, const u64* __restrict__ keys, int KW, std::size_t W,
+    const double2* __restrict__ sums, u64* __restrict__ ox, u64* __restrict__ oz,
+    double2* __restrict__ oc) {
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (!keep[s]) continue;
+    const std::size_t o = out_pos[s];
+    const std::size_t k = seg_start[s];
+    if (KW == 1) {
+      ox[o] = keys[k] >> 32;
+      oz[o] = keys[k] & 0xFFFFFFFFULL;
+    } else {
+      for (std::size_t w = 0; w < W; ++w) {
+        ox[o * W + w] = keys[k * KW + 2 * w];
+        oz[o * W + w] = keys[k * KW + 2 * w + 1];
+      }
+    }
+    oc[o] = sums[s];
+  }
+}
+
+unsigned stride_grid(std::size_t work) {
+  const std::size_t g = (work + 255) / 256;
+  return static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g, 65536)));
+}
+
+// Sampled duplicate rate of the first `sample` keys (reference
+// src/simplify.cpp:67-97).
+double duplicate_rate(const std::vector<u64>& hx, const std::vector<u64>& hz, std::size_t W,
+                      std::size_t sample) {
+  if (sample == 0) return 0.0;
+  struct H {
+    std::size_t W;
+    const std::vector<u64>* x;
+    const std::vector<u64>* z;
+    std::size_t operator()(std::size_t t) const {
+      std::size_t h = 0x9E3779B97F4A7C15ULL;
+      for (std::size_t w = 0; w < W; ++w) {
+        h ^= (*x)[t * W + w] + 0x632BE59BD9B4E019ULL + (h << 6) + (h >> 2);
+        h ^= (*z)[t * W + w] + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+      }
+      return h;
+    }
+  };
+  struct E {
+    std::size_t W;
+    const std::vector<u64>* x;
+    const std::vector<u64>* z;
+    bool operator()(std::size_t a, std::size_t b) const {
+      for (std::size_t w = 0; w < W; ++w) {
+        if ((*x)[a * W + w] != (*x)[b * W + w] || (*z)[a * W + w] != (*z)[b * W + w]) return false;
+      }
+      return true;
+    }
+  };
+  std::unordered_set<std::size_t, H, E> seen(sample, H{W, &hx, &hz}, E{W, &hx, &hz});
+  for (std::size_t t = 0; t < sample; ++t) seen.insert(t);
+  return 1.0 - static_cast<double>(seen.size()) / static_cast<double>(sample);
+}
+
+}  // namespace
+
+void copy_sample(const u64* x, const u64* z, std::size_t W, std::size_t count,
+                 std::vector<u64>& hx, std::vector<u64>& hz, hipStream_t stream) {
+  hx.resize(count * W);
+  hz.resize(count * W);
+  if (count * W == 0) return;
+  check_hip(hipMemcpyAsync(hx.data(), x, count * W * sizeof(u64), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  check_hip(hipMemcpyAsync(hz.data(), z, count * W * sizeof(u64), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+}
+
+bool simplify_takes_hash_path(const u64* x, const u64* z, std::size_t T, std::size_t W,
+                              hipStream_t stream) {
+  // Reference src/simplify.cpp:107-143.
+  if (W != 1 && W != 2) return false;
+  if (T < 4096) return false;
+  std::vector<u64> hx, hz;
+  copy_sample(x, z, W, 4096, hx, hz, stream);
+  const double first = duplicate_rate(hx, hz, W, 512);
+  if (first >= 0.25) return true;
+  if (first <= 0.05) return false;
+  return duplicate_rate(hx, hz, W, 4096) >= 0.25;
+}
+
+bool operand_duplicate_heavy(const wg_pauli_sum* op, hipStream_t stream) {
+  // Reference src/multiply.cpp:29-69: >= 128 terms and a duplicate rate of at
+  // least 0.25 among the first min(T, 512) terms.
+  if (op->num_terms < 128) return false;
+  const std::size_t sample = std::min<std::size_t>(op->num_terms, 512);
+  std::vector<u64> hx, hz;
+  copy_sample(op->x, op->z, op->words, sample, hx, hz, stream);
+  return duplicate_rate(hx, hz, op->words, sample) >= 0.25;
+}
+
+wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std::size_t n,
+                              std::size_t T, int device, double atol, double rtol, int seed_mode,
+                              Scratch& scratch, hipStream_t stream) {
+  const std::size_t W = words_for(n);
+  if (T == 0) return new_pauli_sum(n, 0, device);
+  if (T >= 0xFFFFFFFFULL) fail_invalid("simplify supports at most 2^32 - 2 terms per call");
+
+  // Threshold atol + rtol * max|c| (reference src/simplify.cpp:156-160).
+  const unsigned mgrid = std::min<unsigned>(stride_grid(T), 1024);
+  double* dmax = scratch.take<double>(mgrid);
+  max_abs_kernel<<<mgrid, 256, 0, stream>>>(c, T, dmax);
+  check_launch("max_abs");
+  std::vector<double> hmax(mgrid);
+  check_hip(hipMemcpyAsync(hmax.data(), dmax, mgrid * sizeof(double), hipMemcpyDeviceToHost,
+                           stream),
+            "copy device buffer to host");
+  sync(stream);
+  double max_abs = 0.0;
+  for (double v : hmax) max_abs = std::max(max_abs, v);
+  const double threshold = atol + rtol * max_abs;
+
+  bool hash_seed;
+  if (seed_mode < 0) {
+    hash_seed = simplify_takes_hash_path(x, z, T, W, stream);
+  } else {
+    hash_seed = seed_mode != 0;
+  }
+
+  // Key statistics and the digit plan.
+  std::vector<unsigned long long> ors(2 * W, 0ULL), ands(2 * W, ~0ULL);
+  if (W) {
+    unsigned long long* dstat = scratch.take<unsigned long long>(4 * W);
+    check_hip(hipMemcpyAsync(dstat, ors.data(), 2 * W * sizeof(u64), hipMemcpyHostToDevice, stream),
+              "copy host buffer to device");
+    check_hip(hipMemcpyAsync(dstat + 2 * W, ands.data(), 2 * W * sizeof(u64),
+                             hipMemcpyHostToDevice, stream),
+              "copy host buffer to device");
+    key_stats_kernel<<<std::min<unsigned>(stride_grid(T), 1024), 256, 0, stream>>>(
+        x, z, T, W, dstat, dstat + 2 * W);
+    check_launch("key_stats");
+    check_hip(hipMemcpyAsync(ors.data(), dstat, 2 * W * sizeof(u64), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    check_hip(hipMemcpyAsync(ands.data(), dstat + 2 * W, 2 * W * sizeof(u64),
+                             hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+  }
+  int KW;
+  std::vector<u64> kor, kand;
+  if (W == 0) {
+    KW = 1;
+    kor = {0};
+    kand = {0};
+  } else if (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0) {
+    KW = 1;
+    kor = {(ors[0] << 32) | ors[1]};
+    kand = {(ands[0] << 32) | ands[1]};
+  } else {
+    KW = static_cast<int>(2 * W);
+    kor.assign(ors.begin(), ors.end());
+    kand.assign(ands.begin(), ands.end());
+  }
+  struct Pass {
+    int word, shift;
+  };
+  std::vector<Pass> passes;
+  for (int k = KW - 1; k >= 0; --k) {
+    const u64 varying = kor[k] ^ kand[k];
+    for (int shift = 0; shift < 64; shift += 8) {
+      if ((varying >> shift) & 0xFFULL) passes.push_back({k, shift});
+    }
+  }
+
+  u64* keys_a = scratch.take<u64>(checked_mul(T, KW, "simplify keys"));
+  u64* keys_b = passes.empty() ? nullptr : scratch.take<u64>(T * KW);
+  u32* idx_a = scratch.take<u32>(T);
+  u32* idx_b = passes.empty() ? nullptr : scratch.take<u32>(T);
+  build_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, KW, keys_a, idx_a);
+  check_launch("build_records");
+
+  if (!passes.empty()) {
+    const std::size_t nblocks = (T + kTile - 1) / kTile;
+    u32* hist = scratch.take<u32>(256 * nblocks);
+    u32* offs = scratch.take<u32>(256 * nblocks);
+    for (const Pass& p : passes) {
+      radix_hist_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(keys_a, T, KW, p.word,
+                                                                     p.shift, nblocks, hist);
+      check_launch("radix_hist");
+      exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
+      ProfileScope prof("radix_scatter", stream);
+      radix_scatter_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, T, KW, p.word, p.shift, nblocks, offs);
+      check_launch("radix_scatter");
+      std::swap(keys_a, keys_b);
+      std::swap(idx_a, idx_b);
+    }
+  }
+
+  // Segments.
+  u32* flags = scratch.take<u32>(T);
+  u32* seg_pos = scratch.take<u32>(T);
+  u32* seg_start = scratch.take<u32>(T + 1);
+  u32* d_count = scratch.take<u32>(2);
+  head_flags_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, KW, flags);
+  check_launch("head_flags");
+  exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
+  segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
+  check_launch("segment_starts");
+  u32 S = 0;
+  check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+
+  double2* sums = scratch.take<double2>(S);
+  u32* keep = scratch.take<u32>(S);
+  u32* out_pos = scratch.take<u32>(S);
+  segment_sum_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, c,
+                                                         hash_seed ? 0 : 1, threshold, sums, keep);
+  check_launch("segment_sum");
+  exclusive_scan_u32(keep, out_pos, S, d_count + 1, scratch, stream);
+  u32 U = 0;
+  check_hip(hipMemcpyAsync(&U, d_count + 1, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+
+  std::unique_ptr<wg_pauli_sum> out(new_pauli_sum(n, U, device));
+  if (U) {
+    compact_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, keys_a, KW, W,
+                                                       sums, out->x, out->z, out->coeffs);
+    check_launch("compact");
+  }
+  sync(stream);
+  return out.release();
+}
+
+}  // namespace wg
+
+using namespace wg;
+
+extern "C" {
+
+int wg_simplify(const wg_pauli_sum* op, double atol, double rtol, void* stream_ptr,
+                wg_pauli_sum** out) {
+  return guard([&] {
+    *out = nullptr;
+    require_op(op);
+    // Reference src/hip/device_pauli_sum.hip.hpp:47-54.
+    if (!std::isfinite(atol) || !std::isfinite(rtol) || atol < 0.0 || rtol < 0.0) {
+      fail_invalid("simplify tolerances must be finite and non-negative");
+    }
+    DeviceScope scope(op->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    Scratch scratch(op->device);
+    *out = simplify_device(op->x, op->z, op->coeffs, op->num_qubits, op->num_terms, op->device,
+                           atol, rtol, -1, scratch, stream);
+  });
+}
+
+}  // extern "C"

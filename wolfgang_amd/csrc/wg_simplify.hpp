@@ -1,0 +1,21 @@
+// Device simplify pipeline shared by simplify and matmul.
+#pragma once
+
+#include <vector>
+
+#include "wg_internal.hpp"
+
+namespace wg {
+
+// Sort-reduce simplify of device buffers (x, z: T*W words; c: T coefficients)
+// into a new operator on `device`. seed_mode: -1 picks the reference's
+// simplify() accumulation seed from the data, 1 seeds with the first
+// coefficient (hash accumulation), 0 with 0.0 (sort path).
+wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std::size_t n,
+                              std::size_t T, int device, double atol, double rtol, int seed_mode,
+                              Scratch& scratch, hipStream_t stream);
+
+// matmul's duplicate-heavy operand test (reference src/multiply.cpp:29-69).
+bool operand_duplicate_heavy(const wg_pauli_sum* op, hipStream_t stream);
+
+}  // namespace wg

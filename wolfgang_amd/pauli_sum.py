@@ -1,0 +1,809 @@
+"""Host and device Pauli-sum types mirroring the reference's Python surface.
+
+``PauliSum`` is the host container (packed x/z uint64 words plus complex128
+coefficients, reference include/wolfgang/pauli_sum.hpp:13-139). Its label I/O
+and concatenation run on the host; its hot-path operations (``simplify``,
+``@``/``matmul``, ``commutes_with``, ``expectation_statevector``) run on the
+GPU through the C ABI, copying operands to device 0 and results back.
+
+``DevicePauliSum`` and ``DeviceCommutationMatrix`` are the device-resident
+mirrors (reference include/wolfgang/device_pauli_sum.hpp:93-175,
+include/wolfgang/device_commutation_matrix.hpp:16-62); they own handles of the
+native library and keep data in HBM between calls.
+
+Error types and messages follow the reference bindings
+(bindings/python/pauli_sum_py.cpp): ValueError for invalid input, TypeError for
+dtype/layout, RuntimeError for runtime failures.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import numbers
+from typing import Any, Iterable
+
+import numpy as np
+
+from . import _native
+
+_U64 = np.uint64
+_BIT_WEIGHTS = np.left_shift(np.uint64(1), np.arange(64, dtype=np.uint64))
+
+DEFAULT_MAX_INTERMEDIATE_TERMS = 50_000_000
+DEFAULT_MAX_COMMUTATION_ENTRIES = 100_000_000
+
+_NOT_BUILT = {
+    "cuda": "Wolfgang was built without CUDA support; rebuild from source with "
+    "WOLFGANG_ENABLE_CUDA=ON to use PauliSum.to_device().",
+    "metal": "Wolfgang was built without Metal support; rebuild from source on Apple Silicon with "
+    "WOLFGANG_ENABLE_METAL=ON to use PauliSum.to_device().",
+}
+
+
+def _words(num_qubits: int) -> int:
+    return 0 if num_qubits == 0 else (num_qubits + 63) // 64
+
+
+def _checked_size(value: Any, name: str) -> int:
+    # Reference bindings/python/pauli_sum_py.cpp:105-118.
+    if isinstance(value, bool) or not isinstance(value, numbers.Integral):
+        raise ValueError(f"{name} must be a non-negative integer")
+    value = int(value)
+    if value < 0 or value >= 1 << 64:
+        raise ValueError(f"{name} must be a non-negative integer that fits size_t")
+    return value
+
+
+def _checked_signed(value: Any, name: str) -> int:
+    if isinstance(value, bool) or not isinstance(value, numbers.Integral) or int(value) < 0:
+        raise ValueError(f"{name} must be a non-negative integer")
+    return int(value)
+
+
+def _complex(value: Any, name: str) -> complex:
+    if isinstance(value, (bool, np.bool_)) or not isinstance(value, numbers.Number):
+        raise ValueError(f"{name} must be numeric")
+    return complex(value)
+
+
+def _parse_backend(backend: Any) -> None:
+    # Reference src/accelerator_status.cpp:25-46,66-91 for a HIP-only build.
+    if backend is None:
+        return
+    if not isinstance(backend, str):
+        raise ValueError("backend must be None, 'auto', 'cuda', 'hip', or 'metal'")
+    name = backend.lower()
+    if name in ("auto", "hip"):
+        return
+    if name in _NOT_BUILT:
+        raise RuntimeError(_NOT_BUILT[name])
+    raise ValueError("backend must be None, 'auto', 'cuda', 'hip', or 'metal'")
+
+
+def checked_matmul_intermediate_terms(lhs: int, rhs: int, max_terms: int) -> int:
+    """Reference src/detail/checked_arithmetic.hpp:34-47."""
+    if lhs != 0 and rhs > ((1 << 64) - 1) // lhs:
+        raise ValueError("matmul intermediate term count overflows size_t")
+    if lhs * rhs > max_terms:
+        raise ValueError("matmul intermediate term count exceeds max_intermediate_terms")
+    return lhs * rhs
+
+
+def checked_commutation_entries(lhs: int, rhs: int, max_entries: int) -> int:
+    """Reference src/detail/checked_arithmetic.hpp:49-63."""
+    if lhs != 0 and rhs > ((1 << 64) - 1) // lhs:
+        raise ValueError("commutation matrix entry count overflows size_t")
+    if lhs * rhs > max_entries:
+        raise ValueError("commutation matrix entry count exceeds max_commutation_matrix_entries")
+    return lhs * rhs
+
+
+def _ptr(arr: np.ndarray | None) -> int | None:
+    if arr is None or arr.size == 0:
+        return None
+    return arr.ctypes.data
+
+
+class PauliSum:
+    """Sparse sum of Pauli strings backed by packed x/z masks and complex128 coefficients."""
+
+    __slots__ = ("_n", "_x", "_z", "_c")
+
+    def __init__(self, num_qubits: int, num_terms: int = 0) -> None:
+        # Metadata constructor: num_terms identity terms with zero coefficients
+        # (reference src/pauli_sum.cpp:14-20).
+        n = _checked_signed(num_qubits, "num_qubits")
+        t = _checked_signed(num_terms, "num_terms")
+        self._n = n
+        self._x = np.zeros((t, _words(n)), dtype=_U64)
+        self._z = np.zeros((t, _words(n)), dtype=_U64)
+        self._c = np.zeros(t, dtype=np.complex128)
+
+    # ----------------------------------------------------------- construction
+    @classmethod
+    def _from_arrays(cls, n: int, x: np.ndarray, z: np.ndarray, c: np.ndarray) -> "PauliSum":
+        obj = cls.__new__(cls)
+        obj._n = int(n)
+        w = _words(n)
+        t = int(c.shape[0])
+        obj._x = np.ascontiguousarray(x, dtype=_U64).reshape(t, w)
+        obj._z = np.ascontiguousarray(z, dtype=_U64).reshape(t, w)
+        obj._c = np.ascontiguousarray(c, dtype=np.complex128).reshape(t)
+        return obj
+
+    @classmethod
+    def from_packed(cls, num_qubits: int, x: np.ndarray, z: np.ndarray, coeffs: np.ndarray) -> "PauliSum":
+        """Build from packed words: x, z of shape (T, ceil(n/64)) uint64; coeffs (T,)."""
+        n = _checked_signed(num_qubits, "num_qubits")
+        c = np.asarray(coeffs, dtype=np.complex128).reshape(-1)
+        x = np.asarray(x, dtype=_U64).reshape(c.shape[0], _words(n))
+        z = np.asarray(z, dtype=_U64).reshape(c.shape[0], _words(n))
+        if n % 64:
+            high = ~np.uint64((1 << (n % 64)) - 1)
+            if (x[:, -1] & high).any() or (z[:, -1] & high).any():
+                raise ValueError("packed words have bits set above num_qubits")
+        return cls._from_arrays(n, x, z, c)
+
+    @staticmethod
+    def empty(num_qubits: int) -> "PauliSum":
+        return PauliSum(_checked_signed(num_qubits, "num_qubits"), 0)
+
+    @staticmethod
+    def from_labels(labels: Iterable[str], coeffs: Any = None) -> "PauliSum":
+        """Dense labels in Qiskit display order (right-most character is qubit 0).
+
+        Reference src/parse.cpp:17-54 and bindings/python/pauli_sum_py.cpp:125-185.
+        """
+        if isinstance(labels, str):
+            raise ValueError("labels must be an iterable of dense label strings, not a single string")
+        try:
+            labels = list(labels)
+        except TypeError as exc:
+            raise ValueError("labels must be an iterable of strings") from exc
+        if not all(isinstance(label, str) for label in labels):
+            raise ValueError("labels must be an iterable of strings")
+        t = len(labels)
+        if coeffs is None:
+            c = np.ones(t, dtype=np.complex128)
+        elif isinstance(coeffs, numbers.Number) and not isinstance(coeffs, (bool, np.bool_)):
+            if t != 1:
+                raise ValueError("scalar coeffs are valid only for a single label")
+            c = np.array([complex(coeffs)], dtype=np.complex128)
+        else:
+            try:
+                values = [_complex(v, "coefficients") for v in coeffs]
+            except TypeError as exc:
+                raise ValueError(
+                    "coeffs must be None, a numeric scalar for one label, or an iterable of numeric values"
+                ) from exc
+            c = np.array(values, dtype=np.complex128).reshape(-1)
+            if c.shape[0] != t:
+                raise ValueError("coefficient count must match label count")
+        if t == 0:
+            raise ValueError(
+                "PauliSum.from_labels([]) cannot infer num_qubits; use PauliSum.empty(num_qubits)"
+            )
+        n = len(labels[0])
+        if any(len(label) != n for label in labels):
+            raise ValueError("all dense labels must have the same length")
+        try:
+            raw = np.frombuffer("".join(labels).encode("ascii"), dtype=np.uint8).reshape(t, n)
+        except UnicodeEncodeError as exc:
+            raise ValueError("dense labels may contain only I, X, Y, or Z") from exc
+        valid = np.isin(raw, np.frombuffer(b"IXYZ", dtype=np.uint8))
+        if not valid.all():
+            raise ValueError("dense labels may contain only I, X, Y, or Z")
+        # Column k of the label is qubit n-1-k.
+        xb = ((raw == ord("X")) | (raw == ord("Y")))[:, ::-1]
+        zb = ((raw == ord("Z")) | (raw == ord("Y")))[:, ::-1]
+        return PauliSum._from_arrays(n, _pack_bits(xb, n), _pack_bits(zb, n), c)
+
+    @staticmethod
+    def from_sparse_list(triples: Iterable[Any], num_qubits: int) -> "PauliSum":
+        """Sparse (local_string, qubit_indices, coefficient) triples (reference src/parse.cpp:56-97)."""
+        n = _checked_signed(num_qubits, "num_qubits")
+        w = _words(n)
+        triples = list(triples)
+        x = np.zeros((len(triples), w), dtype=_U64)
+        z = np.zeros((len(triples), w), dtype=_U64)
+        c = np.zeros(len(triples), dtype=np.complex128)
+        for t, triple in enumerate(triples):
+            try:
+                local, indices, coeff = triple
+            except (TypeError, ValueError) as exc:
+                raise ValueError("sparse terms must be (local_pauli_string, qubit_indices, coefficient) triples") from exc
+            indices = [int(q) for q in indices]
+            if len(local) != len(indices):
+                raise ValueError("sparse local_pauli_string length must match qubit_indices length")
+            seen: set[int] = set()
+            for pauli, q in zip(local, indices):
+                if pauli not in "IXYZ":
+                    raise ValueError("sparse local_pauli_string may contain only I, X, Y, or Z")
+                if q < 0 or q >= n:
+                    raise ValueError("sparse qubit index is out of range")
+                if q in seen:
+                    raise ValueError("sparse term contains a duplicate qubit index")
+                seen.add(q)
+                bit = np.uint64(1) << np.uint64(q % 64)
+                if pauli in "XY":
+                    x[t, q // 64] |= bit
+                if pauli in "ZY":
+                    z[t, q // 64] |= bit
+            c[t] = _complex(coeff, "coefficient")
+        return PauliSum._from_arrays(n, x, z, c)
+
+    # ----------------------------------------------------------- accessors
+    @property
+    def num_qubits(self) -> int:
+        return self._n
+
+    @property
+    def num_terms(self) -> int:
+        return int(self._c.shape[0])
+
+    @property
+    def words(self) -> int:
+        return _words(self._n)
+
+    def x_words(self) -> np.ndarray:
+        """Packed x words, shape (num_terms, words), uint64 (a copy)."""
+        return self._x.copy()
+
+    def z_words(self) -> np.ndarray:
+        """Packed z words, shape (num_terms, words), uint64 (a copy)."""
+        return self._z.copy()
+
+    def coeffs(self) -> np.ndarray:
+        return self._c.copy()
+
+    def to_labels(self) -> tuple[list[str], np.ndarray]:
+        """Dense labels in construction order and complex128 coefficients (reference src/export.cpp:13-30)."""
+        t, n = self.num_terms, self._n
+        if t == 0:
+            return [], self._c.copy()
+        xb = _unpack_bits(self._x, n)[:, ::-1]
+        zb = _unpack_bits(self._z, n)[:, ::-1]
+        table = np.frombuffer(b"IXZY", dtype=np.uint8)
+        chars = table[xb.astype(np.uint8) + 2 * zb.astype(np.uint8)]
+        raw = chars.tobytes()
+        labels = [raw[i * n : (i + 1) * n].decode("ascii") for i in range(t)]
+        return labels, self._c.copy()
+
+    def to_sparse_list(self) -> list[tuple[str, list[int], complex]]:
+        """Sparse triples, qubit indices ascending within each term (reference src/export.cpp:32-54)."""
+        out = []
+        xb = _unpack_bits(self._x, self._n)
+        zb = _unpack_bits(self._z, self._n)
+        for t in range(self.num_terms):
+            qubits = np.nonzero(xb[t] | zb[t])[0]
+            local = "".join("IXZY"[int(xb[t, q]) + 2 * int(zb[t, q])] for q in qubits)
+            out.append((local, [int(q) for q in qubits], complex(self._c[t])))
+        return out
+
+    def __repr__(self) -> str:
+        return f"PauliSum(num_qubits={self._n}, num_terms={self.num_terms})"
+
+    # ----------------------------------------------------------- host ops
+    def __add__(self, rhs: "PauliSum") -> "PauliSum":
+        # Reference src/arithmetic.cpp:14-41: concatenation, no simplify.
+        if not isinstance(rhs, PauliSum):
+            return NotImplemented
+        if self._n != rhs._n:
+            raise ValueError("PauliSum addition requires the same num_qubits")
+        return PauliSum._from_arrays(
+            self._n,
+            np.concatenate([self._x, rhs._x]),
+            np.concatenate([self._z, rhs._z]),
+            np.concatenate([self._c, rhs._c]),
+        )
+
+    def __mul__(self, scalar: Any) -> "PauliSum":
+        # Reference src/arithmetic.cpp:43-57.
+        s = _complex(scalar, "scalar")
+        return PauliSum._from_arrays(self._n, self._x, self._z, self._c * s)
+
+    __rmul__ = __mul__
+
+    # ----------------------------------------------------------- device ops
+    def to_device(self, device: int = 0, backend: Any = None) -> "DevicePauliSum":
+        _parse_backend(backend)
+        if isinstance(device, bool) or not isinstance(device, numbers.Integral):
+            raise ValueError("device must be an integer")
+        return DevicePauliSum._create(self, int(device))
+
+    def simplify(self, atol: float = 1.0e-12, rtol: float = 0.0) -> "PauliSum":
+        # Reference src/simplify.cpp:148-150 message.
+        atol, rtol = float(atol), float(rtol)
+        if not (np.isfinite(atol) and np.isfinite(rtol)) or atol < 0.0 or rtol < 0.0:
+            raise ValueError("simplify tolerances must be non-negative finite values")
+        return self.to_device().simplify(atol, rtol).to_host()
+
+    def matmul(
+        self,
+        rhs: "PauliSum",
+        simplify: bool = True,
+        max_intermediate_terms: int = DEFAULT_MAX_INTERMEDIATE_TERMS,
+    ) -> "PauliSum":
+        max_terms = _checked_size(max_intermediate_terms, "max_intermediate_terms")
+        if self._n != rhs._n:
+            raise ValueError("PauliSum matmul requires the same num_qubits")
+        checked_matmul_intermediate_terms(self.num_terms, rhs.num_terms, max_terms)
+        return self.to_device().matmul(rhs.to_device(), bool(simplify), max_terms).to_host()
+
+    def __matmul__(self, rhs: "PauliSum") -> "PauliSum":
+        if not isinstance(rhs, PauliSum):
+            return NotImplemented
+        return self.matmul(rhs)
+
+    def commutes_with(
+        self, other: "PauliSum", max_commutation_matrix_entries: int = DEFAULT_MAX_COMMUTATION_ENTRIES
+    ) -> Any:
+        """bool for 1x1, 1-D bool array when one side has one term, else 2-D (reference
+        bindings/python/pauli_sum_py.cpp:1548-1577)."""
+        max_entries = _checked_size(max_commutation_matrix_entries, "max_commutation_matrix_entries")
+        if self._n != other._n:
+            raise ValueError("PauliSum commutation requires the same num_qubits")
+        checked_commutation_entries(self.num_terms, other.num_terms, max_entries)
+        return self.to_device().commutes_with(other.to_device(), max_entries)
+
+    def expectation_statevector(self, psi: Any) -> complex:
+        return self.to_device().expectation_statevector(psi)
+
+    @staticmethod
+    def _checked_matmul_size_for_testing(
+        lhs_terms: int, rhs_terms: int, max_intermediate_terms: int
+    ) -> int:
+        return checked_matmul_intermediate_terms(
+            _checked_size(lhs_terms, "lhs_terms"),
+            _checked_size(rhs_terms, "rhs_terms"),
+            _checked_size(max_intermediate_terms, "max_intermediate_terms"),
+        )
+
+    @staticmethod
+    def _checked_commutation_size_for_testing(
+        lhs_terms: int, rhs_terms: int, max_commutation_matrix_entries: int
+    ) -> int:
+        return checked_commutation_entries(
+            _checked_size(lhs_terms, "lhs_terms"),
+            _checked_size(rhs_terms, "rhs_terms"),
+            _checked_size(max_commutation_matrix_entries, "max_commutation_matrix_entries"),
+        )
+
+
+def _pack_bits(bits: np.ndarray, n: int) -> np.ndarray:
+    """bits (T, n) bool, column q = qubit q -> (T, words) uint64."""
+    t = bits.shape[0]
+    w = _words(n)
+    out = np.zeros((t, w), dtype=_U64)
+    for word in range(w):
+        chunk = bits[:, 64 * word : 64 * word + 64].astype(_U64)
+        out[:, word] = (chunk * _BIT_WEIGHTS[: chunk.shape[1]]).sum(axis=1, dtype=_U64)
+    return out
+
+
+def _unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
+    """(T, words) uint64 -> (T, n) bool with column q = qubit q."""
+    t = words.shape[0]
+    out = np.zeros((t, n), dtype=bool)
+    for q in range(n):
+        out[:, q] = (words[:, q // 64] >> np.uint64(q % 64)) & np.uint64(1)
+    return out
+
+
+# =====================================================================
+# Device-resident types
+# =====================================================================
+
+
+def _stream_arg(stream: Any) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(getattr(stream, "cuda_stream", stream))
+
+
+class DevicePauliSum:
+    """Owning device mirror of a PauliSum. Construct with PauliSum.to_device()."""
+
+    __slots__ = ("_h", "_n", "_t", "_dev", "__weakref__")
+
+    def __init__(self) -> None:
+        raise TypeError("DevicePauliSum has no public constructor; use PauliSum.to_device()")
+
+    @classmethod
+    def _wrap(cls, handle: int) -> "DevicePauliSum":
+        obj = cls.__new__(cls)
+        obj._h = handle
+        n, t, w, dev = _size_t(), _size_t(), _size_t(), C.c_int()
+        _native.call("wg_pauli_sum_info", handle, C.byref(n), C.byref(t), C.byref(w), C.byref(dev))
+        obj._n, obj._t, obj._dev = n.value, t.value, dev.value
+        return obj
+
+    @classmethod
+    def _create(cls, host: PauliSum, device: int) -> "DevicePauliSum":
+        out = C.c_void_p()
+        coeffs = np.ascontiguousarray(host._c, dtype=np.complex128)
+        _native.call(
+            "wg_pauli_sum_create",
+            host._n,
+            host.num_terms,
+            _ptr(host._x),
+            _ptr(host._z),
+            _ptr(coeffs),
+            device,
+            C.byref(out),
+        )
+        return cls._wrap(out.value)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _native.load().wg_pauli_sum_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _handle(self) -> int:
+        if not getattr(self, "_h", None):
+            raise RuntimeError("DevicePauliSum is empty or moved-from")
+        return self._h
+
+    @property
+    def num_qubits(self) -> int:
+        return self._n
+
+    @property
+    def num_terms(self) -> int:
+        return self._t
+
+    @property
+    def words(self) -> int:
+        return _words(self._n)
+
+    @property
+    def device(self) -> int:
+        return self._dev
+
+    @property
+    def backend(self) -> str:
+        return "hip"
+
+    def device_buffers(self) -> tuple[int, int, int]:
+        """Device addresses of the packed x, z words and the complex128 coefficients."""
+        x, z, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _native.call("wg_pauli_sum_device_buffers", self._handle(), C.byref(x), C.byref(z), C.byref(c))
+        return x.value or 0, z.value or 0, c.value or 0
+
+    def to_host(self) -> PauliSum:
+        t, w = self._t, _words(self._n)
+        x = np.zeros((t, w), dtype=_U64)
+        z = np.zeros((t, w), dtype=_U64)
+        c = np.zeros(t, dtype=np.complex128)
+        _native.call("wg_pauli_sum_to_host", self._handle(), _ptr(x), _ptr(z), _ptr(c))
+        return PauliSum._from_arrays(self._n, x, z, c)
+
+    def simplify(self, atol: float = 1.0e-12, rtol: float = 0.0, stream: Any = None) -> "DevicePauliSum":
+        out = C.c_void_p()
+        _native.call("wg_simplify", self._handle(), float(atol), float(rtol), _stream_arg(stream), C.byref(out))
+        return DevicePauliSum._wrap(out.value)
+
+    def matmul(
+        self,
+        rhs: "DevicePauliSum",
+        simplify: bool = True,
+        max_intermediate_terms: int = DEFAULT_MAX_INTERMEDIATE_TERMS,
+        stream: Any = None,
+    ) -> "DevicePauliSum":
+        max_terms = _checked_size(max_intermediate_terms, "max_intermediate_terms")
+        out = C.c_void_p()
+        _native.call(
+            "wg_matmul", self._handle(), rhs._handle(), int(bool(simplify)), max_terms, _stream_arg(stream), C.byref(out)
+        )
+        return DevicePauliSum._wrap(out.value)
+
+    def __matmul__(self, rhs: "DevicePauliSum") -> "DevicePauliSum":
+        if not isinstance(rhs, DevicePauliSum):
+            return NotImplemented
+        return self.matmul(rhs)
+
+    def commutes_with_into(
+        self,
+        other: "DevicePauliSum",
+        output: np.ndarray,
+        max_commutation_matrix_entries: int = DEFAULT_MAX_COMMUTATION_ENTRIES,
+    ) -> None:
+        max_entries = _checked_size(max_commutation_matrix_entries, "max_commutation_matrix_entries")
+        entries = checked_commutation_entries(self._t, other._t, max_entries)
+        if not isinstance(output, np.ndarray) or output.dtype not in (np.bool_, np.uint8):
+            raise TypeError("commutes_with_into output must be a NumPy bool or uint8 array")
+        if not output.flags.c_contiguous or not output.flags.writeable:
+            raise TypeError("commutes_with_into output must be a writable C-contiguous array")
+        if output.size != entries:
+            raise ValueError("HIP commutes_with output buffer size does not match entry count")
+        _native.call(
+            "wg_commutes_with_host", self._handle(), other._handle(), _ptr(output), entries, max_entries, None
+        )
+
+    def commutes_with(
+        self, other: "DevicePauliSum", max_commutation_matrix_entries: int = DEFAULT_MAX_COMMUTATION_ENTRIES
+    ) -> Any:
+        max_entries = _checked_size(max_commutation_matrix_entries, "max_commutation_matrix_entries")
+        entries = checked_commutation_entries(self._t, other._t, max_entries)
+        flags = np.zeros(entries, dtype=np.bool_)
+        _native.call(
+            "wg_commutes_with_host", self._handle(), other._handle(), _ptr(flags), entries, max_entries, None
+        )
+        if self._t == 1 and other._t == 1:
+            return bool(flags[0])
+        if self._t == 1 or other._t == 1:
+            return flags
+        return flags.reshape(self._t, other._t)
+
+    def commutes_with_device(
+        self,
+        other: "DevicePauliSum",
+        max_commutation_matrix_entries: int = DEFAULT_MAX_COMMUTATION_ENTRIES,
+        output: "DeviceCommutationMatrix | None" = None,
+    ) -> "DeviceCommutationMatrix":
+        max_entries = _checked_size(max_commutation_matrix_entries, "max_commutation_matrix_entries")
+        if output is None:
+            checked_commutation_entries(self._t, other._t, max_entries)
+            output = DeviceCommutationMatrix.empty((self._t, other._t), device=self._dev)
+        elif not isinstance(output, DeviceCommutationMatrix):
+            raise TypeError("commutes_with_device output must be a DeviceCommutationMatrix")
+        _native.call(
+            "wg_commutes_with_device", self._handle(), other._handle(), output._handle(), max_entries, None
+        )
+        return output
+
+    def expectation_statevector(self, psi: Any, stream: Any = None) -> complex:
+        """<psi|H|psi> for a host NumPy array or a device-resident torch tensor /
+        ``__cuda_array_interface__`` object on this operator's device."""
+        ptr, dtype, length, on_device, stream_ptr = _statevector_arg(psi, self._dev, stream)
+        out = (C.c_double * 2)()
+        _native.call("wg_expectation", self._handle(), ptr, dtype, length, on_device, stream_ptr, out)
+        return complex(out[0], out[1])
+
+    def expectation_adjoint(
+        self, psi_ptr: int, length: int, term_values_ptr: int | None, sigma_ptr: int | None,
+        sigma_dagger_ptr: int | None, stream: Any = None,
+    ) -> complex:
+        """Raw device-pointer forward + adjoint (see include/wolfgang_amd.h)."""
+        out = (C.c_double * 2)()
+        _native.call(
+            "wg_expectation_adjoint", self._handle(), psi_ptr, length, term_values_ptr, sigma_ptr,
+            sigma_dagger_ptr, _stream_arg(stream), out,
+        )
+        return complex(out[0], out[1])
+
+    def __repr__(self) -> str:
+        return f"DevicePauliSum(num_qubits={self._n}, num_terms={self._t}, device={self._dev}, backend='hip')"
+
+
+def _size_t() -> C.c_size_t:
+    return C.c_size_t()
+
+
+def _statevector_arg(psi: Any, device: int, stream: Any):
+    # Device-resident torch tensors (the interop the reference HIP path lacks).
+    mod = type(psi).__module__
+    if mod.startswith("torch"):
+        import torch
+
+        if psi.is_cuda:
+            if psi.dim() != 1:
+                raise ValueError("psi must be 1-dimensional")
+            if psi.dtype == torch.complex128:
+                dtype = _native.WG_DTYPE_COMPLEX128
+            elif psi.dtype == torch.complex64:
+                dtype = _native.WG_DTYPE_COMPLEX64
+            else:
+                raise TypeError("psi dtype must be complex64 or complex128")
+            if not psi.is_contiguous():
+                raise TypeError("psi must be C-contiguous")
+            if psi.device.index != device:
+                raise ValueError(
+                    "expectation_statevector requires psi on the operator's device "
+                    f"(operator device={device}, psi device={psi.device.index})"
+                )
+            s = stream if stream is not None else torch.cuda.current_stream(psi.device)
+            return psi.data_ptr() or None, dtype, int(psi.numel()), 1, _stream_arg(s)
+        psi = psi.numpy()
+    if hasattr(psi, "__cuda_array_interface__") and not isinstance(psi, np.ndarray):
+        iface = psi.__cuda_array_interface__
+        shape = tuple(iface["shape"])
+        if len(shape) != 1:
+            raise ValueError("psi must be 1-dimensional")
+        typestr = iface["typestr"]
+        if typestr.endswith("c16"):
+            dtype = _native.WG_DTYPE_COMPLEX128
+        elif typestr.endswith("c8"):
+            dtype = _native.WG_DTYPE_COMPLEX64
+        else:
+            raise TypeError("psi dtype must be complex64 or complex128")
+        if iface.get("strides") not in (None, (16 if dtype else 8,)):
+            raise TypeError("psi must be C-contiguous")
+        return int(iface["data"][0]) or None, dtype, int(shape[0]), 1, _stream_arg(stream)
+    try:
+        arr = np.asarray(psi) if not isinstance(psi, np.ndarray) else psi
+    except Exception as exc:  # pragma: no cover - defensive
+        raise TypeError("psi must be a NumPy array or a device tensor") from exc
+    if arr.ndim != 1:
+        raise ValueError("psi must be 1-dimensional")
+    if arr.dtype == np.complex128:
+        dtype = _native.WG_DTYPE_COMPLEX128
+    elif arr.dtype == np.complex64:
+        dtype = _native.WG_DTYPE_COMPLEX64
+    else:
+        raise TypeError("psi dtype must be complex64 or complex128")
+    if not arr.flags.c_contiguous:
+        raise TypeError("psi must be C-contiguous")
+    return (arr.ctypes.data if arr.size else None), dtype, int(arr.shape[0]), 0, _stream_arg(stream)
+
+
+class DeviceCommutationMatrix:
+    """Owning device-resident commutation matrix, stored as packed bits
+    (row i: ceil(cols/64) uint64 words, bit j%64 of word j/64 = entry (i, j))."""
+
+    __slots__ = ("_h", "_rows", "_cols", "_row_words", "_dev", "__weakref__")
+
+    def __init__(self) -> None:
+        raise TypeError("use DeviceCommutationMatrix.empty(shape) or DevicePauliSum.commutes_with_device")
+
+    @staticmethod
+    def empty(shape: Any, device: int = 0, backend: Any = None) -> "DeviceCommutationMatrix":
+        _parse_backend(backend)
+        try:
+            rows, cols = shape
+        except (TypeError, ValueError) as exc:
+            raise ValueError("shape must be a (rows, cols) pair") from exc
+        rows = _checked_size(rows, "shape[0]")
+        cols = _checked_size(cols, "shape[1]")
+        out = C.c_void_p()
+        _native.call("wg_comm_matrix_create", rows, cols, int(device), C.byref(out))
+        obj = DeviceCommutationMatrix.__new__(DeviceCommutationMatrix)
+        obj._h = out.value
+        r, c, rw, dev = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_int()
+        _native.call("wg_comm_matrix_info", obj._h, C.byref(r), C.byref(c), C.byref(rw), C.byref(dev))
+        obj._rows, obj._cols, obj._row_words, obj._dev = r.value, c.value, rw.value, dev.value
+        return obj
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _native.load().wg_comm_matrix_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _handle(self) -> int:
+        if not getattr(self, "_h", None):
+            raise RuntimeError("DeviceCommutationMatrix is empty or moved-from")
+        return self._h
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self._rows, self._cols)
+
+    @property
+    def device(self) -> int:
+        return self._dev
+
+    @property
+    def backend(self) -> str:
+        return "hip"
+
+    @property
+    def dtype(self) -> str:
+        # Dtype of the dense view returned by to_host(); storage is packed bits.
+        return "uint8"
+
+    @property
+    def storage(self) -> str:
+        return "packed_bits"
+
+    @property
+    def num_entries(self) -> int:
+        return self._rows * self._cols
+
+    @property
+    def row_words(self) -> int:
+        return self._row_words
+
+    def device_bits(self) -> int:
+        p = C.c_void_p()
+        _native.call("wg_comm_matrix_device_bits", self._handle(), C.byref(p))
+        return p.value or 0
+
+    def to_host(self) -> np.ndarray:
+        out = np.zeros((self._rows, self._cols), dtype=np.bool_)
+        _native.call("wg_comm_matrix_to_host_u8", self._handle(), _ptr(out), out.size, None)
+        return out
+
+    def to_host_packed(self) -> np.ndarray:
+        out = np.zeros((self._rows, self._row_words), dtype=_U64)
+        _native.call("wg_comm_matrix_to_host_packed", self._handle(), _ptr(out), out.size, None)
+        return out
+
+    def _count(self, axis: Any, conflicts: int) -> Any:
+        if axis is None:
+            out = np.zeros(1, dtype=_U64)
+            _native.call("wg_comm_matrix_count", self._handle(), -1, conflicts, _ptr(out), 1, None)
+            return int(out[0])
+        if isinstance(axis, bool) or axis not in (0, 1):
+            raise ValueError("axis must be None, 0, or 1")
+        n = self._cols if axis == 0 else self._rows
+        out = np.zeros(n, dtype=_U64)
+        _native.call("wg_comm_matrix_count", self._handle(), int(axis), conflicts, _ptr(out), n, None)
+        return out
+
+    def count_commuting(self, axis: Any = None) -> Any:
+        """None: total (int); 0: per-column counts; 1: per-row counts (uint64)."""
+        return self._count(axis, 0)
+
+    def conflict_degrees(self, axis: Any = None) -> Any:
+        """Anticommuting counts with the same axis convention as count_commuting."""
+        return self._count(axis, 1)
+
+    def __repr__(self) -> str:
+        return f"DeviceCommutationMatrix(shape={self.shape}, device={self._dev}, backend='hip')"
+
+
+def hip_devices() -> list[dict[str, Any]]:
+    lib = _native.load()
+    count = C.c_int()
+    _native.check(lib.wg_device_count(C.byref(count)))
+    devices = []
+    for i in range(count.value):
+        name = C.create_string_buffer(256)
+        gfx = C.create_string_buffer(64)
+        mem = C.c_size_t()
+        cus = C.c_int()
+        _native.call("wg_device_info", i, name, 256, gfx, 64, C.byref(mem), C.byref(cus))
+        devices.append(
+            {
+                "ordinal": i,
+                "name": name.value.decode(),
+                "gfx_target": gfx.value.decode(),
+                "total_memory_bytes": mem.value,
+                "compute_units": cus.value,
+            }
+        )
+    return devices
+
+
+def hip_status() -> dict[str, Any]:
+    """Reference DevicePauliSum::hip_status (src/hip/device_pauli_sum.hip.cpp:171-221)."""
+    lib = _native.load()
+    count = C.c_int()
+    _native.check(lib.wg_device_count(C.byref(count)))
+    reason = _native.last_error() if count.value == 0 else ""
+    rt, drv = C.c_int(), C.c_int()
+    lib.wg_runtime_versions(C.byref(rt), C.byref(drv))
+    return {
+        "built": True,
+        "runtime_available": count.value > 0,
+        "device_count": count.value,
+        "skip_reason": reason,
+        "runtime_version": rt.value,
+        "driver_version": drv.value,
+        "devices": hip_devices() if count.value else [],
+    }
+
+
+def hip_available() -> bool:
+    try:
+        return hip_status()["runtime_available"]
+    except RuntimeError:
+        return False
+
+
+def cuda_available() -> bool:
+    return False
+
+
+def metal_available() -> bool:
+    return False
