@@ -150,6 +150,20 @@ int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t len
                            double* term_values, double* sigma, double* sigma_dagger,
                            void* stream, double* out2);
 
+/* Expectation plan summary for an operator (built and cached on first use;
+ * no reference counterpart): distinct (x, z) keys, distinct x-masks counting
+ * the diagonal group (G_x), sweeps over psi per evaluation (tiled passes, or
+ * work items for the direct sweep) and whether the tiled sweep is used. */
+int wg_expectation_plan_info(const wg_pauli_sum* op, size_t* distinct_keys, size_t* x_groups,
+                             size_t* sweeps, int* tiled, void* stream);
+
+/* Host-only verification of the tiled expectation plan (test hook; no
+ * reference counterpart, no GPU needed): replays the tile decomposition of a
+ * W = 1 operator on the CPU over psi (length 2^n, complex128) and writes <P>
+ * per distinct (x, z) key in first-appearance order. */
+int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* psi, double* key_vals, size_t* num_keys, size_t* num_passes);
+
 /* Kernel timing with HIP events (no reference counterpart; measurement only).
  * While enabled, the library brackets each launch of its hot kernels
  * ("expect_direct", "commute_packed", "product", "radix_scatter", ...) with an

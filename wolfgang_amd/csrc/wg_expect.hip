@@ -7,13 +7,23 @@
 //
 // Device plan (built once per operator on the host, cached on the handle):
 //   * distinct (x, z) keys, with every term mapped to its key;
-//   * keys grouped by x-mask; a group's keys share the amplitude products
-//     a_b = conj(psi[b^x]) psi[b], so one streaming pass over psi per x-group
-//     serves all of its z-masks ("items" hold up to kZ keys of one group);
-//   * per item, per block, partial sums S_z = sum_b (-1)^{|z&b|} a_b written to
-//     a partial buffer; a second kernel reduces them in a fixed order and a
-//     third forms E = sum_t c_t <P_key(t)> in a fixed tree order, so results
-//     are bit-identical from call to call.
+//   * keys grouped by x-mask: a group's keys share the amplitude products
+//     a_b = conj(psi[b^x]) psi[b], so one sweep serves all of its z-masks;
+//     "items" hold up to kZ keys of one group;
+//   * partial sums S_z = sum_b (-1)^{|z&b|} a_b per (item, block) go to a
+//     partial buffer; a second kernel reduces them in a fixed order and a third
+//     forms E = sum_t c_t <P_key(t)> in a fixed tree order, so repeated calls
+//     give identical bits.
+//
+// Two sweeps over psi:
+//   * direct (small states): every item streams psi[b], psi[b^x] from global
+//     memory; the whole state is cache resident.
+//   * tiled (large states): psi is cut into tiles that are cosets of a subspace
+//     U of the index space (the low 6 bits, some further coordinate bits and
+//     some generator masks); a block stages one tile in LDS and each of its
+//     waves applies one item whose x lies in U. A pass covers every x-group in
+//     U with one read of psi from HBM, so the HBM traffic is (#passes) x |psi|
+//     instead of (#x-groups) x |psi|. Passes are chosen greedily on the host.
 //
 // Adjoint: sigma = H psi = dE/dpsi* with (P psi)[b ^ x] = i^{|x&z|}
 // (-1)^{|z&b|} psi[b]; each output amplitude is owned by one thread which
@@ -24,6 +34,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 #include <unordered_map>
 #include <vector>
 
@@ -35,21 +46,24 @@ namespace wg {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kTileBlock = 512;
+constexpr int kTileWaves = kTileBlock / 64;
+// States with at least this many qubits use the tiled sweep.
+constexpr std::size_t kTiledMinQubits = 18;
 
 __device__ inline u64 insert_zero(u64 v, u64 pivot) {
   const u64 low = pivot - 1;
   return ((v & ~low) << 1) | (v & low);
 }
 
-// ------------------------------------------------------------ forward pass
+// ------------------------------------------------------------ direct sweep
 
-template <int ZC>
 __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __restrict__ psi,
                                                                u64 len,
                                                                const ExpItem* __restrict__ items,
                                                                u32 blocks_per_item,
                                                                double2* __restrict__ partials) {
-  __shared__ double2 lds[kBlock / 64][ZC];
+  __shared__ double2 lds[kBlock / 64][kZ];
   const u32 item_id = blockIdx.x / blocks_per_item;
   const u32 chunk = blockIdx.x % blocks_per_item;
   const ExpItem& it = items[item_id];
@@ -60,9 +74,9 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
   const u64 begin = static_cast<u64>(chunk) * per;
   const u64 end = begin + per < range ? begin + per : range;
 
-  double ar[ZC], ai[ZC];
+  double ar[kZ], ai[kZ];
 #pragma unroll
-  for (int j = 0; j < ZC; ++j) {
+  for (int j = 0; j < kZ; ++j) {
     ar[j] = 0.0;
     ai[j] = 0.0;
   }
@@ -76,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
       const double re = fma(t.x, a.x, t.y * a.y);
       const double im = fma(t.x, a.y, -(t.y * a.x));
 #pragma unroll
-      for (int j = 0; j < ZC; ++j) {
+      for (int j = 0; j < kZ; ++j) {
         if (j < nz) {
           const bool odd = __popcll(it.z[j] & b) & 1;
           ar[j] += odd ? -re : re;
@@ -89,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
       const double2 a = psi[b];
       const double p = fma(a.x, a.x, a.y * a.y);
 #pragma unroll
-      for (int j = 0; j < ZC; ++j) {
+      for (int j = 0; j < kZ; ++j) {
         if (j < nz) {
           const bool odd = __popcll(it.z[j] & b) & 1;
           ar[j] += odd ? -p : p;
@@ -99,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < ZC; ++j) {
+  for (int j = 0; j < kZ; ++j) {
     if (j < nz) {
       double r = ar[j], i = ai[j];
       for (int d = 32; d > 0; d >>= 1) {
@@ -110,32 +124,165 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
     }
   }
   __syncthreads();
-  if (threadIdx.x < nz) {
+  if (threadIdx.x < static_cast<unsigned>(nz)) {
     double r = 0.0, i = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) {
       r += lds[w][threadIdx.x].x;
       i += lds[w][threadIdx.x].y;
     }
-    partials[(static_cast<std::size_t>(item_id) * blocks_per_item + chunk) * ZC + threadIdx.x] =
+    partials[(static_cast<std::size_t>(item_id) * blocks_per_item + chunk) * kZ + threadIdx.x] =
         make_double2(r, i);
   }
 }
 
-// One thread per (item, slot): ordered sum over the item's blocks, then the
-// key value <P> = 2 Re(i^y S) (x != 0) or Re S (x == 0).
+// ------------------------------------------------------------ tiled sweep
+
+__device__ inline u64 tile_rep(u64 t, const PassDesc& pd) {
+  u64 rep = 0;
+  for (int j = 0; j < pd.nfree; ++j) rep |= ((t >> j) & 1ULL) << pd.free_pos[j];
+  return rep;
+}
+
+// Grid: item_groups * tile_blocks blocks. Block -> (item group, tile block)
+// such that the blocks sharing a tile block differ by multiples of 8 in their
+// linear id: they are dispatched to the same XCD close together, so the later
+// reads of a tile hit that XCD's L2 instead of HBM.
+__global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
+    const double2* __restrict__ psi, const PassDesc pd, const ExpItem* __restrict__ items,
+    u32 num_items, u32 item_groups, u32 tile_blocks, u32 partial_base,
+    double2* __restrict__ partials) {
+  extern __shared__ double2 tile[];
+  const u32 b = blockIdx.x;
+  const u32 xcd = b & 7U;
+  const u32 slot = b >> 3;
+  const u32 ig = slot % item_groups;
+  const u32 tb = (slot / item_groups) * 8U + xcd;
+  const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const u32 lane = threadIdx.x & 63U;
+  const u32 item_id = ig * kTileWaves + wave;
+  const bool active = item_id < num_items;
+  const ExpItem& it = items[active ? item_id : 0];
+  const int nz = active ? it.nz : 0;
+  const u32 xl = static_cast<u32>(it.xl);
+  const u32 pbit = static_cast<u32>(it.pbit);
+
+  const int k = pd.k;
+  const int l = pd.l;
+  const u32 lmask = (1U << l) - 1U;
+  const u32 tile_size = 1U << k;
+  const u64 ntiles = 1ULL << (pd.n - k);
+  constexpr int kLoads = (1 << kTileMaxLog) / kTileBlock;
+
+  double ar[kZ], ai[kZ];
+#pragma unroll
+  for (int j = 0; j < kZ; ++j) {
+    ar[j] = 0.0;
+    ai[j] = 0.0;
+  }
+
+  double2 stage[kLoads];
+  u64 t = tb;
+  if (t < ntiles) {
+    const u64 rep = tile_rep(t, pd);
+#pragma unroll
+    for (int i = 0; i < kLoads; ++i) {
+      const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
+      if (u < tile_size) stage[i] = psi[(rep ^ pd.seg_off[u >> l]) | (u & lmask)];
+    }
+  }
+  for (; t < ntiles; t += tile_blocks) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kLoads; ++i) {
+      const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
+      if (u < tile_size) tile[u] = stage[i];
+    }
+    __syncthreads();
+    const u64 rep = tile_rep(t, pd);
+    const u64 tn = t + tile_blocks;
+    if (tn < ntiles) {
+      // Prefetch the next tile into registers while this one is consumed.
+      const u64 rep_next = tile_rep(tn, pd);
+#pragma unroll
+      for (int i = 0; i < kLoads; ++i) {
+        const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
+        if (u < tile_size) stage[i] = psi[(rep_next ^ pd.seg_off[u >> l]) | (u & lmask)];
+      }
+    }
+    if (nz == 0) continue;
+    // Sign of the coset representative per key (wave-uniform).
+    u32 srep = 0;
+#pragma unroll
+    for (int j = 0; j < kZ; ++j) {
+      if (j < nz) srep |= static_cast<u32>(__popcll(it.z[j] & rep) & 1) << j;
+    }
+    if (xl) {
+      for (u32 v = lane; v < (tile_size >> 1); v += 64) {
+        const u32 u = static_cast<u32>(insert_zero(v, pbit));
+        const double2 a = tile[u];
+        const double2 s = tile[u ^ xl];
+        const double re = fma(s.x, a.x, s.y * a.y);
+        const double im = fma(s.x, a.y, -(s.y * a.x));
+#pragma unroll
+        for (int j = 0; j < kZ; ++j) {
+          if (j < nz) {
+            const bool odd = (__popc(static_cast<u32>(it.zl[j]) & u) ^ (srep >> j)) & 1;
+            ar[j] += odd ? -re : re;
+            ai[j] += odd ? -im : im;
+          }
+        }
+      }
+    } else {
+      for (u32 u = lane; u < tile_size; u += 64) {
+        const double2 a = tile[u];
+        const double p = fma(a.x, a.x, a.y * a.y);
+#pragma unroll
+        for (int j = 0; j < kZ; ++j) {
+          if (j < nz) {
+            const bool odd = (__popc(static_cast<u32>(it.zl[j]) & u) ^ (srep >> j)) & 1;
+            ar[j] += odd ? -p : p;
+          }
+        }
+      }
+    }
+  }
+  if (!active) return;
+  // Wave-level reduction; each wave owns its item's partial row for this tile block.
+#pragma unroll
+  for (int j = 0; j < kZ; ++j) {
+    if (j < nz) {
+      double r = ar[j], i = ai[j];
+      for (int d = 32; d > 0; d >>= 1) {
+        r += __shfl_xor(r, d, 64);
+        i += __shfl_xor(i, d, 64);
+      }
+      if (lane == 0) {
+        partials[(static_cast<std::size_t>(partial_base) +
+                  static_cast<std::size_t>(item_id) * tile_blocks + tb) * kZ + j] =
+            make_double2(r, i);
+      }
+    }
+  }
+}
+
+// One thread per (item, slot): ordered sum over the item's partial rows, then
+// the key value <P> = 2 Re(i^y S) (x != 0) or Re S (x == 0).
 __global__ __launch_bounds__(256) void expect_reduce_kernel(const ExpItem* __restrict__ items,
-                                                            u32 num_items, u32 blocks_per_item,
+                                                            u32 num_items,
+                                                            const u32* __restrict__ offset,
+                                                            const u32* __restrict__ count,
                                                             const double2* __restrict__ partials,
-                                                            int zc, double* __restrict__ key_vals) {
+                                                            double* __restrict__ key_vals) {
   const std::size_t gid = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::size_t item_id = gid / zc;
-  const int j = static_cast<int>(gid % zc);
+  const std::size_t item_id = gid / kZ;
+  const int j = static_cast<int>(gid % kZ);
   if (item_id >= num_items) return;
   const ExpItem& it = items[item_id];
   if (j >= it.nz) return;
   double r = 0.0, i = 0.0;
-  for (u32 c = 0; c < blocks_per_item; ++c) {
-    const double2 p = partials[(item_id * blocks_per_item + c) * zc + j];
+  const std::size_t base = offset[item_id];
+  for (u32 c = 0; c < count[item_id]; ++c) {
+    const double2 p = partials[(base + c) * kZ + j];
     r += p.x;
     i += p.y;
   }
@@ -143,7 +290,7 @@ __global__ __launch_bounds__(256) void expect_reduce_kernel(const ExpItem* __res
   if (it.x == 0) {
     val = r;
   } else {
-    switch (__popcll(it.x & it.z[j]) & 3) {
+    switch (it.y[j] & 3) {
       case 0: val = 2.0 * r; break;
       case 1: val = -2.0 * i; break;
       case 2: val = -2.0 * r; break;
@@ -239,16 +386,237 @@ unsigned stride_grid(std::size_t work, unsigned block) {
   return static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g, 1u << 20)));
 }
 
+// ------------------------------------------------------------ pass planner
+
+// Subspace U = span(coordinate bits in `coord`, generators `gens`), generators
+// kept in reduced echelon form: each has a distinct pivot (its highest bit),
+// no bit inside `coord`, and no generator has another generator's pivot set.
+struct Span {
+  u64 coord = 0;
+  std::vector<u64> gens;
+
+  int dim() const { return __builtin_popcountll(coord) + static_cast<int>(gens.size()); }
+
+  static int top(u64 v) { return 63 - __builtin_clzll(v); }
+
+  void normalize() {
+    std::vector<u64> out;
+    for (u64 v : gens) {
+      v &= ~coord;
+      for (u64 e : out) {
+        if ((v >> top(e)) & 1ULL) v ^= e;
+      }
+      if (v) {
+        for (u64& e : out) {
+          if ((e >> top(v)) & 1ULL) e ^= v;
+        }
+        out.push_back(v);
+      }
+    }
+    gens = out;
+  }
+
+  // Reduces x against the generators; returns the residual (inside `coord`
+  // iff x lies in U) and the generator selection.
+  u64 reduce(u64 x, u32* sel) const {
+    u32 s = 0;
+    for (std::size_t i = 0; i < gens.size(); ++i) {
+      if ((x >> top(gens[i])) & 1ULL) {
+        x ^= gens[i];
+        s |= 1U << i;
+      }
+    }
+    if (sel) *sel = s;
+    return x;
+  }
+
+  bool contains(u64 x) const { return (reduce(x, nullptr) & ~coord) == 0; }
+
+  void add_coords(u64 bits) {
+    coord |= bits;
+    normalize();
+  }
+
+  void add_gen(u64 v) {
+    gens.push_back(v);
+    normalize();
+  }
+};
+
 }  // namespace
+
+void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes) {
+  const int l = kTileLowLog;
+  const int kmax = std::min<int>(kTileMaxLog, static_cast<int>(n) - 3);
+  const u64 lowmask = (1ULL << l) - 1;
+  // Distinct nonzero x in first-appearance order.
+  std::vector<u64> xs;
+  std::unordered_map<u64, int> seen;
+  for (const ExpItem& it : items) {
+    if (it.x && seen.emplace(it.x, 1).second) xs.push_back(it.x);
+  }
+  std::vector<char> covered(xs.size(), 0);
+  std::size_t remaining = xs.size();
+  std::vector<Span> spans;
+  std::vector<std::vector<u64>> pass_xs;
+  const bool structured = xs.size() <= 4096;
+  std::size_t cursor = 0;
+  while (remaining > 0 || spans.empty()) {
+    Span sp;
+    sp.coord = lowmask;
+    if (remaining > 0) {
+      if (structured) {
+        // Greedy: repeatedly extend U by the option (the coordinate bits of one
+        // candidate's residual, or that residual as a generator) that covers
+        // the most uncovered x-groups per added dimension.
+        while (sp.dim() < kmax) {
+          double best_score = 0.0;
+          int best_cost = 0;
+          u64 best_bits = 0;
+          bool best_is_gen = false;
+          std::size_t tried = 0;
+          for (std::size_t i = 0; i < xs.size() && tried < 192; ++i) {
+            if (covered[i] || sp.contains(xs[i])) continue;
+            ++tried;
+            const u64 resid = sp.reduce(xs[i], nullptr) & ~sp.coord;
+            for (int option = 0; option < 2; ++option) {
+              const bool as_gen = option == 1;
+              const int cost = as_gen ? 1 : __builtin_popcountll(resid);
+              if (as_gen && __builtin_popcountll(resid) == 1) continue;
+              if (sp.dim() + cost > kmax) continue;
+              Span trial = sp;
+              if (as_gen) trial.add_gen(resid); else trial.add_coords(resid);
+              std::size_t gain = 0;
+              for (std::size_t j = 0; j < xs.size(); ++j) {
+                if (!covered[j] && trial.contains(xs[j])) ++gain;
+              }
+              const double score = static_cast<double>(gain) / cost;
+              if (score > best_score + 1e-12 || (score > best_score - 1e-12 && cost < best_cost)) {
+                best_score = score;
+                best_cost = cost;
+                best_bits = resid;
+                best_is_gen = as_gen;
+              }
+            }
+          }
+          if (best_cost == 0) break;
+          if (best_is_gen) sp.add_gen(best_bits); else sp.add_coords(best_bits);
+        }
+      } else {
+        // Unstructured x-masks: one generator per x-group, in order.
+        while (sp.dim() < kmax && cursor < xs.size()) {
+          if (!covered[cursor] && !sp.contains(xs[cursor])) {
+            sp.add_gen(sp.reduce(xs[cursor], nullptr) & ~sp.coord);
+          }
+          ++cursor;
+        }
+      }
+    }
+    std::vector<u64> in_pass;
+    for (std::size_t j = 0; j < xs.size(); ++j) {
+      if (!covered[j] && sp.contains(xs[j])) {
+        covered[j] = 1;
+        --remaining;
+        in_pass.push_back(xs[j]);
+      }
+    }
+    spans.push_back(sp);
+    pass_xs.push_back(std::move(in_pass));
+    if (remaining == 0) break;
+  }
+
+  // Emit passes: local coordinates, segment offsets, free positions; items are
+  // reordered pass by pass (diagonal items ride in the first pass).
+  std::unordered_map<u64, int> pass_of;
+  for (std::size_t p = 0; p < pass_xs.size(); ++p) {
+    for (u64 x : pass_xs[p]) pass_of[x] = static_cast<int>(p);
+  }
+  std::vector<ExpItem> ordered;
+  passes.clear();
+  for (std::size_t p = 0; p < spans.size(); ++p) {
+    const Span& sp = spans[p];
+    PassLaunch pl{};
+    PassDesc& d = pl.desc;
+    d.n = static_cast<int>(n);
+    d.l = l;
+    std::vector<int> apos;
+    for (int q = l; q < 64; ++q) {
+      if ((sp.coord >> q) & 1ULL) apos.push_back(q);
+    }
+    const int a = static_cast<int>(apos.size());
+    const int m = static_cast<int>(sp.gens.size());
+    d.k = l + a + m;
+    u64 occupied = sp.coord;
+    for (u64 g : sp.gens) occupied |= 1ULL << Span::top(g);
+    d.nfree = 0;
+    for (int q = 0; q < static_cast<int>(n); ++q) {
+      if (!((occupied >> q) & 1ULL)) d.free_pos[d.nfree++] = static_cast<unsigned char>(q);
+    }
+    for (int s = 0; s < (1 << (a + m)); ++s) {
+      u64 off = 0;
+      for (int i = 0; i < a; ++i) {
+        if ((s >> i) & 1) off |= 1ULL << apos[i];
+      }
+      for (int i = 0; i < m; ++i) {
+        if ((s >> (a + i)) & 1) off ^= sp.gens[i];
+      }
+      d.seg_off[s] = off;
+    }
+    // x in U -> its coordinates in the basis (low bits, apos, gens).
+    auto x_local = [&](u64 v) -> u64 {
+      u32 sel = 0;
+      const u64 r = sp.reduce(v, &sel);
+      u64 out = r & lowmask;
+      for (int i = 0; i < a; ++i) {
+        if ((r >> apos[i]) & 1ULL) out |= 1ULL << (l + i);
+      }
+      return out | (static_cast<u64>(sel) << (l + a));
+    };
+    // z -> local sign mask: parity(z & (rep ^ b_local(u))) = parity(z & rep) ^ parity(zl & u).
+    auto z_local = [&](u64 v) -> u64 {
+      u64 out = v & lowmask;
+      for (int i = 0; i < a; ++i) {
+        if ((v >> apos[i]) & 1ULL) out |= 1ULL << (l + i);
+      }
+      for (int i = 0; i < m; ++i) {
+        if (__builtin_popcountll(v & sp.gens[i]) & 1) out |= 1ULL << (l + a + i);
+      }
+      return out;
+    };
+    pl.first_item = static_cast<u32>(ordered.size());
+    for (int diag_round = 1; diag_round >= 0; --diag_round) {
+      for (const ExpItem& it : items) {
+        const bool is_diag = it.x == 0;
+        if (is_diag != (diag_round == 1)) continue;
+        if (is_diag ? p != 0 : pass_of.at(it.x) != static_cast<int>(p)) continue;
+        ExpItem e = it;
+        e.xl = is_diag ? 0 : x_local(it.x);
+        e.pbit = e.xl & (0 - e.xl);
+        for (int j = 0; j < e.nz; ++j) e.zl[j] = z_local(it.z[j]);
+        ordered.push_back(e);
+      }
+    }
+    pl.num_items = static_cast<u32>(ordered.size()) - pl.first_item;
+    pl.item_groups = std::max<u32>(1, (pl.num_items + kTileWaves - 1) / kTileWaves);
+    const u64 ntiles = 1ULL << (n - d.k);
+    u64 tb = (1024 + pl.item_groups - 1) / pl.item_groups;
+    tb = std::max<u64>(8, (tb + 7) / 8 * 8);
+    tb = std::min<u64>(tb, std::max<u64>(8, ntiles / 8 * 8));
+    pl.tile_blocks = static_cast<u32>(tb);
+    passes.push_back(pl);
+  }
+  items = std::move(ordered);
+}
 
 // ------------------------------------------------------------ plan
 
 ExpectationPlan::~ExpectationPlan() {
-  if (!d_items && !d_term_key && !d_group_x) return;
   int prev = -1;
   const hipError_t r = hipGetDevice(&prev);
   (void)hipSetDevice(device);
   device_free(d_items);
+  device_free(d_partial_offset);
+  device_free(d_partial_count);
   device_free(d_term_key);
   device_free(d_group_x);
   device_free(d_group_begin);
@@ -257,6 +625,8 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_key_c);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
+
+namespace {
 
 std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t stream) {
   auto plan = std::make_shared<ExpectationPlan>();
@@ -315,13 +685,11 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     }
     gkeys[itr->second].push_back(static_cast<u32>(k));
   }
-  // Items of up to kZ keys.
   std::vector<ExpItem> items;
   std::vector<u64> flat_z;
   std::vector<int> flat_y;
   std::vector<double2> flat_c;
   std::vector<u32> group_begin;
-  std::vector<u32> key_order;  // key ids in group order
   for (std::size_t g = 0; g < gx.size(); ++g) {
     group_begin.push_back(static_cast<u32>(flat_z.size()));
     const auto& ks = gkeys[g];
@@ -332,6 +700,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
       for (int j = 0; j < it.nz; ++j) {
         it.z[j] = kz[ks[s + j]];
         it.key[j] = ks[s + j];
+        it.y[j] = __builtin_popcountll(gx[g] & kz[ks[s + j]]) & 3;
       }
       items.push_back(it);
     }
@@ -344,16 +713,38 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   group_begin.push_back(static_cast<u32>(flat_z.size()));
 
   plan->num_keys = K;
-  plan->num_items = items.size();
   plan->num_groups = gx.size();
-  const u64 half = plan->n == 0 ? 1 : (u64{1} << (plan->n - 1));
-  plan->half_range = plan->n == 0 ? 0 : half;
-  // Blocks per item: enough to fill the device, each thread doing a few
-  // hundred amplitude pairs at large n.
-  const u64 range = plan->n == 0 ? 1 : half;
-  u64 bpi = (range + kBlock * 64 - 1) / (kBlock * 64);
-  bpi = std::max<u64>(1, std::min<u64>(bpi, 1024));
-  plan->blocks_per_item = static_cast<u32>(bpi);
+  plan->distinct_x = gx.size();
+  plan->tiled = plan->n >= kTiledMinQubits && !items.empty();
+  std::vector<u32> offset, count;
+  if (plan->tiled) {
+    plan_tiled_passes(plan->n, items, plan->passes);
+    offset.resize(items.size());
+    count.resize(items.size());
+    std::size_t slot = 0;
+    for (PassLaunch& p : plan->passes) {
+      p.partial_base = static_cast<u32>(slot);
+      for (u32 i = 0; i < p.num_items; ++i) {
+        offset[p.first_item + i] = static_cast<u32>(slot + static_cast<std::size_t>(i) * p.tile_blocks);
+        count[p.first_item + i] = p.tile_blocks;
+      }
+      slot += static_cast<std::size_t>(p.num_items) * p.tile_blocks;
+    }
+    plan->partial_slots = slot;
+  } else {
+    const u64 range = plan->n == 0 ? 1 : (u64{1} << (plan->n - 1));
+    u64 bpi = (range + kBlock * 64 - 1) / (kBlock * 64);
+    bpi = std::max<u64>(1, std::min<u64>(bpi, 1024));
+    plan->blocks_per_item = static_cast<u32>(bpi);
+    offset.resize(items.size());
+    count.resize(items.size());
+    for (std::size_t i = 0; i < items.size(); ++i) {
+      offset[i] = static_cast<u32>(i * bpi);
+      count[i] = static_cast<u32>(bpi);
+    }
+    plan->partial_slots = items.size() * bpi;
+  }
+  plan->num_items = items.size();
 
   DeviceScope scope(op->device);
   auto upload = [&](auto*& dst, const auto& src) {
@@ -366,6 +757,8 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     }
   };
   upload(plan->d_items, items);
+  upload(plan->d_partial_offset, offset);
+  upload(plan->d_partial_count, count);
   upload(plan->d_term_key, term_key);
   upload(plan->d_group_x, gx);
   upload(plan->d_group_begin, group_begin);
@@ -376,28 +769,45 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   return plan;
 }
 
+}  // namespace
+
 std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(op->plan_mutex);
   if (!op->plan) op->plan = build_plan(op, stream);
   return op->plan;
 }
 
-// Runs the forward path on a device-resident complex128 psi. key_vals receives
-// <P> per distinct key; out (device) receives E; term_values optional.
+namespace {
+
+// Runs the forward path on a device-resident complex128 psi: <P> per distinct
+// key, E into out_dev, per-term values optionally.
 void run_forward(const wg_pauli_sum* op, const ExpectationPlan& plan, const double2* psi,
                  double2* term_values, double2* out_dev, Scratch& scratch, hipStream_t stream) {
   double* key_vals = scratch.take<double>(std::max<std::size_t>(plan.num_keys, 1));
   if (plan.num_items) {
-    const std::size_t nblocks = plan.num_items * plan.blocks_per_item;
-    double2* partials = scratch.take<double2>(checked_mul(nblocks, kZ, "expectation partials"));
-    const u64 len = u64{1} << plan.n;
-    ProfileScope prof("expect_direct", stream);
-    expect_direct_kernel<kZ><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
-        psi, len, plan.d_items, plan.blocks_per_item, partials);
-    check_launch("expect_direct");
+    double2* partials =
+        scratch.take<double2>(checked_mul(plan.partial_slots, kZ, "expectation partials"));
+    if (plan.tiled) {
+      for (const PassLaunch& p : plan.passes) {
+        if (p.num_items == 0) continue;
+        const unsigned grid = p.item_groups * p.tile_blocks;
+        const std::size_t lds = (std::size_t{1} << p.desc.k) * sizeof(double2);
+        ProfileScope prof("expect_tiled", stream);
+        expect_tiled_kernel<<<grid, kTileBlock, lds, stream>>>(
+            psi, p.desc, plan.d_items + p.first_item, p.num_items, p.item_groups, p.tile_blocks,
+            p.partial_base, partials);
+        check_launch("expect_tiled");
+      }
+    } else {
+      const u64 len = u64{1} << plan.n;
+      ProfileScope prof("expect_direct", stream);
+      expect_direct_kernel<<<grid_for(plan.num_items * plan.blocks_per_item, 1), kBlock, 0,
+                             stream>>>(psi, len, plan.d_items, plan.blocks_per_item, partials);
+      check_launch("expect_direct");
+    }
     expect_reduce_kernel<<<grid_for(plan.num_items * kZ, 256), 256, 0, stream>>>(
-        plan.d_items, static_cast<u32>(plan.num_items), plan.blocks_per_item, partials, kZ,
-        key_vals);
+        plan.d_items, static_cast<u32>(plan.num_items), plan.d_partial_offset,
+        plan.d_partial_count, partials, key_vals);
     check_launch("expect_reduce");
   }
   expect_finalize_kernel<<<1, 1024, 0, stream>>>(op->coeffs, plan.d_term_key, plan.T, key_vals,
@@ -433,6 +843,7 @@ std::size_t validate_psi(const wg_pauli_sum* op, const void* psi, std::size_t le
   return expected;
 }
 
+}  // namespace
 }  // namespace wg
 
 using namespace wg;
@@ -520,6 +931,135 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
     sync(stream);
     out2[0] = h.x;
     out2[1] = h.y;
+  });
+}
+
+int wg_expectation_plan_info(const wg_pauli_sum* op_c, size_t* distinct_keys, size_t* x_groups,
+                             size_t* sweeps, int* tiled, void* stream_ptr) {
+  return guard([&] {
+    require_op(op_c);
+    auto* op = const_cast<wg_pauli_sum*>(op_c);
+    DeviceScope scope(op->device);
+    auto plan = get_plan(op, as_stream(stream_ptr));
+    if (distinct_keys) *distinct_keys = plan->num_keys;
+    if (x_groups) *x_groups = plan->distinct_x;
+    if (sweeps) *sweeps = plan->tiled ? plan->passes.size() : plan->num_items;
+    if (tiled) *tiled = plan->tiled ? 1 : 0;
+  });
+}
+
+// Host-only check of the tiled pass plan for an operator given as host masks
+// (W = 1): builds the items and passes exactly as the device plan does, then
+// replays the tile decomposition on the CPU over `psi` (length 2^n) with the
+// same local-coordinate arithmetic as expect_tiled_kernel and writes <P> per
+// distinct key (first-appearance order) to key_vals. Returns the number of
+// passes in *num_passes. Test-only: no product entry point calls it.
+int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* psi, double* key_vals, size_t* num_keys, size_t* num_passes) {
+  return guard([&] {
+    const std::size_t n = num_qubits;
+    if (n < 9 || n > 26) fail_invalid("debug tiled plan supports 9 <= n <= 26");
+    // Distinct keys and groups, as build_plan.
+    std::vector<u64> kx, kz;
+    std::unordered_map<u64, std::vector<u32>> by_x;
+    std::vector<u64> order_x;
+    {
+      std::unordered_map<u64, std::unordered_map<u64, u32>> idx;
+      for (std::size_t t = 0; t < num_terms; ++t) {
+        auto& zs = idx[x[t]];
+        if (zs.emplace(z[t], static_cast<u32>(kx.size())).second) {
+          if (by_x.find(x[t]) == by_x.end()) order_x.push_back(x[t]);
+          by_x[x[t]].push_back(static_cast<u32>(kx.size()));
+          kx.push_back(x[t]);
+          kz.push_back(z[t]);
+        }
+      }
+    }
+    std::vector<ExpItem> items;
+    for (u64 gx : order_x) {
+      const auto& ks = by_x[gx];
+      for (std::size_t s = 0; s < ks.size(); s += kZ) {
+        ExpItem it{};
+        it.x = gx;
+        it.nz = static_cast<int>(std::min<std::size_t>(kZ, ks.size() - s));
+        for (int j = 0; j < it.nz; ++j) {
+          it.z[j] = kz[ks[s + j]];
+          it.key[j] = ks[s + j];
+          it.y[j] = __builtin_popcountll(gx & kz[ks[s + j]]) & 3;
+        }
+        items.push_back(it);
+      }
+    }
+    std::vector<PassLaunch> passes;
+    plan_tiled_passes(n, items, passes);
+    *num_passes = passes.size();
+    *num_keys = kx.size();
+    const auto* amp = reinterpret_cast<const double2*>(psi);
+    std::vector<double2> tile;
+    std::vector<char> seen(std::size_t{1} << n);
+    for (const PassLaunch& p : passes) {
+      const PassDesc& d = p.desc;
+      const u32 tile_size = 1U << d.k;
+      const u64 ntiles = 1ULL << (n - d.k);
+      std::fill(seen.begin(), seen.end(), 0);
+      tile.assign(tile_size, make_double2(0, 0));
+      std::vector<double2> acc(static_cast<std::size_t>(p.num_items) * kZ, make_double2(0, 0));
+      for (u64 t = 0; t < ntiles; ++t) {
+        u64 rep = 0;
+        for (int j = 0; j < d.nfree; ++j) rep |= ((t >> j) & 1ULL) << d.free_pos[j];
+        for (u32 u = 0; u < tile_size; ++u) {
+          const u64 b = (rep ^ d.seg_off[u >> d.l]) | (u & ((1U << d.l) - 1));
+          if (b >= (u64{1} << n) || seen[b]) fail_runtime("tiles do not partition the index space");
+          seen[b] = 1;
+          tile[u] = amp[b];
+        }
+        for (u32 i = 0; i < p.num_items; ++i) {
+          const ExpItem& it = items[p.first_item + i];
+          for (int j = 0; j < it.nz; ++j) {
+            const u32 srep = __builtin_popcountll(it.z[j] & rep) & 1;
+            double2& a = acc[static_cast<std::size_t>(i) * kZ + j];
+            if (it.xl) {
+              for (u32 v = 0; v < tile_size / 2; ++v) {
+                const u32 low = static_cast<u32>(it.pbit) - 1;
+                const u32 u = ((v & ~low) << 1) | (v & low);
+                const double2 A = tile[u], S = tile[u ^ static_cast<u32>(it.xl)];
+                const double re = S.x * A.x + S.y * A.y, im = S.x * A.y - S.y * A.x;
+                const bool odd = ((__builtin_popcount(static_cast<u32>(it.zl[j]) & u)) ^ srep) & 1;
+                a.x += odd ? -re : re;
+                a.y += odd ? -im : im;
+              }
+            } else {
+              for (u32 u = 0; u < tile_size; ++u) {
+                const double pr = tile[u].x * tile[u].x + tile[u].y * tile[u].y;
+                const bool odd = ((__builtin_popcount(static_cast<u32>(it.zl[j]) & u)) ^ srep) & 1;
+                a.x += odd ? -pr : pr;
+              }
+            }
+          }
+        }
+      }
+      for (char c : seen) {
+        if (!c) fail_runtime("tiles do not cover the index space");
+      }
+      for (u32 i = 0; i < p.num_items; ++i) {
+        const ExpItem& it = items[p.first_item + i];
+        for (int j = 0; j < it.nz; ++j) {
+          const double2 S = acc[static_cast<std::size_t>(i) * kZ + j];
+          double v;
+          if (it.x == 0) {
+            v = S.x;
+          } else {
+            switch (it.y[j] & 3) {
+              case 0: v = 2 * S.x; break;
+              case 1: v = -2 * S.y; break;
+              case 2: v = -2 * S.x; break;
+              default: v = 2 * S.y; break;
+            }
+          }
+          key_vals[it.key[j]] = v;
+        }
+      }
+    }
   });
 }
 

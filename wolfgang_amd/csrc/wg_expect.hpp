@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "wg_internal.hpp"
 
@@ -10,12 +11,47 @@ namespace wg {
 // Keys per work item: one x-group's amplitude products feed up to kZ z-masks.
 constexpr int kZ = 8;
 
+// Largest tile of the tiled pass: 2^kTileMaxLog amplitudes (64 KiB of complex128
+// in LDS, two blocks per CU) and the contiguous low-bit run inside it
+// (2^kTileLowLog amplitudes = 1 KiB per segment).
+constexpr int kTileMaxLog = 12;
+constexpr int kTileLowLog = 6;
+constexpr int kMaxSegments = 1 << (kTileMaxLog - kTileLowLog);
+
+// One x-group (or a slice of up to kZ of its keys).
 struct ExpItem {
-  u64 x;
+  u64 x;        // global x mask (0 = diagonal group)
+  u64 xl;       // x in tile-local coordinates (tiled path)
+  u64 pbit;     // local pivot bit of xl (tiled path)
   int nz;
   int pad;
-  u64 z[kZ];
-  u32 key[kZ];
+  u64 z[kZ];    // global z masks
+  u64 zl[kZ];   // z in tile-local coordinates: parity(z & b) = parity(z & rep) ^ parity(zl & u)
+  u32 key[kZ];  // distinct-key ids
+  int y[kZ];    // |x & z| mod 4
+};
+
+// A tiled pass: tiles are the cosets rep ^ U of the subspace U spanned by the
+// low kTileLowLog bits, `a` further coordinate bits and `m` generator masks.
+// Local coordinate u: bits [0, l) -> low index bits, [l, l+a) -> the coordinate
+// positions, [l+a, k) -> generators; the global index is
+// (rep ^ seg_off[u >> l]) | (u & (2^l - 1)).
+struct PassDesc {
+  int n;
+  int l;
+  int k;
+  int nfree;
+  u64 seg_off[kMaxSegments];
+  unsigned char free_pos[64];
+};
+
+struct PassLaunch {
+  PassDesc desc;
+  u32 first_item;    // items of this pass are [first_item, first_item + num_items)
+  u32 num_items;
+  u32 item_groups;   // blocks of kTileWaves items
+  u32 tile_blocks;   // blocks per item group (multiple of 8)
+  u32 partial_base;  // first partial row of this pass
 };
 
 struct ExpectationPlan {
@@ -25,18 +61,28 @@ struct ExpectationPlan {
   std::size_t num_keys = 0;
   std::size_t num_items = 0;
   std::size_t num_groups = 0;
-  u64 half_range = 0;
-  u32 blocks_per_item = 1;
-  ExpItem* d_items = nullptr;     // [num_items]
-  u32* d_term_key = nullptr;      // [T] term -> distinct key
-  u64* d_group_x = nullptr;       // [num_groups]
-  u32* d_group_begin = nullptr;   // [num_groups + 1] into the key arrays below
-  u64* d_key_z = nullptr;         // keys in group order
-  int* d_key_y = nullptr;         // |x & z| mod 4 per key
-  double2* d_key_c = nullptr;     // coefficient sum per key (terms in order)
+  std::size_t distinct_x = 0;        // G_x: distinct nonzero x + 1 if diagonal terms exist
+  bool tiled = false;
+  u32 blocks_per_item = 1;           // direct path
+  std::vector<PassLaunch> passes;    // tiled path
+  std::vector<u32> item_partial_offset;  // host copy; device copy below
+  std::size_t partial_slots = 0;     // total partial entries (x kZ)
+  ExpItem* d_items = nullptr;        // [num_items]
+  u32* d_partial_offset = nullptr;   // [num_items] first partial slot of each item
+  u32* d_partial_count = nullptr;    // [num_items] partial slots per item
+  u32* d_term_key = nullptr;         // [T] term -> distinct key
+  u64* d_group_x = nullptr;          // [num_groups]
+  u32* d_group_begin = nullptr;      // [num_groups + 1] into the key arrays below
+  u64* d_key_z = nullptr;            // keys in group order
+  int* d_key_y = nullptr;            // |x & z| mod 4 per key
+  double2* d_key_c = nullptr;        // coefficient sum per key (terms in order)
   ~ExpectationPlan();
 };
 
 std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
+
+// Builds the tiled passes for the given items (host only; exposed for tests of
+// the planner through wg_expectation_plan_info).
+void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes);
 
 }  // namespace wg

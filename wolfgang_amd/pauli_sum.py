@@ -566,6 +566,14 @@ class DevicePauliSum:
         _native.call("wg_expectation", self._handle(), ptr, dtype, length, on_device, stream_ptr, out)
         return complex(out[0], out[1])
 
+    def expectation_plan_info(self) -> dict[str, Any]:
+        """Summary of the cached expectation plan (built on first use)."""
+        keys, groups, sweeps, tiled = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_int()
+        _native.call("wg_expectation_plan_info", self._handle(), C.byref(keys), C.byref(groups),
+                     C.byref(sweeps), C.byref(tiled), None)
+        return {"distinct_keys": keys.value, "x_groups": groups.value, "sweeps": sweeps.value,
+                "tiled": bool(tiled.value)}
+
     def expectation_adjoint(
         self, psi_ptr: int, length: int, term_values_ptr: int | None, sigma_ptr: int | None,
         sigma_dagger_ptr: int | None, stream: Any = None,
