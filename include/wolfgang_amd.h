@@ -140,15 +140,17 @@ int wg_expectation(const wg_pauli_sum* op, const void* psi, int dtype, size_t le
                    int psi_on_device, void* stream, double* out2);
 
 /* Forward plus adjoint in one call, all pointers device-resident complex128 on
- * op's device (no reference counterpart: the new adjoint):
+ * op's device (no reference counterpart: the new adjoint). coeffs (num_terms
+ * complex128, may be NULL) replaces the operator's own coefficients for this
+ * call, so a caller training the coefficients reuses one cached plan:
  *   out2            <- E = sum_k c_k <P_k>
  *   term_values     <- <psi|P_k|psi> per term (num_terms complex128; real for
  *                      Hermitian P_k), i.e. dE/dc_k          (may be NULL)
  *   sigma           <- H psi (length complex128), i.e. dE/dpsi*  (may be NULL)
  *   sigma_dagger    <- H^dagger psi (coefficients conjugated)    (may be NULL) */
 int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t length,
-                           double* term_values, double* sigma, double* sigma_dagger,
-                           void* stream, double* out2);
+                           const double* coeffs, double* term_values, double* sigma,
+                           double* sigma_dagger, void* stream, double* out2);
 
 /* Expectation plan summary for an operator (built and cached on first use;
  * no reference counterpart): distinct (x, z) keys, distinct x-masks counting

@@ -381,6 +381,24 @@ __global__ __launch_bounds__(kBlock) void sigma_direct_kernel(
   }
 }
 
+// Per-key coefficient sums from per-term coefficients, each key's terms in term
+// order (the same order the host plan uses for the operator's own coefficients).
+__global__ __launch_bounds__(256) void key_coeff_kernel(const double2* __restrict__ coeffs,
+                                                        const u32* __restrict__ begin,
+                                                        const u32* __restrict__ terms,
+                                                        std::size_t num_keys,
+                                                        double2* __restrict__ key_c) {
+  const std::size_t k = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= num_keys) return;
+  const u32 b = begin[k], e = begin[k + 1];
+  double2 acc = coeffs[terms[b]];
+  for (u32 i = b + 1; i < e; ++i) {
+    acc.x += coeffs[terms[i]].x;
+    acc.y += coeffs[terms[i]].y;
+  }
+  key_c[k] = acc;
+}
+
 unsigned stride_grid(std::size_t work, unsigned block) {
   const std::size_t g = (work + block - 1) / block;
   return static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g, 1u << 20)));
@@ -623,6 +641,8 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_key_z);
   device_free(d_key_y);
   device_free(d_key_c);
+  device_free(d_key_term_begin);
+  device_free(d_key_terms);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -711,6 +731,21 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     }
   }
   group_begin.push_back(static_cast<u32>(flat_z.size()));
+  // Terms of each key (keys in group order, terms in term order).
+  std::vector<u32> flat_pos(K);
+  {
+    u32 pos = 0;
+    for (std::size_t g = 0; g < gx.size(); ++g) {
+      for (u32 k : gkeys[g]) flat_pos[k] = pos++;
+    }
+  }
+  std::vector<u32> key_term_begin(K + 1, 0), key_terms(T);
+  for (std::size_t t = 0; t < T; ++t) key_term_begin[flat_pos[term_key[t]] + 1]++;
+  for (std::size_t k = 0; k < K; ++k) key_term_begin[k + 1] += key_term_begin[k];
+  {
+    std::vector<u32> fill(key_term_begin.begin(), key_term_begin.end() - 1);
+    for (std::size_t t = 0; t < T; ++t) key_terms[fill[flat_pos[term_key[t]]]++] = static_cast<u32>(t);
+  }
 
   plan->num_keys = K;
   plan->num_groups = gx.size();
@@ -765,6 +800,8 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   upload(plan->d_key_z, flat_z);
   upload(plan->d_key_y, flat_y);
   upload(plan->d_key_c, flat_c);
+  upload(plan->d_key_term_begin, key_term_begin);
+  upload(plan->d_key_terms, key_terms);
   sync(stream);
   return plan;
 }
@@ -781,7 +818,7 @@ namespace {
 
 // Runs the forward path on a device-resident complex128 psi: <P> per distinct
 // key, E into out_dev, per-term values optionally.
-void run_forward(const wg_pauli_sum* op, const ExpectationPlan& plan, const double2* psi,
+void run_forward(const double2* coeffs, const ExpectationPlan& plan, const double2* psi,
                  double2* term_values, double2* out_dev, Scratch& scratch, hipStream_t stream) {
   double* key_vals = scratch.take<double>(std::max<std::size_t>(plan.num_keys, 1));
   if (plan.num_items) {
@@ -810,13 +847,13 @@ void run_forward(const wg_pauli_sum* op, const ExpectationPlan& plan, const doub
         plan.d_partial_count, partials, key_vals);
     check_launch("expect_reduce");
   }
-  expect_finalize_kernel<<<1, 1024, 0, stream>>>(op->coeffs, plan.d_term_key, plan.T, key_vals,
+  expect_finalize_kernel<<<1, 1024, 0, stream>>>(coeffs, plan.d_term_key, plan.T, key_vals,
                                                  term_values, out_dev);
   check_launch("expect_finalize");
 }
 
-void run_sigma(const ExpectationPlan& plan, const double2* psi, std::size_t len, double2* sigma,
-               double2* sigma_dag, hipStream_t stream) {
+void run_sigma(const ExpectationPlan& plan, const double2* key_c, const double2* psi,
+               std::size_t len, double2* sigma, double2* sigma_dag, hipStream_t stream) {
   if (plan.num_groups == 0) {
     if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * sizeof(double2), stream), "clear sigma");
     if (sigma_dag) {
@@ -827,7 +864,7 @@ void run_sigma(const ExpectationPlan& plan, const double2* psi, std::size_t len,
   ProfileScope prof("sigma_direct", stream);
   sigma_direct_kernel<<<stride_grid(len, kBlock), kBlock, 0, stream>>>(
       psi, len, plan.d_group_x, plan.d_group_begin, static_cast<u32>(plan.num_groups),
-      plan.d_key_z, plan.d_key_y, plan.d_key_c, sigma, sigma_dag);
+      plan.d_key_z, plan.d_key_y, key_c, sigma, sigma_dag);
   check_launch("sigma_direct");
 }
 
@@ -889,7 +926,7 @@ int wg_expectation(const wg_pauli_sum* op_c, const void* psi, int dtype, size_t 
       dpsi = buf;
     }
     double2* dout = scratch.take<double2>(1);
-    run_forward(op, *plan, dpsi, nullptr, dout, scratch, stream);
+    run_forward(op->coeffs, *plan, dpsi, nullptr, dout, scratch, stream);
     double2 h;
     check_hip(hipMemcpyAsync(&h, dout, sizeof(double2), hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
@@ -900,8 +937,8 @@ int wg_expectation(const wg_pauli_sum* op_c, const void* psi, int dtype, size_t 
 }
 
 int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t length,
-                           double* term_values, double* sigma, double* sigma_dagger,
-                           void* stream_ptr, double* out2) {
+                           const double* coeffs, double* term_values, double* sigma,
+                           double* sigma_dagger, void* stream_ptr, double* out2) {
   return guard([&] {
     require_op(op_c);
     auto* op = const_cast<wg_pauli_sum*>(op_c);
@@ -920,9 +957,18 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
     Scratch scratch(op->device);
     const double2* dpsi = reinterpret_cast<const double2*>(psi);
     double2* dout = scratch.take<double2>(1);
-    run_forward(op, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+    const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
+    run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
     if (sigma || sigma_dagger) {
-      run_sigma(*plan, dpsi, len, reinterpret_cast<double2*>(sigma),
+      const double2* key_c = plan->d_key_c;
+      if (coeffs) {
+        double2* kc = scratch.take<double2>(plan->num_keys);
+        key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+            c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
+        check_launch("key_coeff");
+        key_c = kc;
+      }
+      run_sigma(*plan, key_c, dpsi, len, reinterpret_cast<double2*>(sigma),
                 reinterpret_cast<double2*>(sigma_dagger), stream);
     }
     double2 h;

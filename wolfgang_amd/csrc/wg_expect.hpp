@@ -76,6 +76,8 @@ struct ExpectationPlan {
   u64* d_key_z = nullptr;            // keys in group order
   int* d_key_y = nullptr;            // |x & z| mod 4 per key
   double2* d_key_c = nullptr;        // coefficient sum per key (terms in order)
+  u32* d_key_term_begin = nullptr;   // [num_keys + 1] CSR over keys in group order
+  u32* d_key_terms = nullptr;        // [T] terms of each key, in term order
   ~ExpectationPlan();
 };
 

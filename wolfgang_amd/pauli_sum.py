@@ -576,13 +576,13 @@ class DevicePauliSum:
 
     def expectation_adjoint(
         self, psi_ptr: int, length: int, term_values_ptr: int | None, sigma_ptr: int | None,
-        sigma_dagger_ptr: int | None, stream: Any = None,
+        sigma_dagger_ptr: int | None, stream: Any = None, coeffs_ptr: int | None = None,
     ) -> complex:
         """Raw device-pointer forward + adjoint (see include/wolfgang_amd.h)."""
         out = (C.c_double * 2)()
         _native.call(
-            "wg_expectation_adjoint", self._handle(), psi_ptr, length, term_values_ptr, sigma_ptr,
-            sigma_dagger_ptr, _stream_arg(stream), out,
+            "wg_expectation_adjoint", self._handle(), psi_ptr, length, coeffs_ptr, term_values_ptr,
+            sigma_ptr, sigma_dagger_ptr, _stream_arg(stream), out,
         )
         return complex(out[0], out[1])
 
