@@ -152,6 +152,26 @@ int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t len
                            const double* coeffs, double* term_values, double* sigma,
                            double* sigma_dagger, void* stream, double* out2);
 
+/* Batched forward + adjoint over `count` (psi, H) instances on one device (no
+ * reference counterpart). Per instance: psi[i] (2^n_i complex128, device),
+ * optional coeffs[i] (num_terms complex128, device; NULL = the operator's own),
+ * optional outputs term_values[i], sigma[i] = H psi, sigma_dagger[i] =
+ * H^dagger psi; energies holds 2*count doubles (device) for E_i. All pointers
+ * are fixed at creation: the work is cut into units served by one persistent
+ * kernel from a device work queue and the launch sequence is captured into a
+ * HIP graph. tunables = {log2 forward chunk, sigma outputs per unit,
+ * persistent grid} or NULL to pick them by timing the batch. */
+typedef struct wg_batch wg_batch;
+int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* const* psi,
+                    const double* const* coeffs, double* const* term_values, double* const* sigma,
+                    double* const* sigma_dagger, double* energies, const int* tunables,
+                    void* stream, wg_batch** out);
+/* Enqueues one graph replay on `stream` (does not synchronize). */
+int wg_batch_run(wg_batch* batch, void* stream);
+int wg_batch_sync(wg_batch* batch, void* stream);
+int wg_batch_info(const wg_batch* batch, int* tunables, size_t* units, size_t* items);
+void wg_batch_destroy(wg_batch* batch);
+
 /* Expectation plan summary for an operator (built and cached on first use;
  * no reference counterpart): distinct (x, z) keys, distinct x-masks counting
  * the diagonal group (G_x), sweeps over psi per evaluation (tiled passes, or

@@ -13,10 +13,12 @@ from .pauli_sum import (
     checked_commutation_entries,
     checked_matmul_intermediate_terms,
     cuda_available,
+    cuda_devices,
     hip_available,
     hip_devices,
     hip_status,
     metal_available,
+    metal_devices,
 )
 
 __all__ = [
@@ -27,7 +29,9 @@ __all__ = [
     "hip_devices",
     "hip_status",
     "cuda_available",
+    "cuda_devices",
     "metal_available",
+    "metal_devices",
     "checked_matmul_intermediate_terms",
     "checked_commutation_entries",
 ]

@@ -750,6 +750,13 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   plan->num_keys = K;
   plan->num_groups = gx.size();
   plan->distinct_x = gx.size();
+  plan->direct_items = items;
+  plan->h_group_x = gx;
+  plan->h_group_begin = group_begin;
+  plan->h_key_z = flat_z;
+  plan->h_key_y = flat_y;
+  plan->h_key_c = flat_c;
+  plan->h_term_key = term_key;
   plan->tiled = plan->n >= kTiledMinQubits && !items.empty();
   std::vector<u32> offset, count;
   if (plan->tiled) {

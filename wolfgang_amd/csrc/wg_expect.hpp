@@ -78,6 +78,14 @@ struct ExpectationPlan {
   double2* d_key_c = nullptr;        // coefficient sum per key (terms in order)
   u32* d_key_term_begin = nullptr;   // [num_keys + 1] CSR over keys in group order
   u32* d_key_terms = nullptr;        // [T] terms of each key, in term order
+  // Host copies used to assemble batched launches.
+  std::vector<ExpItem> direct_items;  // items before tiling (direct-sweep form)
+  std::vector<u64> h_group_x;
+  std::vector<u32> h_group_begin;
+  std::vector<u64> h_key_z;
+  std::vector<int> h_key_y;
+  std::vector<double2> h_key_c;
+  std::vector<u32> h_term_key;
   ~ExpectationPlan();
 };
 

@@ -813,5 +813,13 @@ def cuda_available() -> bool:
     return False
 
 
+def cuda_devices() -> list[dict[str, Any]]:
+    return []
+
+
 def metal_available() -> bool:
     return False
+
+
+def metal_devices() -> list[dict[str, Any]]:
+    return []
