@@ -137,9 +137,16 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
 
 // ------------------------------------------------------------ tiled sweep
 
-__device__ inline u64 tile_rep(u64 t, const PassDesc& pd) {
+// Coset representative of tile t: the bits of t deposited into the free
+// positions (ascending), i.e. a software pdep over the pass's free mask.
+__device__ inline u64 tile_rep(u64 t, u64 free_mask) {
   u64 rep = 0;
-  for (int j = 0; j < pd.nfree; ++j) rep |= ((t >> j) & 1ULL) << pd.free_pos[j];
+  while (free_mask) {
+    const u64 low = free_mask & (0 - free_mask);
+    if (t & 1ULL) rep |= low;
+    t >>= 1;
+    free_mask ^= low;
+  }
   return rep;
 }
 
@@ -148,7 +155,8 @@ __device__ inline u64 tile_rep(u64 t, const PassDesc& pd) {
 // linear id: they are dispatched to the same XCD close together, so the later
 // reads of a tile hit that XCD's L2 instead of HBM.
 __global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
-    const double2* __restrict__ psi, const PassDesc pd, const ExpItem* __restrict__ items,
+    const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
+    const ExpItem* __restrict__ items,
     u32 num_items, u32 item_groups, u32 tile_blocks, u32 partial_base,
     double2* __restrict__ partials) {
   extern __shared__ double2 tile[];
@@ -166,12 +174,15 @@ __global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
   const u32 xl = static_cast<u32>(it.xl);
   const u32 pbit = static_cast<u32>(it.pbit);
 
-  const int k = pd.k;
-  const int l = pd.l;
-  const u32 lmask = (1U << l) - 1U;
+  const int k = pd->k;
+  const u64 free_mask = pd->free_mask;
+  constexpr u32 lmask = (1U << kTileLowLog) - 1U;
   const u32 tile_size = 1U << k;
-  const u64 ntiles = 1ULL << (pd.n - k);
+  const u64 ntiles = 1ULL << (pd->n - k);
   constexpr int kLoads = (1 << kTileMaxLog) / kTileBlock;
+  // Thread t loads points u = t + i*kTileBlock; their segment index
+  // u >> kTileLowLog = wave + i*kSegStride is wave-uniform (scalar loads).
+  constexpr u32 kSegStride = kTileBlock >> kTileLowLog;
 
   double ar[kZ], ai[kZ];
 #pragma unroll
@@ -180,70 +191,115 @@ __global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
     ai[j] = 0.0;
   }
 
-  double2 stage[kLoads];
-  u64 t = tb;
-  if (t < ntiles) {
-    const u64 rep = tile_rep(t, pd);
+  // The lane's points in a tile are u = f ^ g(s), s = 0..steps-1, with f the
+  // lane's own bits and g(s) wave-uniform (both through the pivot-bit insertion,
+  // which is linear over GF(2)). Hence parity(zl & u) = parity(zl & f) ^
+  // parity(zl & g(s)): the first term is a per-lane sign applied once after the
+  // sweep, the second a wave-uniform bit, so each (point, key) update is one
+  // fused multiply-add by a uniform +-1 per component.
+  const u32 span = xl ? (tile_size >> 1) : tile_size;
+  const u32 steps = span > 64 ? span >> 6 : 1;
+  const bool lane_on = lane < span;
+  const u32 f_lane = xl ? static_cast<u32>(insert_zero(lane, pbit)) : lane;
+  u64 smask[kZ];
+  u32 lane_par = 0;
 #pragma unroll
-    for (int i = 0; i < kLoads; ++i) {
-      const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
-      if (u < tile_size) stage[i] = psi[(rep ^ pd.seg_off[u >> l]) | (u & lmask)];
+  for (int j = 0; j < kZ; ++j) {
+    smask[j] = 0;
+    if (j < nz) {
+      const u32 zlj = static_cast<u32>(it.zl[j]);
+      lane_par |= static_cast<u32>(__popc(zlj & f_lane) & 1) << j;
+      u64 m = 0;
+      for (u32 st = 0; st < steps; ++st) {
+        const u32 g = xl ? static_cast<u32>(insert_zero(st << 6, pbit)) : (st << 6);
+        m |= static_cast<u64>(__popc(zlj & g) & 1) << st;
+      }
+      smask[j] = m;
     }
   }
+
+  // Tiles smaller than kLoads * kTileBlock points wrap (u & (tile_size - 1)):
+  // several threads then load and store the same point with the same value,
+  // which keeps the staging loops branch-free and the staging in registers.
+  const u32 tile_mask = tile_size - 1;
+  const u32 seg_mask = (tile_size >> kTileLowLog) - 1;
+  static_assert(kLoads == 8, "staging below is written for 8 points per thread");
+  double2 s0, s1, s2, s3, s4, s5, s6, s7;
+  auto fetch = [&](u64 rep) {
+    const double2* base = psi;
+    const u32 lo = threadIdx.x & lmask;
+#define WG_FETCH(reg, i) \
+    reg = base[(rep ^ pd->seg_off[(wave + (i) * kSegStride) & seg_mask]) | lo];
+    WG_FETCH(s0, 0) WG_FETCH(s1, 1) WG_FETCH(s2, 2) WG_FETCH(s3, 3)
+    WG_FETCH(s4, 4) WG_FETCH(s5, 5) WG_FETCH(s6, 6) WG_FETCH(s7, 7)
+#undef WG_FETCH
+  };
+  u64 t = tb;
+  if (t < ntiles) fetch(tile_rep(t, free_mask));
   for (; t < ntiles; t += tile_blocks) {
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < kLoads; ++i) {
-      const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
-      if (u < tile_size) tile[u] = stage[i];
+    {
+      const u32 u = threadIdx.x;
+      tile[(u + 0 * kTileBlock) & tile_mask] = s0;
+      tile[(u + 1 * kTileBlock) & tile_mask] = s1;
+      tile[(u + 2 * kTileBlock) & tile_mask] = s2;
+      tile[(u + 3 * kTileBlock) & tile_mask] = s3;
+      tile[(u + 4 * kTileBlock) & tile_mask] = s4;
+      tile[(u + 5 * kTileBlock) & tile_mask] = s5;
+      tile[(u + 6 * kTileBlock) & tile_mask] = s6;
+      tile[(u + 7 * kTileBlock) & tile_mask] = s7;
     }
     __syncthreads();
-    const u64 rep = tile_rep(t, pd);
+    const u64 rep = tile_rep(t, free_mask);
     const u64 tn = t + tile_blocks;
-    if (tn < ntiles) {
-      // Prefetch the next tile into registers while this one is consumed.
-      const u64 rep_next = tile_rep(tn, pd);
-#pragma unroll
-      for (int i = 0; i < kLoads; ++i) {
-        const u32 u = threadIdx.x + static_cast<u32>(i) * kTileBlock;
-        if (u < tile_size) stage[i] = psi[(rep_next ^ pd.seg_off[u >> l]) | (u & lmask)];
-      }
-    }
-    if (nz == 0) continue;
-    // Sign of the coset representative per key (wave-uniform).
-    u32 srep = 0;
+    // Prefetch the next tile into registers while this one is consumed.
+    if (tn < ntiles) fetch(tile_rep(tn, free_mask));
+    if (nz == 0 || !lane_on) continue;
+    // Per-tile uniform sign masks: bit s of sm[j] is the sign of step s.
+    u64 sm[kZ];
 #pragma unroll
     for (int j = 0; j < kZ; ++j) {
-      if (j < nz) srep |= static_cast<u32>(__popcll(it.z[j] & rep) & 1) << j;
+      sm[j] = (j < nz && (__popcll(it.z[j] & rep) & 1)) ? ~smask[j] : smask[j];
     }
     if (xl) {
-      for (u32 v = lane; v < (tile_size >> 1); v += 64) {
-        const u32 u = static_cast<u32>(insert_zero(v, pbit));
+#pragma unroll 4
+      for (u32 st = 0; st < steps; ++st) {
+        const u32 u = f_lane ^ static_cast<u32>(insert_zero(st << 6, pbit));
         const double2 a = tile[u];
-        const double2 s = tile[u ^ xl];
-        const double re = fma(s.x, a.x, s.y * a.y);
-        const double im = fma(s.x, a.y, -(s.y * a.x));
+        const double2 b = tile[u ^ xl];
+        const double re = fma(b.x, a.x, b.y * a.y);
+        const double im = fma(b.x, a.y, -(b.y * a.x));
 #pragma unroll
         for (int j = 0; j < kZ; ++j) {
           if (j < nz) {
-            const bool odd = (__popc(static_cast<u32>(it.zl[j]) & u) ^ (srep >> j)) & 1;
-            ar[j] += odd ? -re : re;
-            ai[j] += odd ? -im : im;
+            const double sg = ((sm[j] >> st) & 1ULL) ? -1.0 : 1.0;
+            ar[j] = fma(sg, re, ar[j]);
+            ai[j] = fma(sg, im, ai[j]);
           }
         }
       }
     } else {
-      for (u32 u = lane; u < tile_size; u += 64) {
-        const double2 a = tile[u];
+#pragma unroll 4
+      for (u32 st = 0; st < steps; ++st) {
+        const double2 a = tile[f_lane ^ (st << 6)];
         const double p = fma(a.x, a.x, a.y * a.y);
 #pragma unroll
         for (int j = 0; j < kZ; ++j) {
           if (j < nz) {
-            const bool odd = (__popc(static_cast<u32>(it.zl[j]) & u) ^ (srep >> j)) & 1;
-            ar[j] += odd ? -p : p;
+            const double sg = ((sm[j] >> st) & 1ULL) ? -1.0 : 1.0;
+            ar[j] = fma(sg, p, ar[j]);
           }
         }
       }
+    }
+  }
+  // Per-lane sign, then the wave-level reduction; each wave owns its item's
+  // partial row for this tile block.
+#pragma unroll
+  for (int j = 0; j < kZ; ++j) {
+    if ((lane_par >> j) & 1U) {
+      ar[j] = -ar[j];
+      ai[j] = -ai[j];
     }
   }
   if (!active) return;
@@ -567,8 +623,12 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
     u64 occupied = sp.coord;
     for (u64 g : sp.gens) occupied |= 1ULL << Span::top(g);
     d.nfree = 0;
+    d.free_mask = 0;
     for (int q = 0; q < static_cast<int>(n); ++q) {
-      if (!((occupied >> q) & 1ULL)) d.free_pos[d.nfree++] = static_cast<unsigned char>(q);
+      if (!((occupied >> q) & 1ULL)) {
+        d.free_pos[d.nfree++] = static_cast<unsigned char>(q);
+        d.free_mask |= 1ULL << q;
+      }
     }
     for (int s = 0; s < (1 << (a + m)); ++s) {
       u64 off = 0;
@@ -642,6 +702,7 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_key_y);
   device_free(d_key_c);
   device_free(d_key_term_begin);
+  device_free(d_passes);
   device_free(d_key_terms);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
@@ -808,6 +869,11 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   upload(plan->d_key_y, flat_y);
   upload(plan->d_key_c, flat_c);
   upload(plan->d_key_term_begin, key_term_begin);
+  {
+    std::vector<PassDesc> descs;
+    for (const PassLaunch& p : plan->passes) descs.push_back(p.desc);
+    upload(plan->d_passes, descs);
+  }
   upload(plan->d_key_terms, key_terms);
   sync(stream);
   return plan;
@@ -838,8 +904,8 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
         const std::size_t lds = (std::size_t{1} << p.desc.k) * sizeof(double2);
         ProfileScope prof("expect_tiled", stream);
         expect_tiled_kernel<<<grid, kTileBlock, lds, stream>>>(
-            psi, p.desc, plan.d_items + p.first_item, p.num_items, p.item_groups, p.tile_blocks,
-            p.partial_base, partials);
+            psi, plan.d_passes + (&p - plan.passes.data()), plan.d_items + p.first_item,
+            p.num_items, p.item_groups, p.tile_blocks, p.partial_base, partials);
         check_launch("expect_tiled");
       }
     } else {
@@ -1068,25 +1134,37 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
         }
         for (u32 i = 0; i < p.num_items; ++i) {
           const ExpItem& it = items[p.first_item + i];
+          const u32 xl = static_cast<u32>(it.xl), pbit = static_cast<u32>(it.pbit);
+          auto ins0 = [&](u32 v) {
+            const u32 low = pbit - 1;
+            return ((v & ~low) << 1) | (v & low);
+          };
+          const u32 span = xl ? tile_size / 2 : tile_size;
+          const u32 steps = span > 64 ? span / 64 : 1;
           for (int j = 0; j < it.nz; ++j) {
+            const u32 zlj = static_cast<u32>(it.zl[j]);
             const u32 srep = __builtin_popcountll(it.z[j] & rep) & 1;
-            double2& a = acc[static_cast<std::size_t>(i) * kZ + j];
-            if (it.xl) {
-              for (u32 v = 0; v < tile_size / 2; ++v) {
-                const u32 low = static_cast<u32>(it.pbit) - 1;
-                const u32 u = ((v & ~low) << 1) | (v & low);
-                const double2 A = tile[u], S = tile[u ^ static_cast<u32>(it.xl)];
-                const double re = S.x * A.x + S.y * A.y, im = S.x * A.y - S.y * A.x;
-                const bool odd = ((__builtin_popcount(static_cast<u32>(it.zl[j]) & u)) ^ srep) & 1;
-                a.x += odd ? -re : re;
-                a.y += odd ? -im : im;
+            double2& acc2 = acc[static_cast<std::size_t>(i) * kZ + j];
+            // Same decomposition as the kernel: lane part + uniform step part.
+            for (u32 lane = 0; lane < 64 && lane < span; ++lane) {
+              const u32 f = xl ? ins0(lane) : lane;
+              const u32 lane_par = __builtin_popcount(zlj & f) & 1;
+              double lr = 0.0, li = 0.0;
+              for (u32 st = 0; st < steps; ++st) {
+                const u32 g = xl ? ins0(st << 6) : (st << 6);
+                const u32 u = f ^ g;
+                const u32 sbit = (__builtin_popcount(zlj & g) & 1) ^ srep;
+                const double sg = sbit ? -1.0 : 1.0;
+                if (xl) {
+                  const double2 A = tile[u], B = tile[u ^ xl];
+                  lr += sg * (B.x * A.x + B.y * A.y);
+                  li += sg * (B.x * A.y - B.y * A.x);
+                } else {
+                  lr += sg * (tile[u].x * tile[u].x + tile[u].y * tile[u].y);
+                }
               }
-            } else {
-              for (u32 u = 0; u < tile_size; ++u) {
-                const double pr = tile[u].x * tile[u].x + tile[u].y * tile[u].y;
-                const bool odd = ((__builtin_popcount(static_cast<u32>(it.zl[j]) & u)) ^ srep) & 1;
-                a.x += odd ? -pr : pr;
-              }
+              acc2.x += lane_par ? -lr : lr;
+              acc2.y += lane_par ? -li : li;
             }
           }
         }

@@ -41,6 +41,7 @@ struct PassDesc {
   int l;
   int k;
   int nfree;
+  u64 free_mask;  // index bits not spanned by the tile (coset representative bits)
   u64 seg_off[kMaxSegments];
   unsigned char free_pos[64];
 };
@@ -78,6 +79,7 @@ struct ExpectationPlan {
   double2* d_key_c = nullptr;        // coefficient sum per key (terms in order)
   u32* d_key_term_begin = nullptr;   // [num_keys + 1] CSR over keys in group order
   u32* d_key_terms = nullptr;        // [T] terms of each key, in term order
+  PassDesc* d_passes = nullptr;      // [passes] descriptors read by the tiled kernel
   // Host copies used to assemble batched launches.
   std::vector<ExpItem> direct_items;  // items before tiling (direct-sweep form)
   std::vector<u64> h_group_x;
