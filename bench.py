@@ -45,6 +45,14 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--psi-seed", type=int, default=3002)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-terms", type=int, default=8)
+    p.add_argument(
+        "--workload",
+        default="expectation",
+        choices=["expectation", "commutation", "matmul", "batch", "copy"],
+        help="expectation (the headline line, default); commutation / matmul: the 128-qubit "
+        "algebra workloads; batch: batched forward+adjoint vs the per-instance path; copy: a "
+        "plain device copy of known size, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE",
+    )
     return p.parse_args()
 
 
@@ -91,6 +99,22 @@ def cpu_baseline(op, psi_dev, sample_terms: int) -> dict:
     }
 
 
+def copy_calibration(args) -> int:
+    """A plain 4 GiB device-to-device copy (torch's copy kernel): the reference
+    point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.complex128, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.steps)):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": src.numel() * 16,
+                      "bytes_written_per_launch": dst.numel() * 16, "launches": max(1, args.steps)}))
+    return 0
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -100,6 +124,12 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if args.workload == "copy":
+        return copy_calibration(args)
+    if args.workload != "expectation":
+        from wolfgang_amd import bench_workloads
+
+        return bench_workloads.run(args)
 
     import torch
 

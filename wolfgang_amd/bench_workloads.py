@@ -1,0 +1,144 @@
+"""Secondary benchmark workloads for bench.py (one JSON line each).
+
+* commutation: 128-qubit, 65,536-term operator (each qubit non-identity w.p.
+  0.25) against itself -> 65,536^2 packed commutation matrix on the device.
+* matmul: two 8,192-term 128-qubit operators -> 67,108,864 products, then the
+  sort-reduce simplify (atol 1e-12).
+* batch: 256 (statevector, Hamiltonian) pairs, n ~ U{12..22}, terms
+  log-uniform in [100, 50,000] (random masks, density 0.25, complex
+  coefficients): forward + adjoint in one graph replay, against the
+  per-instance path on the same buffers.
+"""
+
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+
+
+def _timed(fn, steps: int, warmup: int) -> float:
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def commutation(args) -> dict:
+    from . import _native
+    from .workloads import random_pauli_masks
+
+    n, T = 128, 65_536
+    op = random_pauli_masks(n, T, 0.25, 2001)
+    d = op.to_device()
+    m = d.commutes_with_device(d, max_commutation_matrix_entries=T * T)
+    _native.profile_enable(True)
+    _native.profile_reset()
+    dt = _timed(lambda: d.commutes_with_device(d, T * T, output=m), args.steps, args.warmup)
+    ms, launches = _native.profile_get("commute_packed")
+    _native.profile_enable(False)
+    kernel_s = ms / 1000.0 / max(1, launches)
+    alg_bytes = 2 * (T * 2 * 2 * 8) + T * T // 8
+    count = m.count_commuting()
+    return {
+        "metric": "commutation pairs/s (65,536^2, 128 qubits, packed bit output)",
+        "value": T * T / dt,
+        "unit": "pairs/s",
+        "ms_per_step": dt * 1000,
+        "kernel_ms": kernel_s * 1000,
+        "config": {"workload": "commutation", "num_qubits": n, "num_terms": T, "seed": 2001,
+                   "commuting_count": count},
+        "roofline": {"bound": "integer VALU (HBM floor shown)", "achieved": alg_bytes / kernel_s / 1e9,
+                     "peak": 8000.0, "unit": "GB/s", "frac": alg_bytes / kernel_s / 1e9 / 8000.0,
+                     "algorithmic_bytes": alg_bytes},
+    }
+
+
+def matmul(args) -> dict:
+    from . import _native
+    from .workloads import random_pauli_masks
+
+    n, T = 128, 8_192
+    a = random_pauli_masks(n, T, 0.25, 2002).to_device()
+    b = random_pauli_masks(n, T, 0.25, 2003).to_device()
+    P = T * T
+    _native.profile_enable(True)
+    _native.profile_reset()
+    out = {}
+
+    def step():
+        out["r"] = a.matmul(b, simplify=True, max_intermediate_terms=P)
+
+    dt = _timed(step, args.steps, args.warmup)
+    ms_prod, lp = _native.profile_get("product")
+    ms_sort, ls = _native.profile_get("radix_scatter")
+    _native.profile_enable(False)
+    prod_s = ms_prod / 1000.0 / max(1, lp)
+    write_bytes = P * (2 * 2 * 8 + 16)
+    return {
+        "metric": "term-pair products/s with simplify (8,192 x 8,192, 128 qubits)",
+        "value": P / dt,
+        "unit": "products/s",
+        "ms_per_step": dt * 1000,
+        "config": {"workload": "matmul+simplify", "num_qubits": n, "lhs_terms": T, "rhs_terms": T,
+                   "products": P, "output_terms": out["r"].num_terms},
+        "product_kernel_ms": prod_s * 1000,
+        "radix_scatter_ms_per_step": ms_sort / max(1, args.steps + args.warmup),
+        "roofline": {"bound": "hbm", "kernel": "product", "achieved": write_bytes / prod_s / 1e9,
+                     "peak": 8000.0, "unit": "GB/s", "frac": write_bytes / prod_s / 1e9 / 8000.0},
+    }
+
+
+def batch(args) -> dict:
+    import torch
+
+    from .batch import BatchedExpectation, plain_forward_adjoint
+    from .workloads import batch_shapes, random_pauli_masks
+
+    shapes = batch_shapes(256, 5001)
+    ops, psis = [], []
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(5002)
+    for i, (n, T) in enumerate(shapes):
+        ops.append(random_pauli_masks(n, T, 0.25, 5003 + i).to_device())
+        p = torch.randn(1 << n, dtype=torch.complex128, device="cuda:0", generator=g)
+        psis.append(p / torch.linalg.vector_norm(p))
+    t0 = time.perf_counter()
+    b = BatchedExpectation(ops, psis, sigma=True, sigma_dagger=True)
+    setup_s = time.perf_counter() - t0
+    batched = _timed(b.run, args.steps, args.warmup)
+    e_batched = b.energies.clone()
+    plain = _timed(lambda: plain_forward_adjoint(b), max(1, args.steps // 2), 1)
+    rel = float(torch.max(torch.abs(b.energies - e_batched)).item())
+    pairs = sum(T * (1 << n) for n, T in shapes)
+    return {
+        "metric": "batched fwd+adjoint speedup vs plain per-instance GPU path",
+        "value": plain / batched,
+        "unit": "x",
+        "batched_ms": batched * 1000,
+        "plain_ms": plain * 1000,
+        "term_amp_pairs_per_s_batched": pairs / batched,
+        "max_abs_energy_difference": rel,
+        "setup_s": setup_s,
+        "tunables": b.tunables,
+        "units": b.units,
+        "config": {"workload": "batch", "instances": len(shapes), "seed": 5001,
+                   "qubits": [12, 22], "terms": [100, 50000],
+                   "sum_terms": sum(T for _, T in shapes)},
+    }
+
+
+def run(args) -> int:
+    fn = {"commutation": commutation, "matmul": matmul, "batch": batch}[args.workload]
+    line = fn(args)
+    line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+                 "data": "synthetic"})
+    print(json.dumps(line), flush=True)
+    return 0

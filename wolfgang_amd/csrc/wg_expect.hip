@@ -48,8 +48,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTileBlock = 512;
 constexpr int kTileWaves = kTileBlock / 64;
-// States with at least this many qubits use the tiled sweep.
+// States with at least this many qubits use the tiled sweep, unless the plan
+// needs more passes (launches) than this.
 constexpr std::size_t kTiledMinQubits = 18;
+constexpr std::size_t kMaxTiledPasses = 256;
 
 __device__ inline u64 insert_zero(u64 v, u64 pivot) {
   const u64 low = pivot - 1;
@@ -533,7 +535,7 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
   std::size_t remaining = xs.size();
   std::vector<Span> spans;
   std::vector<std::vector<u64>> pass_xs;
-  const bool structured = xs.size() <= 4096;
+  const bool structured = xs.size() <= 1024;
   std::size_t cursor = 0;
   while (remaining > 0 || spans.empty()) {
     Span sp;
@@ -586,8 +588,13 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
         }
       }
     }
+    // Membership scan: all x-groups for structured sets; for unstructured
+    // sets only a window after the generators (the span rarely contains more),
+    // which keeps planning linear in the number of x-groups.
     std::vector<u64> in_pass;
-    for (std::size_t j = 0; j < xs.size(); ++j) {
+    const std::size_t scan_end =
+        structured ? xs.size() : std::min<std::size_t>(xs.size(), cursor + 4096);
+    for (std::size_t j = 0; j < scan_end; ++j) {
       if (!covered[j] && sp.contains(xs[j])) {
         covered[j] = 1;
         --remaining;
@@ -821,7 +828,17 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   plan->tiled = plan->n >= kTiledMinQubits && !items.empty();
   std::vector<u32> offset, count;
   if (plan->tiled) {
-    plan_tiled_passes(plan->n, items, plan->passes);
+    std::vector<ExpItem> tiled_items = items;
+    std::vector<PassLaunch> passes;
+    plan_tiled_passes(plan->n, tiled_items, passes);
+    if (passes.size() > kMaxTiledPasses) {
+      plan->tiled = false;  // launch-bound: one launch per pass would dominate
+    } else {
+      items = std::move(tiled_items);
+      plan->passes = std::move(passes);
+    }
+  }
+  if (plan->tiled) {
     offset.resize(items.size());
     count.resize(items.size());
     std::size_t slot = 0;
