@@ -26,9 +26,10 @@
 //     instead of (#x-groups) x |psi|. Passes are chosen greedily on the host.
 //
 // Adjoint: sigma = H psi = dE/dpsi* with (P psi)[b ^ x] = i^{|x&z|}
-// (-1)^{|z&b|} psi[b]; each output amplitude is owned by one thread which
-// gathers psi[o ^ x] once per x-group and applies all of the group's keys.
-// dE/dc_k = <P_k> comes from the forward plan.
+// (-1)^{|z&b|} psi[b], and dE/dc_k = <P_k>, in one fused sweep: each thread
+// owns a few output amplitudes, gathers psi[o ^ x] once per x-group and applies
+// all of the group's keys to sigma, sigma^dagger and the per-key partial
+// sums <P> = sum_o conj(psi[o]) (P psi)[o].
 
 #include <hip/hip_runtime.h>
 
@@ -411,32 +412,113 @@ __device__ inline double2 times_i_pow(double2 v, int k) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void sigma_direct_kernel(
+// Fused adjoint sweep: sigma = H psi, sigma_dag = H^dagger psi and, per key,
+// <P> = sum_o conj(psi[o]) (P psi)[o], in one pass. A block owns chunks of
+// kFuseOut * kFuseBlock consecutive outputs (grid-stride); for each x-group it
+// gathers psi[o ^ x] once and applies all the group's keys. Per-key partial
+// sums are reduced over the block every kFuseChunk keys and accumulated into
+// the block's own row of `partials` (row = block, one entry per key in group
+// order), so no two blocks write the same entry and the order is fixed.
+constexpr int kFuseBlock = 256;
+constexpr int kFuseOut = 8;
+constexpr int kFuseChunk = 32;
+
+__global__ __launch_bounds__(kFuseBlock) void adjoint_fused_kernel(
     const double2* __restrict__ psi, u64 len, const u64* __restrict__ group_x,
     const u32* __restrict__ group_begin, u32 num_groups, const u64* __restrict__ key_z,
-    const int* __restrict__ key_y, const double2* __restrict__ key_c, double2* __restrict__ sigma,
-    double2* __restrict__ sigma_dag) {
-  for (u64 o = static_cast<u64>(blockIdx.x) * kBlock + threadIdx.x; o < len;
-       o += static_cast<u64>(gridDim.x) * kBlock) {
-    double sr = 0.0, si = 0.0, dr = 0.0, di = 0.0;
+    const int* __restrict__ key_y, const double2* __restrict__ key_c, u32 num_keys,
+    double2* __restrict__ sigma, double2* __restrict__ sigma_dag, double2* __restrict__ partials) {
+  __shared__ double2 red[kFuseBlock / 64][kFuseChunk];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
+  const u64 nchunks = (len + chunk_out - 1) / chunk_out;
+  double2* row = partials + static_cast<std::size_t>(blockIdx.x) * num_keys;
+  bool first = true;
+  for (u64 ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const u64 base = ch * chunk_out;
+    double2 pv[kFuseOut], sg[kFuseOut], sd[kFuseOut];
+#pragma unroll
+    for (int i = 0; i < kFuseOut; ++i) {
+      const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
+      pv[i] = o < len ? psi[o] : make_double2(0.0, 0.0);
+      sg[i] = make_double2(0.0, 0.0);
+      sd[i] = make_double2(0.0, 0.0);
+    }
     for (u32 g = 0; g < num_groups; ++g) {
-      const u64 src = o ^ group_x[g];
-      const double2 t = psi[src];
+      const u64 x = group_x[g];
+      double2 tv[kFuseOut];
+#pragma unroll
+      for (int i = 0; i < kFuseOut; ++i) {
+        const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
+        tv[i] = o < len ? psi[o ^ x] : make_double2(0.0, 0.0);
+      }
       const u32 kb = group_begin[g], ke = group_begin[g + 1];
       for (u32 k = kb; k < ke; ++k) {
-        double2 w = times_i_pow(t, key_y[k]);
-        if (__popcll(key_z[k] & src) & 1) w = make_double2(-w.x, -w.y);
+        const u64 z = key_z[k];
+        const int y = key_y[k];
         const double2 c = key_c[k];
-        // sigma += c w ; sigma_dag += conj(c) w
-        sr = fma(c.x, w.x, fma(-c.y, w.y, sr));
-        si = fma(c.x, w.y, fma(c.y, w.x, si));
-        dr = fma(c.x, w.x, fma(c.y, w.y, dr));
-        di = fma(c.x, w.y, fma(-c.y, w.x, di));
+        double pr = 0.0, pi = 0.0;
+#pragma unroll
+        for (int i = 0; i < kFuseOut; ++i) {
+          const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
+          double2 w = times_i_pow(tv[i], y);
+          if (__popcll(z & (o ^ x)) & 1) w = make_double2(-w.x, -w.y);
+          sg[i].x = fma(c.x, w.x, fma(-c.y, w.y, sg[i].x));
+          sg[i].y = fma(c.x, w.y, fma(c.y, w.x, sg[i].y));
+          sd[i].x = fma(c.x, w.x, fma(c.y, w.y, sd[i].x));
+          sd[i].y = fma(c.x, w.y, fma(-c.y, w.x, sd[i].y));
+          // conj(psi[o]) * w
+          pr = fma(pv[i].x, w.x, fma(pv[i].y, w.y, pr));
+          pi = fma(pv[i].x, w.y, fma(-pv[i].y, w.x, pi));
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+          pr += __shfl_xor(pr, d, 64);
+          pi += __shfl_xor(pi, d, 64);
+        }
+        const u32 slot = k % kFuseChunk;
+        if (lane == 0) red[wave][slot] = make_double2(pr, pi);
+        if (slot == kFuseChunk - 1 || k + 1 == num_keys) {
+          __syncthreads();
+          const u32 k0 = k - slot;
+          if (threadIdx.x <= slot) {
+            double2 acc = first ? make_double2(0.0, 0.0) : row[k0 + threadIdx.x];
+            for (int w2 = 0; w2 < kFuseBlock / 64; ++w2) {
+              acc.x += red[w2][threadIdx.x].x;
+              acc.y += red[w2][threadIdx.x].y;
+            }
+            row[k0 + threadIdx.x] = acc;
+          }
+          __syncthreads();
+        }
       }
     }
-    if (sigma) sigma[o] = make_double2(sr, si);
-    if (sigma_dag) sigma_dag[o] = make_double2(dr, di);
+#pragma unroll
+    for (int i = 0; i < kFuseOut; ++i) {
+      const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
+      if (o < len) {
+        if (sigma) sigma[o] = sg[i];
+        if (sigma_dag) sigma_dag[o] = sd[i];
+      }
+    }
+    first = false;
   }
+  if (first) {
+    // A block with no chunk still owns a row: clear it.
+    for (u32 k = threadIdx.x; k < num_keys; k += kFuseBlock) row[k] = make_double2(0.0, 0.0);
+  }
+}
+
+// Per-key values from the fused sweep's per-block rows, summed in block order;
+// flat (group-order) key k maps to distinct-key id flat_key[k].
+__global__ __launch_bounds__(256) void fused_reduce_kernel(const double2* __restrict__ partials,
+                                                           u32 rows, u32 num_keys,
+                                                           const u32* __restrict__ flat_key,
+                                                           double* __restrict__ key_vals) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_keys) return;
+  double r = 0.0;
+  for (u32 b = 0; b < rows; ++b) r += partials[static_cast<std::size_t>(b) * num_keys + k].x;
+  key_vals[flat_key[k]] = r;
 }
 
 // Per-key coefficient sums from per-term coefficients, each key's terms in term
@@ -710,6 +792,7 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_key_c);
   device_free(d_key_term_begin);
   device_free(d_passes);
+  device_free(d_flat_key);
   device_free(d_key_terms);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
@@ -807,6 +890,8 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
       for (u32 k : gkeys[g]) flat_pos[k] = pos++;
     }
   }
+  std::vector<u32> flat_key(K);
+  for (std::size_t k = 0; k < K; ++k) flat_key[flat_pos[k]] = static_cast<u32>(k);
   std::vector<u32> key_term_begin(K + 1, 0), key_terms(T);
   for (std::size_t t = 0; t < T; ++t) key_term_begin[flat_pos[term_key[t]] + 1]++;
   for (std::size_t k = 0; k < K; ++k) key_term_begin[k + 1] += key_term_begin[k];
@@ -886,6 +971,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   upload(plan->d_key_y, flat_y);
   upload(plan->d_key_c, flat_c);
   upload(plan->d_key_term_begin, key_term_begin);
+  upload(plan->d_flat_key, flat_key);
   {
     std::vector<PassDesc> descs;
     for (const PassLaunch& p : plan->passes) descs.push_back(p.desc);
@@ -940,22 +1026,6 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
   expect_finalize_kernel<<<1, 1024, 0, stream>>>(coeffs, plan.d_term_key, plan.T, key_vals,
                                                  term_values, out_dev);
   check_launch("expect_finalize");
-}
-
-void run_sigma(const ExpectationPlan& plan, const double2* key_c, const double2* psi,
-               std::size_t len, double2* sigma, double2* sigma_dag, hipStream_t stream) {
-  if (plan.num_groups == 0) {
-    if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * sizeof(double2), stream), "clear sigma");
-    if (sigma_dag) {
-      check_hip(hipMemsetAsync(sigma_dag, 0, len * sizeof(double2), stream), "clear sigma");
-    }
-    return;
-  }
-  ProfileScope prof("sigma_direct", stream);
-  sigma_direct_kernel<<<stride_grid(len, kBlock), kBlock, 0, stream>>>(
-      psi, len, plan.d_group_x, plan.d_group_begin, static_cast<u32>(plan.num_groups),
-      plan.d_key_z, plan.d_key_y, key_c, sigma, sigma_dag);
-  check_launch("sigma_direct");
 }
 
 std::size_t validate_psi(const wg_pauli_sum* op, const void* psi, std::size_t length) {
@@ -1048,8 +1118,8 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
     const double2* dpsi = reinterpret_cast<const double2*>(psi);
     double2* dout = scratch.take<double2>(1);
     const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
-    run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
     if (sigma || sigma_dagger) {
+      // One fused sweep: sigma, sigma^dagger and the per-key values.
       const double2* key_c = plan->d_key_c;
       if (coeffs) {
         double2* kc = scratch.take<double2>(plan->num_keys);
@@ -1058,8 +1128,28 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
         check_launch("key_coeff");
         key_c = kc;
       }
-      run_sigma(*plan, key_c, dpsi, len, reinterpret_cast<double2*>(sigma),
-                reinterpret_cast<double2*>(sigma_dagger), stream);
+      const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
+      const u64 nchunks = (len + chunk_out - 1) / chunk_out;
+      const u32 rows = static_cast<u32>(std::min<u64>(nchunks, 1024));
+      const u32 K = static_cast<u32>(plan->num_keys);
+      double2* rowbuf = scratch.take<double2>(checked_mul(rows, K, "adjoint partials"));
+      double* key_vals = scratch.take<double>(K);
+      {
+        ProfileScope prof("adjoint_fused", stream);
+        adjoint_fused_kernel<<<rows, kFuseBlock, 0, stream>>>(
+            dpsi, len, plan->d_group_x, plan->d_group_begin, static_cast<u32>(plan->num_groups),
+            plan->d_key_z, plan->d_key_y, key_c, K, reinterpret_cast<double2*>(sigma),
+            reinterpret_cast<double2*>(sigma_dagger), rowbuf);
+        check_launch("adjoint_fused");
+      }
+      fused_reduce_kernel<<<grid_for(K, 256), 256, 0, stream>>>(rowbuf, rows, K, plan->d_flat_key,
+                                                                key_vals);
+      check_launch("fused_reduce");
+      expect_finalize_kernel<<<1, 1024, 0, stream>>>(c, plan->d_term_key, plan->T, key_vals,
+                                                     reinterpret_cast<double2*>(term_values), dout);
+      check_launch("expect_finalize");
+    } else {
+      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
     }
     double2 h;
     check_hip(hipMemcpyAsync(&h, dout, sizeof(double2), hipMemcpyDeviceToHost, stream),

@@ -80,6 +80,7 @@ struct ExpectationPlan {
   u32* d_key_term_begin = nullptr;   // [num_keys + 1] CSR over keys in group order
   u32* d_key_terms = nullptr;        // [T] terms of each key, in term order
   PassDesc* d_passes = nullptr;      // [passes] descriptors read by the tiled kernel
+  u32* d_flat_key = nullptr;         // [num_keys] group-order position -> distinct-key id
   // Host copies used to assemble batched launches.
   std::vector<ExpItem> direct_items;  // items before tiling (direct-sweep form)
   std::vector<u64> h_group_x;
