@@ -188,3 +188,11 @@ def test_simplify_edges(gpu):
     assert ident.simplify().coeffs()[0] == 6.0
     with pytest.raises(ValueError, match="tolerances"):
         PauliSum.from_labels(["X"]).to_device().simplify(atol=float("nan"))
+    # Zero-qubit products (no mask words at all), simplified and not.
+    a = PauliSum._from_arrays(0, np.zeros((3, 0), np.uint64), np.zeros((3, 0), np.uint64),
+                              np.array([1.0, 2.0j, -1.0]))
+    b = PauliSum._from_arrays(0, np.zeros((2, 0), np.uint64), np.zeros((2, 0), np.uint64),
+                              np.array([0.5, 1.0]))
+    assert a.matmul(b, simplify=False).num_terms == 6
+    prod = a @ b
+    assert prod.num_terms == 1 and prod.coeffs()[0] == pytest.approx(1.5 * (2.0j))

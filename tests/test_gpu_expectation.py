@@ -113,3 +113,25 @@ def test_adjoint_sigma_and_term_values(gpu, orc, n):
     assert abs(e - orc.expectation_compensated(*packed(op), psi)) <= RTOL * s
     # <psi|sigma> = E
     assert abs(np.vdot(psi, sigma.cpu().numpy()) - e) <= RTOL * s
+
+
+@pytest.mark.parametrize("gen,n,terms", [("chemistry", 18, 2000), ("chemistry", 20, 3000),
+                                         ("random", 18, 400), ("diagonal", 19, 300)])
+def test_tiled_sweep_matches_oracle(gpu, orc, gen, n, terms):
+    from wolfgang_amd import PauliSum
+
+    if gen == "chemistry":
+        op = chemistry_masks(n, terms, 40 + n)
+    elif gen == "random":
+        op = random_pauli_masks(n, terms, 0.3, 41 + n)
+    else:
+        r = random_pauli_masks(n, terms, 0.4, 42 + n)
+        op = PauliSum.from_packed(n, np.zeros_like(r.x_words()), r.z_words(), r.coeffs())
+    psi = random_statevector(n, 43 + n)
+    dop = op.to_device()
+    info = dop.expectation_plan_info()
+    assert info["tiled"], info
+    got = dop.expectation_statevector(psi)
+    want = orc.expectation_compensated(*packed(op), psi, threads=8)
+    assert abs(got - want) <= RTOL * scale(op, psi)
+    assert dop.expectation_statevector(psi) == got

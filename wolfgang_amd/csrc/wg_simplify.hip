@@ -242,7 +242,9 @@ __global__ __launch_bounds__(256) void compact_kernel(
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
     const std::size_t k = seg_start[s];
-    if (KW == 1) {
+    if (W == 0) {
+      // Zero-qubit operators have no mask words (their buffers are null).
+    } else if (KW == 1) {
       ox[o] = keys[k] >> 32;
       oz[o] = keys[k] & 0xFFFFFFFFULL;
     } else {
