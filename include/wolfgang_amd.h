@@ -186,6 +186,12 @@ int wg_expectation_plan_info(const wg_pauli_sum* op, size_t* distinct_keys, size
 int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                         const double* psi, double* key_vals, size_t* num_keys, size_t* num_passes);
 
+/* Host-only: per-pass summary of the tiled plan of a W = 1 operator (test and
+ * tuning hook): rows of {k, items, item groups, tile blocks, diagonal items,
+ * off-diagonal items}, up to max_passes rows; *num_passes gets the count. */
+int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                         int* rows, size_t max_passes, size_t* num_passes);
+
 /* Kernel timing with HIP events (no reference counterpart; measurement only).
  * While enabled, the library brackets each launch of its hot kernels
  * ("expect_direct", "commute_packed", "product", "radix_scatter", ...) with an

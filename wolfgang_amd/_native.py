@@ -64,6 +64,7 @@ SIGNATURES = {
     "wg_expectation_plan_info": (
         C.c_int, [_vp, C.POINTER(_size), C.POINTER(_size), C.POINTER(_size), C.POINTER(C.c_int), _vp]
     ),
+    "wg_debug_plan_passes": (C.c_int, [_size, _size, _vp, _vp, _vp, _size, C.POINTER(_size)]),
     "wg_debug_tiled_plan": (
         C.c_int, [_size, _size, _vp, _vp, _vp, _vp, C.POINTER(_size), C.POINTER(_size)]
     ),
@@ -82,12 +83,38 @@ _lib: C.CDLL | None = None
 _load_error: str | None = None
 
 
+def _share_hip_runtime_with_torch() -> None:
+    """Make this library bind to the HIP runtime PyTorch uses, when PyTorch is
+    installed. PyTorch ships its own libamdhip64.so.7 with the same SONAME as
+    the system ROCm runtime; whichever is loaded first serves the whole process,
+    and PyTorch cannot initialise the GPU on a runtime other than its own. Loading
+    PyTorch's copy (RTLD_GLOBAL) before this library keeps one runtime shared
+    by both, so device pointers and streams can cross between them."""
+    import importlib.util
+
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or spec.origin is None:
+        return
+    libdir = pathlib.Path(spec.origin).parent / "lib"
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        path = libdir / name
+        if path.exists():
+            try:
+                C.CDLL(str(path), mode=C.RTLD_GLOBAL)
+            except OSError:
+                return
+
+
 def load() -> C.CDLL:
     """Return the loaded library, raising RuntimeError if it cannot be loaded."""
     global _lib, _load_error
     with _lock:
         if _lib is not None:
             return _lib
+        _share_hip_runtime_with_torch()
         if not LIB_PATH.exists():
             _load_error = (
                 f"wolfgang_amd native library not found at {LIB_PATH}; build it with "

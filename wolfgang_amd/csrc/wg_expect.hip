@@ -1174,6 +1174,48 @@ int wg_expectation_plan_info(const wg_pauli_sum* op_c, size_t* distinct_keys, si
   });
 }
 
+// Host-only: the tiled pass plan of a W = 1 operator, without replay. Writes
+// up to max_passes rows of {k, items, item_groups, tile_blocks, diagonal
+// items, off-diagonal items} and the pass count.
+int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                         int* rows, size_t max_passes, size_t* num_passes) {
+  return guard([&] {
+    std::unordered_map<u64, std::vector<u64>> zs_of;
+    std::vector<u64> order_x;
+    std::unordered_map<u64, std::unordered_map<u64, int>> seen;
+    for (std::size_t t = 0; t < num_terms; ++t) {
+      if (seen[x[t]].emplace(z[t], 1).second) {
+        if (zs_of.find(x[t]) == zs_of.end()) order_x.push_back(x[t]);
+        zs_of[x[t]].push_back(z[t]);
+      }
+    }
+    std::vector<ExpItem> items;
+    for (u64 gx : order_x) {
+      const auto& zs = zs_of[gx];
+      for (std::size_t s = 0; s < zs.size(); s += kZ) {
+        ExpItem it{};
+        it.x = gx;
+        it.nz = static_cast<int>(std::min<std::size_t>(kZ, zs.size() - s));
+        for (int j = 0; j < it.nz; ++j) it.z[j] = zs[s + j];
+        items.push_back(it);
+      }
+    }
+    std::vector<PassLaunch> passes;
+    plan_tiled_passes(num_qubits, items, passes);
+    *num_passes = passes.size();
+    for (std::size_t p = 0; p < passes.size() && p < max_passes; ++p) {
+      int diag = 0;
+      for (u32 i = 0; i < passes[p].num_items; ++i) diag += items[passes[p].first_item + i].x == 0;
+      rows[6 * p + 0] = passes[p].desc.k;
+      rows[6 * p + 1] = static_cast<int>(passes[p].num_items);
+      rows[6 * p + 2] = static_cast<int>(passes[p].item_groups);
+      rows[6 * p + 3] = static_cast<int>(passes[p].tile_blocks);
+      rows[6 * p + 4] = diag;
+      rows[6 * p + 5] = static_cast<int>(passes[p].num_items) - diag;
+    }
+  });
+}
+
 // Host-only check of the tiled pass plan for an operator given as host masks
 // (W = 1): builds the items and passes exactly as the device plan does, then
 // replays the tile decomposition on the CPU over `psi` (length 2^n) with the
