@@ -48,9 +48,10 @@ def parse_args() -> argparse.Namespace:
     p.add_argument(
         "--workload",
         default="expectation",
-        choices=["expectation", "commutation", "matmul", "batch", "copy"],
+        choices=["expectation", "commutation", "matmul", "batch", "adjoint", "copy"],
         help="expectation (the headline line, default); commutation / matmul: the 128-qubit "
-        "algebra workloads; batch: batched forward+adjoint vs the per-instance path; copy: a "
+        "algebra workloads; batch: batched forward+adjoint vs the per-instance path; adjoint: "
+        "forward + fused adjoint at 24 qubits, 20,000 terms; copy: a "
         "plain device copy of known size, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE",
     )
     return p.parse_args()
@@ -157,8 +158,13 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     info = dop.expectation_plan_info()
-    kernel = "expect_tiled" if info["tiled"] else "expect_direct"
-    kernel_ms, launches = _native.profile_get(kernel)
+    # The sweep kernels over psi: the tiled passes plus, when the diagonal keys
+    # use it, the Walsh-Hadamard sweep; the direct kernel for small states.
+    kernels = ["expect_tiled", "expect_diag_wht"] if info["tiled"] else ["expect_direct"]
+    per_kernel = {k: _native.profile_get(k) for k in kernels}
+    kernel_ms = sum(ms for ms, _ in per_kernel.values())
+    launches = sum(n_ for _, n_ in per_kernel.values())
+    kernel = "+".join(k for k in kernels if per_kernel[k][1])
     _native.profile_enable(False)
 
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -207,6 +213,7 @@ def main() -> int:
             "kernel_ms_per_step": kernel_s_per_step * 1000.0,
             "launches_per_step": launches_per_step,
             "kernel_avg_ms": kernel_ms / launches if launches else None,
+            "kernel_ms_per_step_each": {k: ms / args.steps for k, (ms, _) in per_kernel.items()},
             "algorithmic_bytes_per_step": bytes_per_step,
         },
     }

@@ -53,6 +53,9 @@ constexpr int kTileWaves = kTileBlock / 64;
 // needs more passes (launches) than this.
 constexpr std::size_t kTiledMinQubits = 18;
 constexpr std::size_t kMaxTiledPasses = 256;
+// Diagonal keys use the Walsh-Hadamard sweep beyond this many items' worth.
+constexpr int kDiagWhtMinItems = 4;
+constexpr u32 kDiagWhtBlocks = 1024;
 
 __device__ inline u64 insert_zero(u64 v, u64 pivot) {
   const u64 low = pivot - 1;
@@ -140,6 +143,27 @@ __global__ __launch_bounds__(kBlock) void expect_direct_kernel(const double2* __
 
 // ------------------------------------------------------------ tiled sweep
 
+// Every tiled pass uses tiles of kTilePoints = 2^kTileMaxLog amplitudes: the
+// planner pads each subspace with coordinate bits up to that dimension.
+constexpr u32 kTilePoints = 1U << kTileMaxLog;
+constexpr int kOffSteps = kTilePoints / 2 / 64;  // pair steps per wave (b & pivot = 0)
+constexpr int kDiagSteps = kTilePoints / 64;     // point steps per wave (x = 0)
+
+// The bits of v at the positions of `mask`, packed into the low bits in
+// ascending order (software pext); parity(z & tile_rep(t, mask)) equals
+// parity(compress_bits(z, mask) & t).
+__device__ inline u64 compress_bits(u64 v, u64 mask) {
+  u64 out = 0;
+  int k = 0;
+  while (mask) {
+    const u64 low = mask & (0 - mask);
+    if (v & low) out |= 1ULL << k;
+    ++k;
+    mask ^= low;
+  }
+  return out;
+}
+
 // Coset representative of tile t: the bits of t deposited into the free
 // positions (ascending), i.e. a software pdep over the pass's free mask.
 __device__ inline u64 tile_rep(u64 t, u64 free_mask) {
@@ -153,16 +177,248 @@ __device__ inline u64 tile_rep(u64 t, u64 free_mask) {
   return rep;
 }
 
+__device__ __forceinline__ double2 lds_at(const double2* tile, u32 byte_off) {
+  return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(tile) + byte_off);
+}
+
+// Thread t stages points u = t + i * kTileBlock (i < 8) of a tile, i.e. the
+// segments wave + 8 i. Segment offsets are linear over GF(2) in the segment
+// index, so seg_off[wave + 8 i] = seg_off[wave] ^ (the offsets of the set bits
+// of 8 i): four uniform values per wave instead of eight.
+constexpr u32 kSegStride = kTileBlock >> kTileLowLog;
+static_assert(kTilePoints / kTileBlock == 8 && kSegStride == 8,
+              "staging is written for 8 points per thread and 8 waves");
+
+// The staged points live in eight named registers (an array or a struct
+// passed by reference may be left in scratch memory by the compiler).
+#define WG_FETCH_TILE(rep)                                                  \
+  do {                                                                      \
+    const u64 r0_ = (rep) ^ sb_w, r1_ = r0_ ^ sb_e0, r2_ = r0_ ^ sb_e1;      \
+    const u64 r3_ = r1_ ^ sb_e1;                                            \
+    st0 = psi[r0_ | lo];                                                    \
+    st1 = psi[r1_ | lo];                                                    \
+    st2 = psi[r2_ | lo];                                                    \
+    st3 = psi[r3_ | lo];                                                    \
+    st4 = psi[(r0_ ^ sb_e2) | lo];                                          \
+    st5 = psi[(r1_ ^ sb_e2) | lo];                                          \
+    st6 = psi[(r2_ ^ sb_e2) | lo];                                          \
+    st7 = psi[(r3_ ^ sb_e2) | lo];                                          \
+  } while (0)
+
+// The block's tile loop: stage tile t in LDS, prefetch tile t + tile_blocks
+// into registers, let this wave's work consume the staged tile. Every wave of
+// the block runs the same number of iterations (same barriers), whatever its
+// work type.
+template <class Work>
+__device__ __forceinline__ void sweep_tiles(const double2* __restrict__ psi,
+                                            const PassDesc* __restrict__ pd, u32 wave, u64 t0,
+                                            u32 tile_blocks, double2* __restrict__ tile,
+                                            Work& work) {
+  const u64 free_mask = pd->free_mask;
+  const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[8], sb_e1 = pd->seg_off[16],
+            sb_e2 = pd->seg_off[32];
+  const u32 lo = threadIdx.x & ((1U << kTileLowLog) - 1U);
+  double2 st0, st1, st2, st3, st4, st5, st6, st7;
+  u64 t = t0;
+  if (t < ntiles) WG_FETCH_TILE(tile_rep(t, free_mask));
+  for (; t < ntiles; t += tile_blocks) {
+    __syncthreads();
+    {
+      const u32 u = threadIdx.x;
+      tile[u + 0 * kTileBlock] = st0;
+      tile[u + 1 * kTileBlock] = st1;
+      tile[u + 2 * kTileBlock] = st2;
+      tile[u + 3 * kTileBlock] = st3;
+      tile[u + 4 * kTileBlock] = st4;
+      tile[u + 5 * kTileBlock] = st5;
+      tile[u + 6 * kTileBlock] = st6;
+      tile[u + 7 * kTileBlock] = st7;
+    }
+    __syncthreads();
+    const u64 tn = t + tile_blocks;
+    // Prefetch the next tile into registers while this one is consumed.
+    if (tn < ntiles) WG_FETCH_TILE(tile_rep(tn, free_mask));
+    work(tile, t);
+  }
+}
+#undef WG_FETCH_TILE
+
+struct IdleWork {
+  __device__ void operator()(const double2*, u64) {}
+};
+// (Work functors get the tile index t; the tile's coset representative is
+// tile_rep(t, free_mask).)
+
+// Writes a wave's NZ (sum over lanes) partial values for one tile block; the
+// per-lane sign parity(zl_j & f_lane) is applied first.
+template <int NZ>
+__device__ __forceinline__ void write_partials(const double (&ar)[NZ], const double (&ai)[NZ],
+                                               u32 lane_par, u32 lane, double2* __restrict__ row) {
+#pragma unroll
+  for (int j = 0; j < NZ; ++j) {
+    const bool neg = (lane_par >> j) & 1U;
+    double r = neg ? -ar[j] : ar[j];
+    double i = neg ? -ai[j] : ai[j];
+    for (int d = 32; d > 0; d >>= 1) {
+      r += __shfl_xor(r, d, 64);
+      i += __shfl_xor(i, d, 64);
+    }
+    if (lane == 0) row[j] = make_double2(r, i);
+  }
+}
+
+// One off-diagonal item on one wave. Lane points u = f ^ g(s), s < kOffSteps,
+// with f = insert_zero(lane, pivot) and g(s) = insert_zero(s << 6, pivot):
+// they enumerate the pair representatives b & pivot = 0 of the tile. Since
+// parity(zl & u) = parity(zl & f) ^ parity(zl & g(s)), the lane part is a sign
+// applied once at the end and the step part (bit s of smask) is wave-uniform,
+// so each (pair, key) update is one FMA by a uniform +-1 per component. Steps
+// go in batches of 4: g(s0 + s) = g(s0) ^ g(s) for s < 4 (disjoint bits).
+template <int NZ>
+struct OffDiagWork {
+  static constexpr int B = NZ <= 4 ? 4 : 2;
+  u64 zt[NZ];       // z compressed to the free (tile index) bits: parity(z & rep(t)) = parity(zt & t)
+  u32 smask[NZ];
+  u32 a16[B], b16[B];  // (f ^ g(s)) * 16 and (f ^ xl ^ g(s)) * 16 for s < B
+  u64 pbit;
+  u32 lane_par;
+  double ar[NZ], ai[NZ];
+
+  __device__ OffDiagWork(const ExpItem& it, u64 free_mask, u32 lane) {
+    const u32 xl = static_cast<u32>(it.xl);
+    pbit = it.pbit;
+    const u32 f = static_cast<u32>(insert_zero(lane, pbit));
+#pragma unroll
+    for (int s = 0; s < B; ++s) {
+      const u32 g = static_cast<u32>(insert_zero(static_cast<u64>(s) << 6, pbit));
+      a16[s] = (f ^ g) << 4;
+      b16[s] = (f ^ g ^ xl) << 4;
+    }
+    lane_par = 0;
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+      zt[j] = compress_bits(it.z[j], free_mask);
+      const u32 zlj = static_cast<u32>(it.zl[j]);
+      lane_par |= static_cast<u32>(__popc(zlj & f) & 1) << j;
+      u32 m = 0;
+      for (int s = 0; s < kOffSteps; ++s) {
+        const u32 g = static_cast<u32>(insert_zero(static_cast<u64>(s) << 6, pbit));
+        m |= static_cast<u32>(__popc(zlj & g) & 1) << s;
+      }
+      smask[j] = m;
+      ar[j] = 0.0;
+      ai[j] = 0.0;
+    }
+  }
+
+  __device__ __forceinline__ void operator()(const double2* __restrict__ tile, u64 t) {
+    u32 sm[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) sm[j] = (__popcll(zt[j] & t) & 1) ? ~smask[j] : smask[j];
+    constexpr int kUnroll = NZ <= 2 ? 2 : 1;
+#pragma unroll kUnroll
+    for (int s0 = 0; s0 < kOffSteps; s0 += B) {
+      const u32 g0 = static_cast<u32>(insert_zero(static_cast<u64>(s0) << 6, pbit)) << 4;
+      double2 a[B], b[B];
+#pragma unroll
+      for (int s = 0; s < B; ++s) {
+        a[s] = lds_at(tile, a16[s] ^ g0);
+        b[s] = lds_at(tile, b16[s] ^ g0);
+      }
+#pragma unroll
+      for (int s = 0; s < B; ++s) {
+        const double re = fma(b[s].x, a[s].x, b[s].y * a[s].y);
+        const double im = fma(b[s].x, a[s].y, -(b[s].y * a[s].x));
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+          const double sg = ((sm[j] >> (s0 + s)) & 1U) ? -1.0 : 1.0;
+          ar[j] = fma(sg, re, ar[j]);
+          ai[j] = fma(sg, im, ai[j]);
+        }
+      }
+    }
+  }
+};
+
+// One diagonal item (x = 0) on one wave: lane points u = lane + 64 s.
+template <int NZ>
+struct DiagWork {
+  u64 zt[NZ];
+  u64 smask[NZ];
+  u32 lane_par;
+  u32 f16;
+  double ar[NZ], ai[NZ];
+
+  __device__ DiagWork(const ExpItem& it, u64 free_mask, u32 lane) {
+    f16 = lane << 4;
+    lane_par = 0;
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) {
+      zt[j] = compress_bits(it.z[j], free_mask);
+      const u32 zlj = static_cast<u32>(it.zl[j]);
+      lane_par |= static_cast<u32>(__popc(zlj & lane) & 1) << j;
+      u64 m = 0;
+      for (int s = 0; s < kDiagSteps; ++s) m |= static_cast<u64>(__popc(zlj & (static_cast<u32>(s) << 6)) & 1) << s;
+      smask[j] = m;
+      ar[j] = 0.0;
+      ai[j] = 0.0;
+    }
+  }
+
+  __device__ __forceinline__ void operator()(const double2* __restrict__ tile, u64 t) {
+    u64 sm[NZ];
+#pragma unroll
+    for (int j = 0; j < NZ; ++j) sm[j] = (__popcll(zt[j] & t) & 1) ? ~smask[j] : smask[j];
+    constexpr int B = NZ <= 4 ? 4 : 2;
+    constexpr int kUnroll = NZ <= 2 ? 2 : 1;
+#pragma unroll kUnroll
+    for (int s0 = 0; s0 < kDiagSteps; s0 += B) {
+      double2 a[B];
+#pragma unroll
+      for (int s = 0; s < B; ++s) a[s] = lds_at(tile, f16 + (static_cast<u32>(s0 + s) << 10));
+#pragma unroll
+      for (int s = 0; s < B; ++s) {
+        const double p = fma(a[s].x, a[s].x, a[s].y * a[s].y);
+#pragma unroll
+        for (int j = 0; j < NZ; ++j) {
+          const double sg = ((sm[j] >> (s0 + s)) & 1ULL) ? -1.0 : 1.0;
+          ar[j] = fma(sg, p, ar[j]);
+        }
+      }
+    }
+  }
+};
+
+template <int NZ, bool kDiag>
+__device__ __forceinline__ void run_item(const double2* __restrict__ psi,
+                                         const PassDesc* __restrict__ pd, const ExpItem& it,
+                                         u32 wave, u32 lane, u64 t0, u32 tile_blocks,
+                                         double2* __restrict__ tile, double2* __restrict__ row) {
+  if constexpr (kDiag) {
+    DiagWork<NZ> w(it, pd->free_mask, lane);
+    sweep_tiles(psi, pd, wave, t0, tile_blocks, tile, w);
+    write_partials<NZ>(w.ar, w.ai, w.lane_par, lane, row);
+  } else {
+    OffDiagWork<NZ> w(it, pd->free_mask, lane);
+    sweep_tiles(psi, pd, wave, t0, tile_blocks, tile, w);
+    write_partials<NZ>(w.ar, w.ai, w.lane_par, lane, row);
+  }
+}
+
 // Grid: item_groups * tile_blocks blocks. Block -> (item group, tile block)
 // such that the blocks sharing a tile block differ by multiples of 8 in their
 // linear id: they are dispatched to the same XCD close together, so the later
-// reads of a tile hit that XCD's L2 instead of HBM.
-__global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
+// reads of a tile hit that XCD's L2 instead of HBM. Wave w of a block applies
+// item (group * 8 + w) to every tile of its tile block.
+// Two blocks per CU (LDS 2 x 64 KiB) need 4 waves per SIMD: <= 128 VGPRs.
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void expect_tiled_kernel(
     const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
     const ExpItem* __restrict__ items,
     u32 num_items, u32 item_groups, u32 tile_blocks, u32 partial_base,
     double2* __restrict__ partials) {
-  extern __shared__ double2 tile[];
+  __shared__ double2 tile[kTilePoints];
   const u32 b = blockIdx.x;
   const u32 xcd = b & 7U;
   const u32 slot = b >> 3;
@@ -171,155 +427,127 @@ __global__ __launch_bounds__(kTileBlock) void expect_tiled_kernel(
   const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const u32 lane = threadIdx.x & 63U;
   const u32 item_id = ig * kTileWaves + wave;
-  const bool active = item_id < num_items;
-  const ExpItem& it = items[active ? item_id : 0];
-  const int nz = active ? it.nz : 0;
-  const u32 xl = static_cast<u32>(it.xl);
-  const u32 pbit = static_cast<u32>(it.pbit);
-
-  const int k = pd->k;
-  const u64 free_mask = pd->free_mask;
-  constexpr u32 lmask = (1U << kTileLowLog) - 1U;
-  const u32 tile_size = 1U << k;
-  const u64 ntiles = 1ULL << (pd->n - k);
-  constexpr int kLoads = (1 << kTileMaxLog) / kTileBlock;
-  // Thread t loads points u = t + i*kTileBlock; their segment index
-  // u >> kTileLowLog = wave + i*kSegStride is wave-uniform (scalar loads).
-  constexpr u32 kSegStride = kTileBlock >> kTileLowLog;
-
-  double ar[kZ], ai[kZ];
-#pragma unroll
-  for (int j = 0; j < kZ; ++j) {
-    ar[j] = 0.0;
-    ai[j] = 0.0;
+  if (item_id >= num_items) {
+    IdleWork w;
+    sweep_tiles(psi, pd, wave, tb, tile_blocks, tile, w);
+    return;
   }
+  const ExpItem& it = items[item_id];
+  double2* row = partials + (static_cast<std::size_t>(partial_base) +
+                             static_cast<std::size_t>(item_id) * tile_blocks + tb) * kZ;
+#define WG_ITEM(N)                                                                   \
+  case N:                                                                            \
+    if (it.xl) run_item<N, false>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row); \
+    else run_item<N, true>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row);      \
+    break;
+  switch (it.nz) {
+    WG_ITEM(1) WG_ITEM(2) WG_ITEM(3) WG_ITEM(4) WG_ITEM(5) WG_ITEM(6) WG_ITEM(7)
+    default: WG_ITEM(8)
+  }
+#undef WG_ITEM
+}
 
-  // The lane's points in a tile are u = f ^ g(s), s = 0..steps-1, with f the
-  // lane's own bits and g(s) wave-uniform (both through the pivot-bit insertion,
-  // which is linear over GF(2)). Hence parity(zl & u) = parity(zl & f) ^
-  // parity(zl & g(s)): the first term is a per-lane sign applied once after the
-  // sweep, the second a wave-uniform bit, so each (point, key) update is one
-  // fused multiply-add by a uniform +-1 per component.
-  const u32 span = xl ? (tile_size >> 1) : tile_size;
-  const u32 steps = span > 64 ? span >> 6 : 1;
-  const bool lane_on = lane < span;
-  const u32 f_lane = xl ? static_cast<u32>(insert_zero(lane, pbit)) : lane;
-  u64 smask[kZ];
-  u32 lane_par = 0;
+// ------------------------------------------------------------ diagonal keys
+
+// All diagonal keys (x = 0) at once: for a tile of kTilePoints consecutive
+// amplitudes (index = t * kTilePoints + u) the block forms p(u) = |psi|^2 and
+// its Walsh-Hadamard transform P(v) = sum_u (-1)^{|v & u|} p(u); key z then
+// gains (-1)^{|(z >> kTileMaxLog) & t|} P(z mod kTilePoints). The transform is
+// kTileMaxLog butterfly stages, 3 in registers and 9 through LDS, so a tile
+// costs O(k 2^k) whatever the number of keys.
+constexpr int kWhtKeysPerThread = 4;
+constexpr u32 kWhtMaxKeys = kTileBlock * kWhtKeysPerThread;
+
+// LDS slot of transform index u: one pad double every 16 spreads the strided
+// accesses of the butterfly rounds over the banks.
+__device__ __forceinline__ u32 wht_slot(u32 u) { return u + (u >> 4); }
+
+__device__ __forceinline__ void wht8(double (&v)[8]) {
 #pragma unroll
-  for (int j = 0; j < kZ; ++j) {
-    smask[j] = 0;
-    if (j < nz) {
-      const u32 zlj = static_cast<u32>(it.zl[j]);
-      lane_par |= static_cast<u32>(__popc(zlj & f_lane) & 1) << j;
-      u64 m = 0;
-      for (u32 st = 0; st < steps; ++st) {
-        const u32 g = xl ? static_cast<u32>(insert_zero(st << 6, pbit)) : (st << 6);
-        m |= static_cast<u64>(__popc(zlj & g) & 1) << st;
+  for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (!(i & h)) {
+        const double a = v[i], c = v[i | h];
+        v[i] = a + c;
+        v[i | h] = a - c;
       }
-      smask[j] = m;
     }
   }
+}
 
-  // Tiles smaller than kLoads * kTileBlock points wrap (u & (tile_size - 1)):
-  // several threads then load and store the same point with the same value,
-  // which keeps the staging loops branch-free and the staging in registers.
-  const u32 tile_mask = tile_size - 1;
-  const u32 seg_mask = (tile_size >> kTileLowLog) - 1;
-  static_assert(kLoads == 8, "staging below is written for 8 points per thread");
-  double2 s0, s1, s2, s3, s4, s5, s6, s7;
-  auto fetch = [&](u64 rep) {
-    const double2* base = psi;
-    const u32 lo = threadIdx.x & lmask;
-#define WG_FETCH(reg, i) \
-    reg = base[(rep ^ pd->seg_off[(wave + (i) * kSegStride) & seg_mask]) | lo];
-    WG_FETCH(s0, 0) WG_FETCH(s1, 1) WG_FETCH(s2, 2) WG_FETCH(s3, 3)
-    WG_FETCH(s4, 4) WG_FETCH(s5, 5) WG_FETCH(s6, 6) WG_FETCH(s7, 7)
-#undef WG_FETCH
-  };
-  u64 t = tb;
-  if (t < ntiles) fetch(tile_rep(t, free_mask));
-  for (; t < ntiles; t += tile_blocks) {
+// Keys [key_begin, key_begin + num_keys) of the diagonal items (kZ keys per
+// item, all items full but the last); item i's partial row for block c is
+// partial_base + i * gridDim.x + c.
+__global__ __launch_bounds__(kTileBlock) void expect_diag_wht_kernel(
+    const double2* __restrict__ psi, u32 log_tiles, const ExpItem* __restrict__ items,
+    u32 key_begin, u32 num_keys, u32 partial_base, double2* __restrict__ partials) {
+  __shared__ double w[kTilePoints + kTilePoints / 16];
+  const u32 tid = threadIdx.x;
+  u64 zh[kWhtKeysPerThread];
+  u32 zslot[kWhtKeysPerThread];
+  double acc[kWhtKeysPerThread];
+#pragma unroll
+  for (int r = 0; r < kWhtKeysPerThread; ++r) {
+    const u32 kk = tid + r * kTileBlock;
+    acc[r] = 0.0;
+    zh[r] = 0;
+    zslot[r] = 0;
+    if (kk < num_keys) {
+      const u32 g = key_begin + kk;
+      const u64 z = items[g / kZ].z[g % kZ];
+      zh[r] = z >> kTileMaxLog;
+      zslot[r] = wht_slot(static_cast<u32>(z) & (kTilePoints - 1));
+    }
+  }
+  const u64 ntiles = 1ULL << log_tiles;
+  double2 s[8];
+  u64 t = blockIdx.x;
+  if (t < ntiles) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[i] = psi[(t << kTileMaxLog) + tid + i * kTileBlock];
+  }
+  for (; t < ntiles; t += gridDim.x) {
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = fma(s[i].x, s[i].x, s[i].y * s[i].y);
+    const u64 tn = t + gridDim.x;
+    if (tn < ntiles) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) s[i] = psi[(tn << kTileMaxLog) + tid + i * kTileBlock];
+    }
+    wht8(v);  // bits 9..11 (register index)
     __syncthreads();
-    {
-      const u32 u = threadIdx.x;
-      tile[(u + 0 * kTileBlock) & tile_mask] = s0;
-      tile[(u + 1 * kTileBlock) & tile_mask] = s1;
-      tile[(u + 2 * kTileBlock) & tile_mask] = s2;
-      tile[(u + 3 * kTileBlock) & tile_mask] = s3;
-      tile[(u + 4 * kTileBlock) & tile_mask] = s4;
-      tile[(u + 5 * kTileBlock) & tile_mask] = s5;
-      tile[(u + 6 * kTileBlock) & tile_mask] = s6;
-      tile[(u + 7 * kTileBlock) & tile_mask] = s7;
-    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[wht_slot(tid + i * kTileBlock)] = v[i];
     __syncthreads();
-    const u64 rep = tile_rep(t, free_mask);
-    const u64 tn = t + tile_blocks;
-    // Prefetch the next tile into registers while this one is consumed.
-    if (tn < ntiles) fetch(tile_rep(tn, free_mask));
-    if (nz == 0 || !lane_on) continue;
-    // Per-tile uniform sign masks: bit s of sm[j] is the sign of step s.
-    u64 sm[kZ];
 #pragma unroll
-    for (int j = 0; j < kZ; ++j) {
-      sm[j] = (j < nz && (__popcll(it.z[j] & rep) & 1)) ? ~smask[j] : smask[j];
+    for (int r = 0; r < 3; ++r) {  // bits 3r .. 3r+2
+      const u32 lowbits = tid & ((1U << (3 * r)) - 1U);
+      const u32 high = (tid >> (3 * r)) << (3 * r + 3);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = w[wht_slot(high | (static_cast<u32>(j) << (3 * r)) | lowbits)];
+      wht8(v);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) w[wht_slot(high | (static_cast<u32>(j) << (3 * r)) | lowbits)] = v[j];
+      __syncthreads();
     }
-    if (xl) {
-#pragma unroll 4
-      for (u32 st = 0; st < steps; ++st) {
-        const u32 u = f_lane ^ static_cast<u32>(insert_zero(st << 6, pbit));
-        const double2 a = tile[u];
-        const double2 b = tile[u ^ xl];
-        const double re = fma(b.x, a.x, b.y * a.y);
-        const double im = fma(b.x, a.y, -(b.y * a.x));
 #pragma unroll
-        for (int j = 0; j < kZ; ++j) {
-          if (j < nz) {
-            const double sg = ((sm[j] >> st) & 1ULL) ? -1.0 : 1.0;
-            ar[j] = fma(sg, re, ar[j]);
-            ai[j] = fma(sg, im, ai[j]);
-          }
-        }
-      }
-    } else {
-#pragma unroll 4
-      for (u32 st = 0; st < steps; ++st) {
-        const double2 a = tile[f_lane ^ (st << 6)];
-        const double p = fma(a.x, a.x, a.y * a.y);
-#pragma unroll
-        for (int j = 0; j < kZ; ++j) {
-          if (j < nz) {
-            const double sg = ((sm[j] >> st) & 1ULL) ? -1.0 : 1.0;
-            ar[j] = fma(sg, p, ar[j]);
-          }
-        }
+    for (int r = 0; r < kWhtKeysPerThread; ++r) {
+      if (tid + r * kTileBlock < num_keys) {
+        const double val = w[zslot[r]];
+        acc[r] += (__popcll(zh[r] & t) & 1) ? -val : val;
       }
     }
   }
-  // Per-lane sign, then the wave-level reduction; each wave owns its item's
-  // partial row for this tile block.
 #pragma unroll
-  for (int j = 0; j < kZ; ++j) {
-    if ((lane_par >> j) & 1U) {
-      ar[j] = -ar[j];
-      ai[j] = -ai[j];
-    }
-  }
-  if (!active) return;
-  // Wave-level reduction; each wave owns its item's partial row for this tile block.
-#pragma unroll
-  for (int j = 0; j < kZ; ++j) {
-    if (j < nz) {
-      double r = ar[j], i = ai[j];
-      for (int d = 32; d > 0; d >>= 1) {
-        r += __shfl_xor(r, d, 64);
-        i += __shfl_xor(i, d, 64);
-      }
-      if (lane == 0) {
-        partials[(static_cast<std::size_t>(partial_base) +
-                  static_cast<std::size_t>(item_id) * tile_blocks + tb) * kZ + j] =
-            make_double2(r, i);
-      }
+  for (int r = 0; r < kWhtKeysPerThread; ++r) {
+    const u32 kk = tid + r * kTileBlock;
+    if (kk < num_keys) {
+      const u32 g = key_begin + kk;
+      partials[(static_cast<std::size_t>(partial_base) +
+                static_cast<std::size_t>(g / kZ) * gridDim.x + blockIdx.x) * kZ + g % kZ] =
+          make_double2(acc[r], 0.0);
     }
   }
 }
@@ -603,7 +831,8 @@ struct Span {
 
 }  // namespace
 
-void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes) {
+void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
+                       bool diag_separate, u32* num_diag_front) {
   const int l = kTileLowLog;
   const int kmax = std::min<int>(kTileMaxLog, static_cast<int>(n) - 3);
   const u64 lowmask = (1ULL << l) - 1;
@@ -670,6 +899,14 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
         }
       }
     }
+    // Pad U with the lowest unused index bits up to the full tile dimension:
+    // every pass then runs with tiles of 2^kmax points (more work per tile,
+    // the same single read of psi).
+    for (int q = l; q < static_cast<int>(n) && sp.dim() < kmax; ++q) {
+      u64 occupied = sp.coord;
+      for (u64 g : sp.gens) occupied |= 1ULL << Span::top(g);
+      if (!((occupied >> q) & 1ULL)) sp.add_coords(1ULL << q);
+    }
     // Membership scan: all x-groups for structured sets; for unstructured
     // sets only a window after the generators (the span rarely contains more),
     // which keeps planning linear in the number of x-groups.
@@ -689,12 +926,19 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
   }
 
   // Emit passes: local coordinates, segment offsets, free positions; items are
-  // reordered pass by pass (diagonal items ride in the first pass).
+  // reordered pass by pass. Diagonal items either lead the item list (swept
+  // separately) or ride in the first pass.
   std::unordered_map<u64, int> pass_of;
   for (std::size_t p = 0; p < pass_xs.size(); ++p) {
     for (u64 x : pass_xs[p]) pass_of[x] = static_cast<int>(p);
   }
   std::vector<ExpItem> ordered;
+  if (diag_separate) {
+    for (const ExpItem& it : items) {
+      if (it.x == 0) ordered.push_back(it);
+    }
+  }
+  if (num_diag_front) *num_diag_front = static_cast<u32>(ordered.size());
   passes.clear();
   for (std::size_t p = 0; p < spans.size(); ++p) {
     const Span& sp = spans[p];
@@ -755,6 +999,7 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
       for (const ExpItem& it : items) {
         const bool is_diag = it.x == 0;
         if (is_diag != (diag_round == 1)) continue;
+        if (is_diag && diag_separate) continue;
         if (is_diag ? p != 0 : pass_of.at(it.x) != static_cast<int>(p)) continue;
         ExpItem e = it;
         e.xl = is_diag ? 0 : x_local(it.x);
@@ -763,6 +1008,10 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
         ordered.push_back(e);
       }
     }
+    // Waves of a block wait for each other at every tile: group items of
+    // similar cost (key count) into the same block.
+    std::stable_sort(ordered.begin() + pl.first_item, ordered.end(),
+                     [](const ExpItem& a, const ExpItem& b) { return a.nz > b.nz; });
     pl.num_items = static_cast<u32>(ordered.size()) - pl.first_item;
     pl.item_groups = std::max<u32>(1, (pl.num_items + kTileWaves - 1) / kTileWaves);
     const u64 ntiles = 1ULL << (n - d.k);
@@ -915,18 +1164,36 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   if (plan->tiled) {
     std::vector<ExpItem> tiled_items = items;
     std::vector<PassLaunch> passes;
-    plan_tiled_passes(plan->n, tiled_items, passes);
+    std::size_t diag_keys = 0;
+    for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
+    // The transform costs about as much as a few items per tile, so the
+    // diagonal keys take it once they fill more than kDiagWhtMinItems items.
+    const bool diag_separate = diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ;
+    u32 diag_front = 0;
+    plan_tiled_passes(plan->n, tiled_items, passes, diag_separate, &diag_front);
     if (passes.size() > kMaxTiledPasses) {
       plan->tiled = false;  // launch-bound: one launch per pass would dominate
     } else {
+      for (const PassLaunch& p : passes) {
+        if (p.desc.k != kTileMaxLog) fail_runtime("tiled expectation plan has a short tile");
+      }
       items = std::move(tiled_items);
       plan->passes = std::move(passes);
+      plan->diag_items = diag_front;
+      plan->diag_keys = diag_front ? static_cast<u32>(diag_keys) : 0;
+      const u64 ntiles = u64{1} << (plan->n - kTileMaxLog);
+      plan->diag_blocks = static_cast<u32>(std::min<u64>(ntiles, kDiagWhtBlocks));
     }
   }
   if (plan->tiled) {
     offset.resize(items.size());
     count.resize(items.size());
     std::size_t slot = 0;
+    for (u32 i = 0; i < plan->diag_items; ++i) {
+      offset[i] = static_cast<u32>(slot);
+      count[i] = plan->diag_blocks;
+      slot += plan->diag_blocks;
+    }
     for (PassLaunch& p : plan->passes) {
       p.partial_base = static_cast<u32>(slot);
       for (u32 i = 0; i < p.num_items; ++i) {
@@ -1001,12 +1268,18 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
     double2* partials =
         scratch.take<double2>(checked_mul(plan.partial_slots, kZ, "expectation partials"));
     if (plan.tiled) {
+      for (u32 k0 = 0; k0 < plan.diag_keys; k0 += kWhtMaxKeys) {
+        ProfileScope prof("expect_diag_wht", stream);
+        expect_diag_wht_kernel<<<plan.diag_blocks, kTileBlock, 0, stream>>>(
+            psi, static_cast<u32>(plan.n - kTileMaxLog), plan.d_items, k0,
+            std::min<u32>(kWhtMaxKeys, plan.diag_keys - k0), 0, partials);
+        check_launch("expect_diag_wht");
+      }
       for (const PassLaunch& p : plan.passes) {
         if (p.num_items == 0) continue;
         const unsigned grid = p.item_groups * p.tile_blocks;
-        const std::size_t lds = (std::size_t{1} << p.desc.k) * sizeof(double2);
         ProfileScope prof("expect_tiled", stream);
-        expect_tiled_kernel<<<grid, kTileBlock, lds, stream>>>(
+        expect_tiled_kernel<<<grid, kTileBlock, 0, stream>>>(
             psi, plan.d_passes + (&p - plan.passes.data()), plan.d_items + p.first_item,
             p.num_items, p.item_groups, p.tile_blocks, p.partial_base, partials);
         check_launch("expect_tiled");
@@ -1201,7 +1474,10 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
       }
     }
     std::vector<PassLaunch> passes;
-    plan_tiled_passes(num_qubits, items, passes);
+    std::size_t diag_keys = 0;
+    for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
+    plan_tiled_passes(num_qubits, items, passes,
+                      diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ, nullptr);
     *num_passes = passes.size();
     for (std::size_t p = 0; p < passes.size() && p < max_passes; ++p) {
       int diag = 0;
@@ -1259,10 +1535,47 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
       }
     }
     std::vector<PassLaunch> passes;
-    plan_tiled_passes(n, items, passes);
+    std::size_t diag_keys = 0;
+    for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
+    const bool diag_separate =
+        n >= kTileMaxLog + 3 && diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ;
+    u32 diag_front = 0;
+    plan_tiled_passes(n, items, passes, diag_separate, &diag_front);
     *num_passes = passes.size();
     *num_keys = kx.size();
     const auto* amp = reinterpret_cast<const double2*>(psi);
+    // Diagonal items swept by the Walsh-Hadamard kernel: same tile split
+    // (consecutive runs of kTilePoints) and key lookup.
+    if (diag_front) {
+      std::vector<double> P(kTilePoints);
+      std::vector<double> acc(static_cast<std::size_t>(diag_front) * kZ, 0.0);
+      for (u64 t = 0; t < (u64{1} << (n - kTileMaxLog)); ++t) {
+        for (u32 u = 0; u < kTilePoints; ++u) {
+          const double2 a = amp[(t << kTileMaxLog) + u];
+          P[u] = a.x * a.x + a.y * a.y;
+        }
+        for (u32 h = 1; h < kTilePoints; h <<= 1) {
+          for (u32 u = 0; u < kTilePoints; ++u) {
+            if (!(u & h)) {
+              const double a = P[u], c = P[u | h];
+              P[u] = a + c;
+              P[u | h] = a - c;
+            }
+          }
+        }
+        for (u32 i = 0; i < diag_front; ++i) {
+          const ExpItem& it = items[i];
+          if (it.x != 0) fail_runtime("diagonal sweep holds an off-diagonal item");
+          for (int j = 0; j < it.nz; ++j) {
+            const double v = P[it.z[j] & (kTilePoints - 1)];
+            acc[i * kZ + j] += (__builtin_popcountll((it.z[j] >> kTileMaxLog) & t) & 1) ? -v : v;
+          }
+        }
+      }
+      for (u32 i = 0; i < diag_front; ++i) {
+        for (int j = 0; j < items[i].nz; ++j) key_vals[items[i].key[j]] = acc[i * kZ + j];
+      }
+    }
     std::vector<double2> tile;
     std::vector<char> seen(std::size_t{1} << n);
     for (const PassLaunch& p : passes) {

@@ -66,6 +66,10 @@ struct ExpectationPlan {
   bool tiled = false;
   u32 blocks_per_item = 1;           // direct path
   std::vector<PassLaunch> passes;    // tiled path
+  u32 diag_items = 0;                // tiled path: leading diagonal items swept by the
+                                     // Walsh-Hadamard kernel (0: they ride in pass 0)
+  u32 diag_keys = 0;
+  u32 diag_blocks = 0;               // its grid; item i's partial rows are i * diag_blocks + c
   std::vector<u32> item_partial_offset;  // host copy; device copy below
   std::size_t partial_slots = 0;     // total partial entries (x kZ)
   ExpItem* d_items = nullptr;        // [num_items]
@@ -95,7 +99,10 @@ struct ExpectationPlan {
 std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
 
 // Builds the tiled passes for the given items (host only; exposed for tests of
-// the planner through wg_expectation_plan_info).
-void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes);
+// the planner through wg_expectation_plan_info). With diag_separate the
+// diagonal items are moved to the front of `items` (their count in
+// *num_diag_front) and left out of the passes.
+void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
+                       bool diag_separate, u32* num_diag_front);
 
 }  // namespace wg
