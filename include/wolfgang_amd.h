@@ -172,6 +172,27 @@ int wg_batch_sync(wg_batch* batch, void* stream);
 int wg_batch_info(const wg_batch* batch, int* tunables, size_t* units, size_t* items);
 void wg_batch_destroy(wg_batch* batch);
 
+/* Batched Hamiltonian products (no reference counterpart; each pair follows
+ * DevicePauliSum::matmul, src/hip/matmul_hip.hip.cpp:91-147, and its result is
+ * bit-identical to wg_matmul on that pair): lhs[i] @ rhs[i] for `count` pairs
+ * of uneven term and qubit counts on one device. Creation validates every pair
+ * with matmul's messages and limits (max_intermediate_terms per pair), packs
+ * the pairs into chunks whose intermediate products fit budget_bytes (0 =
+ * 16 GiB) and captures, per chunk, one persistent product kernel fed by a
+ * device work queue into a HIP graph. The operands must outlive the batch.
+ * wg_product_batch_run replays the graphs and writes `count` new operators to
+ * outs (simplified with matmul's rule when simplify_output != 0); it
+ * synchronizes `stream` before returning. */
+typedef struct wg_product_batch wg_product_batch;
+int wg_product_batch_create(size_t count, const wg_pauli_sum* const* lhs,
+                            const wg_pauli_sum* const* rhs, size_t max_intermediate_terms,
+                            size_t budget_bytes, void* stream, wg_product_batch** out);
+int wg_product_batch_run(wg_product_batch* batch, int simplify_output, void* stream,
+                         wg_pauli_sum** outs);
+int wg_product_batch_info(const wg_product_batch* batch, size_t* pairs, size_t* chunks,
+                          size_t* total_products);
+void wg_product_batch_destroy(wg_product_batch* batch);
+
 /* Expectation plan summary for an operator (built and cached on first use;
  * no reference counterpart): distinct (x, z) keys, distinct x-masks counting
  * the diagonal group (G_x), sweeps over psi per evaluation (tiled passes, or

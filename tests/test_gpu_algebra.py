@@ -165,6 +165,33 @@ def test_simplify_bit_exact_random(gpu, orc, n, terms):
         assert_bit_exact(op.to_device().simplify(atol, rtol).to_host(), orc.simplify(*packed(op), atol, rtol))
 
 
+@pytest.mark.parametrize("n", [40, 64, 130, 256])
+@pytest.mark.parametrize("terms", [3000, 70000])
+def test_simplify_word_ties_bit_exact(gpu, orc, n, terms):
+    # Keys wider than one word sort word-major; here every word takes only a
+    # few values, so long runs tie on the leading words and are ordered by the
+    # later ones (and equal keys keep construction order for their sums).
+    rng = np.random.default_rng(n * 7 + terms)
+    W = (n + 63) // 64
+    choices = []
+    for w in range(W):
+        width = min(64, n - 64 * w)
+        top = np.uint64(1) << np.uint64(width - 1)
+        mask = np.uint64((1 << width) - 1)
+        choices.append(np.array([0, 1, top, (top | np.uint64(5)) & mask], dtype=np.uint64))
+    x = np.stack([choices[w][rng.integers(0, 4, size=terms)] for w in range(W)], axis=1)
+    z = np.stack([choices[w][rng.integers(0, 3, size=terms)] for w in range(W)], axis=1)
+    c = rng.normal(size=terms) + 1j * rng.normal(size=terms)
+    op = PauliSum.from_packed(n, x, z, c)
+    assert_bit_exact(op.to_device().simplify(0.0, 0.0).to_host(), orc.simplify(*packed(op), 0.0, 0.0))
+    # Mostly distinct keys with a tied minority.
+    pool = random_pauli_masks(n, terms, 0.25, n + terms)
+    xs, zs = pool.x_words().copy(), pool.z_words().copy()
+    xs[: terms // 4, 0] = xs[terms // 4: terms // 2, 0]
+    op2 = PauliSum.from_packed(n, xs, zs, c)
+    assert_bit_exact(op2.to_device().simplify().to_host(), orc.simplify(*packed(op2)))
+
+
 def test_simplify_signed_zero_seed_matches_oracle(gpu, orc):
     for n in (1, 65):
         pool = random_pauli_masks(n, 4, 0.6, 3)

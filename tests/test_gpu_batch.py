@@ -74,3 +74,56 @@ def test_batch_with_coefficient_overrides(gpu, orc):
         np.testing.assert_allclose(
             b.sigma[i].cpu().numpy(), orc.apply(op.num_qubits, op.x_words(), op.z_words(), new_c[i], psi),
             rtol=0, atol=1e-12 * scale)
+
+
+# ------------------------------------------------------------ batched products
+
+def make_product_pairs(seed, count=9):
+    from wolfgang_amd.workloads import product_shapes
+
+    pairs = []
+    for i, (n, tl, tr) in enumerate(product_shapes(count, seed, t_range=(1, 400), n_range=(1, 256))):
+        a = random_pauli_masks(n, tl, 0.25, seed + 3 * i)
+        b = random_pauli_masks(n, tr, 0.25, seed + 3 * i + 1)
+        pairs.append((a, b))
+    # Duplicate-heavy operand (matmul's hash seed) and a zero-term pair.
+    pool = random_pauli_masks(70, 5, 0.3, seed)
+    idx = np.arange(4096) % 5
+    dup = type(pool).from_packed(70, pool.x_words()[idx], pool.z_words()[idx],
+                                 np.arange(4096, dtype=float) % 7 - 3)
+    pairs.append((dup, random_pauli_masks(70, 3, 0.3, seed + 99)))
+    pairs.append((random_pauli_masks(40, 0, 0.3, seed), random_pauli_masks(40, 7, 0.3, seed + 1)))
+    return pairs
+
+
+@pytest.mark.parametrize("budget", [0, 200_000])
+def test_product_batch_bit_exact_vs_matmul(gpu, orc, budget):
+    from conftest import assert_bit_exact
+    from wolfgang_amd.batch import ProductBatch
+
+    pairs = make_product_pairs(4242)
+    lhs = [a.to_device() for a, _ in pairs]
+    rhs = [b.to_device() for _, b in pairs]
+    pb = ProductBatch(lhs, rhs, budget_bytes=budget)
+    if budget:
+        assert pb.chunks > 1
+    for simplify in (True, False, True):  # repeated graph replays
+        outs = pb.run(simplify=simplify)
+        for (a, b), dl, dr, got in zip(pairs, lhs, rhs, outs):
+            host = got.to_host()
+            assert_bit_exact(host, packed(dl.matmul(dr, simplify=simplify).to_host()))
+            assert_bit_exact(host, orc.matmul(a.num_qubits, packed(a)[1:], packed(b)[1:], simplify))
+
+
+def test_product_batch_errors(gpu):
+    from wolfgang_amd.batch import ProductBatch
+
+    a = random_pauli_masks(8, 10, 0.3, 1).to_device()
+    b = random_pauli_masks(9, 10, 0.3, 2).to_device()
+    with pytest.raises(ValueError, match="same num_qubits"):
+        ProductBatch([a], [b])
+    c = random_pauli_masks(8, 10, 0.3, 3).to_device()
+    with pytest.raises(ValueError, match="exceeds max_intermediate_terms"):
+        ProductBatch([a], [c], max_intermediate_terms=99)
+    with pytest.raises(ValueError):
+        ProductBatch([], [])

@@ -164,3 +164,57 @@ def plain_forward_adjoint(batch: BatchedExpectation) -> None:
         )
         energies.append(e)
     batch.energies.copy_(torch.tensor(energies, dtype=torch.complex128))
+
+
+class ProductBatch:
+    """Many Hamiltonian products ``lhs[i] @ rhs[i]`` through one persistent,
+    queue-fed product kernel per chunk (captured in a HIP graph), then each
+    pair's simplify. ``run()`` returns one new ``DevicePauliSum`` per pair,
+    bit-identical to ``lhs[i].matmul(rhs[i], simplify)``. The operands must
+    stay alive while the batch exists."""
+
+    def __init__(
+        self,
+        lhs: Sequence[DevicePauliSum],
+        rhs: Sequence[DevicePauliSum],
+        max_intermediate_terms: int = 50_000_000,
+        budget_bytes: int = 0,
+    ) -> None:
+        if len(lhs) != len(rhs) or not lhs:
+            raise ValueError("lhs and rhs must be non-empty and of equal length")
+        self.lhs, self.rhs = list(lhs), list(rhs)
+        handle = C.c_void_p()
+        stream = torch.cuda.current_stream(torch.device("cuda", lhs[0].device)).cuda_stream
+        _native.call(
+            "wg_product_batch_create", len(lhs), _ptr_array([a._handle() for a in self.lhs]),
+            _ptr_array([b._handle() for b in self.rhs]), int(max_intermediate_terms),
+            int(budget_bytes), stream, C.byref(handle),
+        )
+        self._h = handle.value
+        self._device = lhs[0].device
+        pairs, chunks, total = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _native.call("wg_product_batch_info", self._h, C.byref(pairs), C.byref(chunks), C.byref(total))
+        self.chunks = chunks.value
+        self.total_products = total.value
+
+    def run(self, simplify: bool = True) -> list[DevicePauliSum]:
+        outs = (C.c_void_p * len(self.lhs))()
+        stream = torch.cuda.current_stream(torch.device("cuda", self._device)).cuda_stream
+        _native.call("wg_product_batch_run", self._h, int(bool(simplify)), stream, outs)
+        return [DevicePauliSum._wrap(outs[i]) for i in range(len(self.lhs))]
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _native.load().wg_product_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def plain_products(lhs: Sequence[DevicePauliSum], rhs: Sequence[DevicePauliSum],
+                   simplify: bool = True, max_intermediate_terms: int = 50_000_000) -> list[DevicePauliSum]:
+    """The same products pair by pair (one synchronous matmul call each)."""
+    return [a.matmul(b, simplify=simplify, max_intermediate_terms=max_intermediate_terms)
+            for a, b in zip(lhs, rhs)]

@@ -34,7 +34,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
+#include <utility>
 #include <numeric>
 #include <unordered_map>
 #include <vector>
@@ -341,6 +343,117 @@ struct OffDiagWork {
   }
 };
 
+// One off-diagonal item whose step signs take 2^D classes (ExpItem::cls_log):
+// steps are enumerated class by class, so a step only adds its pair product
+// to its class sum Q[class] (no per-step sign work); at the end of a tile,
+// key j gains (-1)^{parity(z_j & rep)} sum_a (-1)^{lam_j . a} Q[a]. Lane
+// points and the lane sign are as in OffDiagWork. Reads: own point at
+// f * 16 + G(e), partner at (f ^ x_lane) * 16 + (G(e) ^ xh16), where G(e) is
+// the uniform step offset (XOR of the enumeration basis offsets) and xh16 the
+// non-lane bits of x (pivot included).
+template <int D, int NZK>
+struct OffDiagClassWork {
+  static constexpr int NC = 1 << D;
+  static constexpr int B = NZK <= 2 ? 4 : 2;
+  int nz;
+  u32 zt[NZK];  // tile indices fit 32 bits on any state a GPU holds
+  u32 lam[NZK];
+  u32 gb[5];
+  u32 xh16;
+  u32 f16, fx16;
+  u32 lane_par;
+  double ar[NZK], ai[NZK];
+
+  __device__ OffDiagClassWork(const ExpItem& it, u64 free_mask, u32 lane) {
+    nz = it.nz;
+    const u64 pbit = it.pbit;
+    const u32 xl = static_cast<u32>(it.xl);
+    const u32 lanepos = static_cast<u32>(insert_zero(63, pbit));
+    const u32 f = static_cast<u32>(insert_zero(lane, pbit));
+    f16 = f << 4;
+    fx16 = (f ^ (xl & lanepos)) << 4;
+    xh16 = (xl & ~lanepos) << 4;
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      gb[l] = static_cast<u32>(insert_zero(static_cast<u64>(it.ebasis[l]) << 6, pbit)) << 4;
+    }
+    lane_par = 0;
+#pragma unroll
+    for (int j = 0; j < NZK; ++j) {
+      const bool on = j < nz;
+      zt[j] = on ? static_cast<u32>(compress_bits(it.z[j], free_mask)) : 0;
+      lam[j] = on ? it.lam[j] : 0;
+      lane_par |= static_cast<u32>(on && (__popc(static_cast<u32>(it.zl[j]) & f) & 1)) << j;
+      ar[j] = 0.0;
+      ai[j] = 0.0;
+    }
+  }
+
+  __device__ __forceinline__ void operator()(const double2* __restrict__ tile, u64 t64) {
+    // Steps of class c are e = (c << (5 - D)) | k, k < KS: offset
+    // G(e) = G_class(c) ^ G_kernel(k), G_kernel linear in the bits of k. The
+    // k loop is a runtime loop so that the 32 step offsets are formed per
+    // batch instead of being held in scalar registers. Each class sum goes to
+    // the keys as soon as it is complete.
+    constexpr int KS = kOffSteps >> D;
+    constexpr int kLogB = B == 4 ? 2 : 1;
+    const u32 t = static_cast<u32>(t64);
+    u32 flip = 0;  // bit j: parity(z_j & rep)
+#pragma unroll
+    for (int j = 0; j < NZK; ++j) flip |= static_cast<u32>(__popc(zt[j] & t) & 1) << j;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      u32 gc = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        if ((c >> i) & 1) gc ^= gb[5 - D + i];
+      }
+      double cr[B], ci[B];
+#pragma unroll
+      for (int s = 0; s < B; ++s) {
+        cr[s] = 0.0;
+        ci[s] = 0.0;
+      }
+#pragma unroll 1
+      for (int k = 0; k < KS; k += B) {
+        u32 gk = gc;
+        for (int l = kLogB; l < 5 - D; ++l) {
+          if ((k >> l) & 1) gk ^= gb[l];
+        }
+        double2 a[B], b[B];
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+          u32 g = gk;
+#pragma unroll
+          for (int l = 0; l < kLogB; ++l) {
+            if ((s >> l) & 1) g ^= gb[l];
+          }
+          a[s] = lds_at(tile, f16 + g);
+          b[s] = lds_at(tile, fx16 + (g ^ xh16));
+        }
+#pragma unroll
+        for (int s = 0; s < B; ++s) {
+          cr[s] = fma(b[s].x, a[s].x, fma(b[s].y, a[s].y, cr[s]));
+          ci[s] = fma(b[s].x, a[s].y, fma(-b[s].y, a[s].x, ci[s]));
+        }
+      }
+#pragma unroll
+      for (int s = 1; s < B; ++s) {
+        cr[0] += cr[s];
+        ci[0] += ci[s];
+      }
+#pragma unroll
+      for (int j = 0; j < NZK; ++j) {
+        if (j < nz) {
+          const bool neg = ((flip >> j) ^ static_cast<u32>(__popc(lam[j] & static_cast<u32>(c)))) & 1U;
+          ar[j] += neg ? -cr[0] : cr[0];
+          ai[j] += neg ? -ci[0] : ci[0];
+        }
+      }
+    }
+  }
+};
+
 // One diagonal item (x = 0) on one wave: lane points u = lane + 64 s.
 template <int NZ>
 struct DiagWork {
@@ -390,6 +503,29 @@ struct DiagWork {
   }
 };
 
+template <int D, int NZK>
+__device__ __forceinline__ void run_class_item(const double2* __restrict__ psi,
+                                               const PassDesc* __restrict__ pd, const ExpItem& it,
+                                               u32 wave, u32 lane, u64 t0, u32 tile_blocks,
+                                               double2* __restrict__ tile,
+                                               double2* __restrict__ row) {
+  OffDiagClassWork<D, NZK> w(it, pd->free_mask, lane);
+  sweep_tiles(psi, pd, wave, t0, tile_blocks, tile, w);
+#pragma unroll
+  for (int j = 0; j < NZK; ++j) {
+    if (j < w.nz) {
+      const bool neg = (w.lane_par >> j) & 1U;
+      double r = neg ? -w.ar[j] : w.ar[j];
+      double i = neg ? -w.ai[j] : w.ai[j];
+      for (int d = 32; d > 0; d >>= 1) {
+        r += __shfl_xor(r, d, 64);
+        i += __shfl_xor(i, d, 64);
+      }
+      if (lane == 0) row[j] = make_double2(r, i);
+    }
+  }
+}
+
 template <int NZ, bool kDiag>
 __device__ __forceinline__ void run_item(const double2* __restrict__ psi,
                                          const PassDesc* __restrict__ pd, const ExpItem& it,
@@ -406,12 +542,24 @@ __device__ __forceinline__ void run_item(const double2* __restrict__ psi,
   }
 }
 
+// Sweep variants: one kernel instance per kind of item, so that each
+// instance's registers are allocated for one work type.
+//   [0, 8):   off-diagonal, per-step signs, nz = v + 1
+//   [8, 16):  diagonal, nz = v - 7
+//   [16, 24): off-diagonal with step classes, D = (v - 16) / 2, up to 2
+//             (even v) or kZ (odd v) keys
+constexpr int kVariants = 24;
+
+int sweep_variant(const ExpItem& it) { return it.variant; }
+
 // Grid: item_groups * tile_blocks blocks. Block -> (item group, tile block)
 // such that the blocks sharing a tile block differ by multiples of 8 in their
 // linear id: they are dispatched to the same XCD close together, so the later
 // reads of a tile hit that XCD's L2 instead of HBM. Wave w of a block applies
-// item (group * 8 + w) to every tile of its tile block.
+// item (group * 8 + w) to every tile of its tile block; every item of a launch
+// is of sweep variant V.
 // Two blocks per CU (LDS 2 x 64 KiB) need 4 waves per SIMD: <= 128 VGPRs.
+template <int V>
 __global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void expect_tiled_kernel(
     const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
@@ -435,17 +583,26 @@ void expect_tiled_kernel(
   const ExpItem& it = items[item_id];
   double2* row = partials + (static_cast<std::size_t>(partial_base) +
                              static_cast<std::size_t>(item_id) * tile_blocks + tb) * kZ;
-#define WG_ITEM(N)                                                                   \
-  case N:                                                                            \
-    if (it.xl) run_item<N, false>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row); \
-    else run_item<N, true>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row);      \
-    break;
-  switch (it.nz) {
-    WG_ITEM(1) WG_ITEM(2) WG_ITEM(3) WG_ITEM(4) WG_ITEM(5) WG_ITEM(6) WG_ITEM(7)
-    default: WG_ITEM(8)
+  if constexpr (V < 8) {
+    run_item<V + 1, false>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row);
+  } else if constexpr (V < 16) {
+    run_item<V - 7, true>(psi, pd, it, wave, lane, tb, tile_blocks, tile, row);
+  } else {
+    run_class_item<(V - 16) / 2, ((V - 16) & 1) ? kZ : 2>(psi, pd, it, wave, lane, tb,
+                                                           tile_blocks, tile, row);
   }
-#undef WG_ITEM
 }
+
+using TiledKernel = void (*)(const double2*, const PassDesc*, const ExpItem*, u32, u32, u32, u32,
+                             double2*);
+
+template <int... V>
+constexpr std::array<TiledKernel, sizeof...(V)> tiled_kernel_table(std::integer_sequence<int, V...>) {
+  return {&expect_tiled_kernel<V>...};
+}
+
+const std::array<TiledKernel, kVariants> kTiledKernels =
+    tiled_kernel_table(std::make_integer_sequence<int, kVariants>{});
 
 // ------------------------------------------------------------ diagonal keys
 
@@ -831,6 +988,95 @@ struct Span {
 
 }  // namespace
 
+namespace {
+
+u64 host_insert_zero(u64 v, u64 pivot) {
+  const u64 low = pivot - 1;
+  return ((v & ~low) << 1) | (v & low);
+}
+
+int host_parity(u64 v) { return __builtin_popcountll(v) & 1; }
+
+// Step classes of an off-diagonal tiled item (see ExpItem): the step
+// characters c_j (bit i = parity(zl_j & g(2^i)), g(s) = insert_zero(s << 6,
+// pivot)) span a space of dimension d; with a basis b_1..b_d, class(s) =
+// (b_i . s)_i and c_j = sum_i lam_j[i] b_i. The enumeration basis is a basis
+// of the kernel of s -> class(s) (low 5 - d positions) followed by vectors v_i
+// with class(v_i) = e_i (top d positions).
+void assign_step_classes(ExpItem& e, int min_d) {
+  e.cls_log = kNoClasses;
+  if (e.xl == 0) return;
+  constexpr int kStepBits = 5;
+  static_assert((1 << kStepBits) == kOffSteps, "five step bits per off-diagonal tile sweep");
+  u32 c[kZ] = {};
+  for (int j = 0; j < e.nz; ++j) {
+    for (int i = 0; i < kStepBits; ++i) {
+      const u64 g = host_insert_zero(u64{1} << (i + 6), e.pbit);
+      c[j] |= static_cast<u32>(host_parity(e.zl[j] & g)) << i;
+    }
+  }
+  // Basis of span{c_j} (in the order first met) and each c_j's coordinates.
+  std::vector<u32> basis;
+  auto coords = [&](u32 v, u32* lam) -> bool {
+    for (u32 m = 0; m < (1U << basis.size()); ++m) {
+      u32 acc = 0;
+      for (std::size_t i = 0; i < basis.size(); ++i) {
+        if ((m >> i) & 1U) acc ^= basis[i];
+      }
+      if (acc == v) {
+        *lam = m;
+        return true;
+      }
+    }
+    return false;
+  };
+  u32 lam[kZ] = {};
+  for (int j = 0; j < e.nz; ++j) {
+    if (!coords(c[j], &lam[j])) {
+      if (static_cast<int>(basis.size()) == kMaxClassLog) return;  // too many classes
+      basis.push_back(c[j]);
+      coords(c[j], &lam[j]);
+    }
+  }
+  // Extra class dimensions (zero in every lam) up to min_d, so that all items
+  // of a pass can share one kernel instance.
+  for (u32 unit = 1; static_cast<int>(basis.size()) < std::min(min_d, kMaxClassLog) &&
+                     unit < (1U << kStepBits);
+       unit <<= 1) {
+    u32 probe = 0;
+    if (!coords(unit, &probe)) basis.push_back(unit);
+  }
+  const int d = static_cast<int>(basis.size());
+  auto cls = [&](u32 s) {
+    u32 a = 0;
+    for (int i = 0; i < d; ++i) a |= static_cast<u32>(host_parity(basis[i] & s)) << i;
+    return a;
+  };
+  std::vector<u32> kern;  // basis of the kernel, by elimination on the top bit
+  for (u32 s = 1; s < (1U << kStepBits); ++s) {
+    if (cls(s)) continue;
+    u32 w = s;
+    for (u32 b : kern) {
+      if ((w >> (31 - __builtin_clz(b))) & 1U) w ^= b;
+    }
+    if (w) kern.push_back(w);
+  }
+  if (static_cast<int>(kern.size()) != kStepBits - d) fail_runtime("step class kernel has the wrong dimension");
+  for (int l = 0; l < kStepBits - d; ++l) e.ebasis[l] = static_cast<unsigned char>(kern[l]);
+  for (int i = 0; i < d; ++i) {
+    u32 v = 0;
+    for (u32 s = 1; s < (1U << kStepBits) && !v; ++s) {
+      if (cls(s) == (1U << i)) v = s;
+    }
+    if (!v) fail_runtime("step class has no representative");
+    e.ebasis[kStepBits - d + i] = static_cast<unsigned char>(v);
+  }
+  for (int j = 0; j < kZ; ++j) e.lam[j] = static_cast<unsigned char>(j < e.nz ? lam[j] : 0);
+  e.cls_log = static_cast<unsigned char>(d);
+}
+
+}  // namespace
+
 void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
                        bool diag_separate, u32* num_diag_front) {
   const int l = kTileLowLog;
@@ -1003,17 +1249,60 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
         if (is_diag ? p != 0 : pass_of.at(it.x) != static_cast<int>(p)) continue;
         ExpItem e = it;
         e.xl = is_diag ? 0 : x_local(it.x);
-        e.pbit = e.xl & (0 - e.xl);
+        // Pivot: the highest local bit of x. Any set bit of x selects one
+        // representative per pair {b, b ^ x}; a pivot above the low 6 bits
+        // keeps each wave's LDS reads on 64 consecutive points.
+        e.pbit = is_diag ? 0 : (1ULL << (63 - __builtin_clzll(e.xl)));
         for (int j = 0; j < e.nz; ++j) e.zl[j] = z_local(it.z[j]);
+        assign_step_classes(e, 0);
         ordered.push_back(e);
       }
     }
     // Waves of a block wait for each other at every tile: group items of
     // similar cost (key count) into the same block.
+    // Sweep variants: the off-diagonal items with step classes share the
+    // pass's largest class dimension and key count, so a pass usually needs
+    // one launch; others get their own instance.
+    {
+      int dmax = -1;
+      bool wide = false;
+      for (std::size_t i = pl.first_item; i < ordered.size(); ++i) {
+        const ExpItem& it = ordered[i];
+        if (it.xl && it.cls_log != kNoClasses) {
+          dmax = std::max<int>(dmax, it.cls_log);
+          wide = wide || it.nz > 2;
+        }
+      }
+      for (std::size_t i = pl.first_item; i < ordered.size(); ++i) {
+        ExpItem& it = ordered[i];
+        if (it.xl == 0) {
+          it.variant = static_cast<unsigned char>(8 + it.nz - 1);
+        } else if (it.cls_log != kNoClasses) {
+          assign_step_classes(it, dmax);
+          if (it.cls_log != dmax) fail_runtime("step classes could not be widened");
+          it.variant = static_cast<unsigned char>(16 + 2 * dmax + (wide ? 1 : 0));
+        } else {
+          it.variant = static_cast<unsigned char>(it.nz - 1);
+        }
+      }
+    }
+    // One launch per sweep variant present in the pass.
     std::stable_sort(ordered.begin() + pl.first_item, ordered.end(),
-                     [](const ExpItem& a, const ExpItem& b) { return a.nz > b.nz; });
+                     [](const ExpItem& a, const ExpItem& b) {
+                       return sweep_variant(a) < sweep_variant(b);
+                     });
     pl.num_items = static_cast<u32>(ordered.size()) - pl.first_item;
-    pl.item_groups = std::max<u32>(1, (pl.num_items + kTileWaves - 1) / kTileWaves);
+    pl.item_groups = 0;
+    for (u32 i = 0; i < pl.num_items;) {
+      const int v = sweep_variant(ordered[pl.first_item + i]);
+      u32 e = i;
+      while (e < pl.num_items && sweep_variant(ordered[pl.first_item + e]) == v) ++e;
+      const u32 groups = (e - i + kTileWaves - 1) / kTileWaves;
+      pl.subs.push_back({v, i, e - i, groups});
+      pl.item_groups += groups;
+      i = e;
+    }
+    pl.item_groups = std::max<u32>(1, pl.item_groups);
     const u64 ntiles = 1ULL << (n - d.k);
     u64 tb = (1024 + pl.item_groups - 1) / pl.item_groups;
     tb = std::max<u64>(8, (tb + 7) / 8 * 8);
@@ -1276,13 +1565,15 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
         check_launch("expect_diag_wht");
       }
       for (const PassLaunch& p : plan.passes) {
-        if (p.num_items == 0) continue;
-        const unsigned grid = p.item_groups * p.tile_blocks;
-        ProfileScope prof("expect_tiled", stream);
-        expect_tiled_kernel<<<grid, kTileBlock, 0, stream>>>(
-            psi, plan.d_passes + (&p - plan.passes.data()), plan.d_items + p.first_item,
-            p.num_items, p.item_groups, p.tile_blocks, p.partial_base, partials);
-        check_launch("expect_tiled");
+        for (const SubLaunch& sub : p.subs) {
+          const unsigned grid = sub.groups * p.tile_blocks;
+          ProfileScope prof("expect_tiled", stream);
+          kTiledKernels[sub.variant]<<<grid, kTileBlock, 0, stream>>>(
+              psi, plan.d_passes + (&p - plan.passes.data()), plan.d_items + p.first_item + sub.first,
+              sub.count, sub.groups, p.tile_blocks, p.partial_base + sub.first * p.tile_blocks,
+              partials);
+          check_launch("expect_tiled");
+        }
       }
     } else {
       const u64 len = u64{1} << plan.n;
@@ -1603,6 +1894,36 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
           };
           const u32 span = xl ? tile_size / 2 : tile_size;
           const u32 steps = span > 64 ? span / 64 : 1;
+          if (xl && it.cls_log != kNoClasses && steps == kOffSteps) {
+            // The kernel's class decomposition: steps enumerated through
+            // ebasis, class = e >> (5 - D), class sums folded with lam.
+            const int D = it.cls_log;
+            for (u32 lane = 0; lane < 64; ++lane) {
+              const u32 f = ins0(lane);
+              double qr[1 << kMaxClassLog] = {}, qi[1 << kMaxClassLog] = {};
+              for (int e = 0; e < kOffSteps; ++e) {
+                u32 st = 0;
+                for (int l = 0; l < 5; ++l) {
+                  if ((e >> l) & 1) st ^= it.ebasis[l];
+                }
+                const u32 u = f ^ ins0(st << 6);
+                const double2 A = tile[u], B = tile[u ^ xl];
+                qr[e >> (5 - D)] += B.x * A.x + B.y * A.y;
+                qi[e >> (5 - D)] += B.x * A.y - B.y * A.x;
+              }
+              for (int j = 0; j < it.nz; ++j) {
+                const u32 lane_par = __builtin_popcount(static_cast<u32>(it.zl[j]) & f) & 1;
+                const u32 srep = __builtin_popcountll(it.z[j] & rep) & 1;
+                double2& acc2 = acc[static_cast<std::size_t>(i) * kZ + j];
+                for (int c = 0; c < (1 << D); ++c) {
+                  const u32 neg = lane_par ^ srep ^ (__builtin_popcount(it.lam[j] & c) & 1);
+                  acc2.x += neg ? -qr[c] : qr[c];
+                  acc2.y += neg ? -qi[c] : qi[c];
+                }
+              }
+            }
+            continue;
+          }
           for (int j = 0; j < it.nz; ++j) {
             const u32 zlj = static_cast<u32>(it.zl[j]);
             const u32 srep = __builtin_popcountll(it.z[j] & rep) & 1;

@@ -10,6 +10,9 @@ namespace wg {
 
 // Keys per work item: one x-group's amplitude products feed up to kZ z-masks.
 constexpr int kZ = 8;
+// Largest step-class dimension of the tiled off-diagonal sweep.
+constexpr int kMaxClassLog = 3;
+constexpr unsigned char kNoClasses = 0xFF;
 
 // Largest tile of the tiled pass: 2^kTileMaxLog amplitudes (64 KiB of complex128
 // in LDS, two blocks per CU) and the contiguous low-bit run inside it
@@ -24,7 +27,15 @@ struct ExpItem {
   u64 xl;       // x in tile-local coordinates (tiled path)
   u64 pbit;     // local pivot bit of xl (tiled path)
   int nz;
-  int pad;
+  // Tiled path, off-diagonal items: the step signs of key j are
+  // (-1)^{lam[j] . class(s)} with class(s) in {0,1}^cls_log; steps are
+  // enumerated as e -> s = XOR of ebasis[l] over the set bits l of e, so that
+  // class(s) = e >> (5 - cls_log). cls_log = kNoClasses: not decomposed.
+  unsigned char cls_log;
+  unsigned char ebasis[5];
+  unsigned char lam[kZ];
+  unsigned char variant;  // tiled path: the sweep kernel instance (set per pass)
+  unsigned char pad;
   u64 z[kZ];    // global z masks
   u64 zl[kZ];   // z in tile-local coordinates: parity(z & b) = parity(z & rep) ^ parity(zl & u)
   u32 key[kZ];  // distinct-key ids
@@ -46,13 +57,22 @@ struct PassDesc {
   unsigned char free_pos[64];
 };
 
+// Items of one sweep variant inside a pass: one launch.
+struct SubLaunch {
+  int variant;
+  u32 first;   // relative to the pass's first item
+  u32 count;
+  u32 groups;  // blocks of kTileWaves items
+};
+
 struct PassLaunch {
   PassDesc desc;
   u32 first_item;    // items of this pass are [first_item, first_item + num_items)
   u32 num_items;
-  u32 item_groups;   // blocks of kTileWaves items
+  u32 item_groups;   // blocks of kTileWaves items, over all sub-launches
   u32 tile_blocks;   // blocks per item group (multiple of 8)
   u32 partial_base;  // first partial row of this pass
+  std::vector<SubLaunch> subs;
 };
 
 struct ExpectationPlan {

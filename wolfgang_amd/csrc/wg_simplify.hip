@@ -8,10 +8,13 @@
 // Pipeline (all on one stream):
 //   1. key statistics: bitwise OR / AND of every key word over all terms, so
 //      that 8-bit digits that are constant across the operator are skipped;
-//   2. sort records: the key words (packed into one uint64 as x<<32|z when
-//      every mask fits in 32 bits) plus a uint32 construction index;
-//   3. LSD radix sort, 8-bit digits, least significant digit first. Each pass
-//      is histogram -> exclusive scan -> stable scatter. Stability within a
+//   2. sort records: one key word plus a uint32 construction index. Keys
+//      that fit one word (x<<32|z when every mask fits in 32 bits) sort in
+//      one LSD radix sort; wider keys sort word-major: all items by x0, then
+//      only the items whose leading words tie by the next word (records
+//      [run rank, word]), written back into the tied positions;
+//   3. each LSD radix pass uses 8-bit digits (constant digits skipped) and is
+//      histogram -> exclusive scan -> stable scatter. Stability within a
 //      4096-key tile comes from wave-level digit matching (ballots) and
 //      per-wave digit counts in LDS, so equal keys keep construction order;
 //   4. segment heads, one thread per segment summing its coefficients in
@@ -183,22 +186,6 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
 
 // ---------------------------------------------------------------- segments
 
-__device__ inline bool key_equal(const u64* keys, int KW, std::size_t a, std::size_t b) {
-  for (int k = 0; k < KW; ++k) {
-    if (keys[a * KW + k] != keys[b * KW + k]) return false;
-  }
-  return true;
-}
-
-__global__ __launch_bounds__(256) void head_flags_kernel(const u64* __restrict__ keys,
-                                                         std::size_t T, int KW,
-                                                         u32* __restrict__ flags) {
-  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    flags[i] = (i == 0 || !key_equal(keys, KW, i, i - 1)) ? 1U : 0U;
-  }
-}
-
 __global__ __launch_bounds__(256) void segment_starts_kernel(const u32* __restrict__ flags,
                                                              const u32* __restrict__ seg_pos,
                                                              std::size_t T,
@@ -252,6 +239,124 @@ __global__ __launch_bounds__(256) void compact_kernel(
         ox[o * W + w] = keys[k * KW + 2 * w];
         oz[o * W + w] = keys[k * KW + 2 * w + 1];
       }
+    }
+    oc[o] = sums[s];
+  }
+}
+
+// ---------------------------------------------------------------- word-major sort
+//
+// Keys wider than one word are sorted most significant word first: a stable
+// LSD radix sort of (word 0, index) records, then, only for the items whose
+// word-0 values tie, a stable sort of (run, next word) records written back
+// into the tied positions, and so on word by word. Records carry one key word
+// instead of the whole key, and for distinct-ish keys (random products) the
+// later words touch few items.
+
+__device__ inline u64 key_word(const u64* __restrict__ x, const u64* __restrict__ z,
+                               std::size_t W, u32 t, int w) {
+  const std::size_t at = static_cast<std::size_t>(t) * W + static_cast<std::size_t>(w >> 1);
+  return (w & 1) ? z[at] : x[at];
+}
+
+__global__ __launch_bounds__(256) void word_records_kernel(const u64* __restrict__ x,
+                                                           const u64* __restrict__ z,
+                                                           std::size_t T, std::size_t W,
+                                                           u64* __restrict__ keys,
+                                                           u32* __restrict__ idx) {
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    keys[t] = x[t * W];
+    idx[t] = static_cast<u32>(t);
+  }
+}
+
+// head[i] = 1 where a run of equal keys starts (KW-word records).
+__global__ __launch_bounds__(256) void run_heads_kernel(const u64* __restrict__ keys,
+                                                        std::size_t n, int KW,
+                                                        u32* __restrict__ head) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    bool h = i == 0;
+    if (!h) {
+      for (int k = 0; k < KW; ++k) h = h || keys[i * KW + k] != keys[(i - 1) * KW + k];
+    }
+    head[i] = h ? 1U : 0U;
+  }
+}
+
+// tied[i] = 1 when item i's run has at least two items.
+__global__ __launch_bounds__(256) void tied_kernel(const u32* __restrict__ head, std::size_t n,
+                                                   u32* __restrict__ tied) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    tied[i] = (!head[i] || (i + 1 < n && !head[i + 1])) ? 1U : 0U;
+  }
+}
+
+// Refinement records for the tied items of the current list (list == null:
+// the identity over all positions): global position, run rank (monotone in
+// position) and word w of the item's term.
+__global__ __launch_bounds__(256) void refine_records_kernel(
+    const u32* __restrict__ list, std::size_t n, const u32* __restrict__ tied,
+    const u32* __restrict__ tpos, const u32* __restrict__ head, const u32* __restrict__ hscan,
+    const u32* __restrict__ idx, const u64* __restrict__ x, const u64* __restrict__ z,
+    std::size_t W, int w, u32* __restrict__ plist, u64* __restrict__ keys2,
+    u32* __restrict__ idx2) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (!tied[i]) continue;
+    const u32 j = tpos[i];
+    const u32 pos = list ? list[i] : static_cast<u32>(i);
+    const u32 t = idx[pos];
+    plist[j] = pos;
+    keys2[2 * static_cast<std::size_t>(j)] = hscan[i] + head[i] - 1U;
+    keys2[2 * static_cast<std::size_t>(j) + 1] = key_word(x, z, W, t, w);
+    idx2[j] = t;
+  }
+}
+
+__global__ __launch_bounds__(256) void refine_scatter_kernel(const u32* __restrict__ plist,
+                                                             const u32* __restrict__ idx2,
+                                                             std::size_t n, u32* __restrict__ idx) {
+  for (std::size_t j = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    idx[plist[j]] = idx2[j];
+  }
+}
+
+// Segment heads of the fully sorted order: word 0 from the sorted word-0 keys,
+// the other words compared through the terms only where word 0 ties.
+__global__ __launch_bounds__(256) void full_heads_kernel(const u64* __restrict__ k0,
+                                                         const u32* __restrict__ idx,
+                                                         const u64* __restrict__ x,
+                                                         const u64* __restrict__ z, std::size_t W,
+                                                         std::size_t T, u32* __restrict__ flags) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    bool h = i == 0 || k0[i] != k0[i - 1];
+    if (!h) {
+      const std::size_t a = static_cast<std::size_t>(idx[i]) * W;
+      const std::size_t b = static_cast<std::size_t>(idx[i - 1]) * W;
+      for (std::size_t w = 0; w < W && !h; ++w) h = x[a + w] != x[b + w] || z[a + w] != z[b + w];
+    }
+    flags[i] = h ? 1U : 0U;
+  }
+}
+
+__global__ __launch_bounds__(256) void compact_gather_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    const u32* __restrict__ out_pos, const u32* __restrict__ idx, const u64* __restrict__ x,
+    const u64* __restrict__ z, std::size_t W, const double2* __restrict__ sums,
+    u64* __restrict__ ox, u64* __restrict__ oz, double2* __restrict__ oc) {
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (!keep[s]) continue;
+    const std::size_t o = out_pos[s];
+    const std::size_t t = idx[seg_start[s]];
+    for (std::size_t w = 0; w < W; ++w) {
+      ox[o * W + w] = x[t * W + w];
+      oz[o * W + w] = z[t * W + w];
     }
     oc[o] = sums[s];
   }
@@ -333,6 +438,49 @@ bool operand_duplicate_heavy(const wg_pauli_sum* op, hipStream_t stream) {
   return duplicate_rate(hx, hz, op->words, sample) >= 0.25;
 }
 
+namespace {
+
+struct DigitPass {
+  int word, shift;
+};
+
+// 8-bit digit passes over KW-word records, least significant first; a digit
+// is skipped when none of its bits varies (varying[k]: bits of word k that
+// differ somewhere; word 0 is the most significant).
+std::vector<DigitPass> digit_passes(const std::vector<u64>& varying) {
+  std::vector<DigitPass> passes;
+  for (int k = static_cast<int>(varying.size()) - 1; k >= 0; --k) {
+    for (int shift = 0; shift < 64; shift += 8) {
+      if ((varying[k] >> shift) & 0xFFULL) passes.push_back({k, shift});
+    }
+  }
+  return passes;
+}
+
+// Stable LSD radix sort of (KW-word key, index) records; the sorted records
+// end up in (keys_a, idx_a).
+void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_t n, int KW,
+                const std::vector<DigitPass>& passes, Scratch& scratch, hipStream_t stream) {
+  if (passes.empty() || n < 2) return;
+  const std::size_t nblocks = (n + kTile - 1) / kTile;
+  u32* hist = scratch.take<u32>(256 * nblocks);
+  u32* offs = scratch.take<u32>(256 * nblocks);
+  for (const DigitPass& p : passes) {
+    radix_hist_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(keys_a, n, KW, p.word,
+                                                                   p.shift, nblocks, hist);
+    check_launch("radix_hist");
+    exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
+    ProfileScope prof("radix_scatter", stream);
+    radix_scatter_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+        keys_a, idx_a, keys_b, idx_b, n, KW, p.word, p.shift, nblocks, offs);
+    check_launch("radix_scatter");
+    std::swap(keys_a, keys_b);
+    std::swap(idx_a, idx_b);
+  }
+}
+
+}  // namespace
+
 wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std::size_t n,
                               std::size_t T, int device, double atol, double rtol, int seed_mode,
                               Scratch& scratch, hipStream_t stream) {
@@ -380,64 +528,84 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
               "copy device buffer to host");
     sync(stream);
   }
-  int KW;
-  std::vector<u64> kor, kand;
-  if (W == 0) {
-    KW = 1;
-    kor = {0};
-    kand = {0};
-  } else if (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0) {
-    KW = 1;
-    kor = {(ors[0] << 32) | ors[1]};
-    kand = {(ands[0] << 32) | ands[1]};
-  } else {
-    KW = static_cast<int>(2 * W);
-    kor.assign(ors.begin(), ors.end());
-    kand.assign(ands.begin(), ands.end());
-  }
-  struct Pass {
-    int word, shift;
-  };
-  std::vector<Pass> passes;
-  for (int k = KW - 1; k >= 0; --k) {
-    const u64 varying = kor[k] ^ kand[k];
-    for (int shift = 0; shift < 64; shift += 8) {
-      if ((varying >> shift) & 0xFFULL) passes.push_back({k, shift});
-    }
-  }
-
-  u64* keys_a = scratch.take<u64>(checked_mul(T, KW, "simplify keys"));
-  u64* keys_b = passes.empty() ? nullptr : scratch.take<u64>(T * KW);
+  // Segments of equal keys in sorted order, as positions into idx; the
+  // word-0 keys (one-word keys: the whole key) in sorted order.
+  const bool packed = W == 0 || (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0);
+  u32* flags = scratch.take<u32>(T);
+  u64* keys_a = scratch.take<u64>(T);
   u32* idx_a = scratch.take<u32>(T);
-  u32* idx_b = passes.empty() ? nullptr : scratch.take<u32>(T);
-  build_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, KW, keys_a, idx_a);
-  check_launch("build_records");
-
-  if (!passes.empty()) {
-    const std::size_t nblocks = (T + kTile - 1) / kTile;
-    u32* hist = scratch.take<u32>(256 * nblocks);
-    u32* offs = scratch.take<u32>(256 * nblocks);
-    for (const Pass& p : passes) {
-      radix_hist_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(keys_a, T, KW, p.word,
-                                                                     p.shift, nblocks, hist);
-      check_launch("radix_hist");
-      exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
-      ProfileScope prof("radix_scatter", stream);
-      radix_scatter_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
-          keys_a, idx_a, keys_b, idx_b, T, KW, p.word, p.shift, nblocks, offs);
-      check_launch("radix_scatter");
-      std::swap(keys_a, keys_b);
-      std::swap(idx_a, idx_b);
+  u64* keys_b = scratch.take<u64>(T);
+  u32* idx_b = scratch.take<u32>(T);
+  if (packed) {
+    // One-word keys: x << 32 | z (or 0 for zero-qubit operators).
+    const u64 kor = W ? (ors[0] << 32) | ors[1] : 0;
+    const u64 kand = W ? (ands[0] << 32) | ands[1] : 0;
+    build_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, 1, keys_a, idx_a);
+    check_launch("build_records");
+    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
+    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, 1, flags);
+    check_launch("run_heads");
+  } else {
+    // Word 0 (x0) for every item, then the tied items word by word.
+    word_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, keys_a, idx_a);
+    check_launch("word_records");
+    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
+               stream);
+    u64* k0 = keys_a;
+    u32* idx = idx_a;
+    u32* head = flags;
+    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, T, 1, head);
+    check_launch("run_heads");
+    const u32* list = nullptr;  // current list of positions (null: all of [0, T))
+    std::size_t n_cur = T;
+    u32* tied = scratch.take<u32>(T);
+    u32* tpos = scratch.take<u32>(T);
+    u32* hscan = scratch.take<u32>(T);
+    u32* d_tied = scratch.take<u32>(1);
+    for (std::size_t w = 1; w < 2 * W; ++w) {
+      tied_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(head, n_cur, tied);
+      check_launch("tied");
+      exclusive_scan_u32(tied, tpos, n_cur, d_tied, scratch, stream);
+      u32 A = 0;
+      check_hip(hipMemcpyAsync(&A, d_tied, sizeof(u32), hipMemcpyDeviceToHost, stream),
+                "copy device buffer to host");
+      sync(stream);
+      if (A == 0) break;
+      exclusive_scan_u32(head, hscan, n_cur, nullptr, scratch, stream);
+      u32* plist = scratch.take<u32>(A);
+      u64* r_keys_a = scratch.take<u64>(2 * static_cast<std::size_t>(A));
+      u64* r_keys_b = scratch.take<u64>(2 * static_cast<std::size_t>(A));
+      u32* r_idx_a = scratch.take<u32>(A);
+      u32* r_idx_b = scratch.take<u32>(A);
+      refine_records_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(
+          list, n_cur, tied, tpos, head, hscan, idx, x, z, W, static_cast<int>(w), plist, r_keys_a,
+          r_idx_a);
+      check_launch("refine_records");
+      // Records [run, word w]: word w's varying bits, then the run ranks
+      // (all bits below the bit length of n_cur).
+      int run_bits = 0;
+      while (run_bits < 64 && (u64{1} << run_bits) < n_cur) ++run_bits;
+      const u64 run_varying = run_bits >= 64 ? ~0ULL : ((u64{1} << run_bits) - 1);
+      radix_sort(r_keys_a, r_idx_a, r_keys_b, r_idx_b, A, 2,
+                 digit_passes({run_varying, ors[w] ^ ands[w]}), scratch, stream);
+      refine_scatter_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, r_idx_a, A, idx);
+      check_launch("refine_scatter");
+      // The next level works on the tied items, in their new order.
+      u32* head2 = scratch.take<u32>(A);
+      run_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(r_keys_a, A, 2, head2);
+      check_launch("run_heads");
+      head = head2;
+      list = plist;
+      n_cur = A;
     }
+    full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, x, z, W, T, flags);
+    check_launch("full_heads");
   }
 
   // Segments.
-  u32* flags = scratch.take<u32>(T);
   u32* seg_pos = scratch.take<u32>(T);
   u32* seg_start = scratch.take<u32>(T + 1);
   u32* d_count = scratch.take<u32>(2);
-  head_flags_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, KW, flags);
-  check_launch("head_flags");
   exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
   segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
   check_launch("segment_starts");
@@ -459,9 +627,13 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   sync(stream);
 
   std::unique_ptr<wg_pauli_sum> out(new_pauli_sum(n, U, device));
-  if (U) {
-    compact_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, keys_a, KW, W,
+  if (U && packed) {
+    compact_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, keys_a, 1, W,
                                                        sums, out->x, out->z, out->coeffs);
+    check_launch("compact");
+  } else if (U) {
+    compact_gather_kernel<<<stride_grid(S), 256, 0, stream>>>(
+        seg_start, S, keep, out_pos, idx_a, x, z, W, sums, out->x, out->z, out->coeffs);
     check_launch("compact");
   }
   sync(stream);

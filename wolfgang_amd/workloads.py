@@ -87,3 +87,16 @@ def batch_shapes(count: int, seed: int, n_range=(12, 22), t_range=(100, 50_000))
     n = rng.integers(n_range[0], n_range[1] + 1, size=count)
     t = np.rint(np.exp(rng.uniform(math.log(t_range[0]), math.log(t_range[1]), size=count))).astype(int)
     return [(int(a), int(b)) for a, b in zip(n, t)]
+
+
+def product_shapes(count: int, seed: int, t_range=(500, 8_000), n_range=(64, 256)) -> list[tuple[int, int, int]]:
+    """(num_qubits, lhs terms, rhs terms) for batched products: n ~ U{lo..hi},
+    terms ~ U{lo..hi} per operand."""
+    rng = np.random.default_rng([seed, 1])
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        tl = int(rng.integers(t_range[0], t_range[1] + 1))
+        tr = int(rng.integers(t_range[0], t_range[1] + 1))
+        out.append((n, tl, tr))
+    return out
