@@ -48,9 +48,10 @@ def parse_args() -> argparse.Namespace:
     p.add_argument(
         "--workload",
         default="expectation",
-        choices=["expectation", "commutation", "matmul", "batch", "adjoint", "copy"],
+        choices=["expectation", "commutation", "matmul", "batch", "products", "adjoint", "copy"],
         help="expectation (the headline line, default); commutation / matmul: the 128-qubit "
-        "algebra workloads; batch: batched forward+adjoint vs the per-instance path; adjoint: "
+        "algebra workloads; batch: batched forward+adjoint vs the per-instance path; products: "
+        "64 batched Hamiltonian products vs per-pair matmul; adjoint: "
         "forward + fused adjoint at 24 qubits, 20,000 terms; copy: a "
         "plain device copy of known size, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE",
     )
@@ -66,10 +67,12 @@ def distinct_x_groups(op) -> int:
     return nonzero + (1 if (xs == 0).any() else 0)
 
 
-def cpu_baseline(op, psi_dev, sample_terms: int) -> dict:
+def cpu_baseline(op, psi_dev, sample_terms: int, gpu_term_values=None) -> dict:
     """Time the reference's scalar expectation on `sample_terms` terms of the
     same Hamiltonian (the first terms of each kind, in term order) over the full
-    statevector copied to host memory; rate in term*amplitude pairs/s."""
+    statevector copied to host memory; rate in term*amplitude pairs/s. When the
+    GPU's per-term values of the full operator are given, the same sample's
+    energy from them is compared with the reference's result."""
     root = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, root)
     import oracle as oracle_mod
@@ -86,10 +89,22 @@ def cpu_baseline(op, psi_dev, sample_terms: int) -> dict:
     pick = np.asarray(pick, dtype=np.int64)
     psi = psi_dev.cpu().numpy()
     t0 = time.perf_counter()
-    lib.expectation(op.num_qubits, x[pick], z[pick], c[pick], psi)
+    ref_e = lib.expectation(op.num_qubits, x[pick], z[pick], c[pick], psi)
     dt = time.perf_counter() - t0
     pairs = len(pick) * psi.shape[0]
+    parity = None
+    if gpu_term_values is not None:
+        gpu_e = complex(np.sum(c[pick] * gpu_term_values[pick]))
+        scale = float(np.sum(np.abs(c[pick])))
+        parity = {
+            "reference_energy": [ref_e.real, ref_e.imag],
+            "gpu_energy_same_terms": [gpu_e.real, gpu_e.imag],
+            "abs_error": abs(gpu_e - ref_e),
+            "rel_error_vs_sum_abs_c": abs(gpu_e - ref_e) / scale if scale else None,
+            "note": "GPU per-term values of the full operator (tiled sweep) on the sampled terms",
+        }
     return {
+        "parity": parity,
         "value": pairs / dt,
         "unit": "term·amp pairs/s",
         "cores": 1,
@@ -218,7 +233,11 @@ def main() -> int:
         },
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(op, psi, args.cpu_sample_terms)
+        tv = torch.empty(T, dtype=torch.complex128, device=psi.device)
+        dop.expectation_adjoint(psi.data_ptr(), psi.numel(), tv.data_ptr(), None, None,
+                                stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        line["cpu_baseline"] = cpu_baseline(op, psi, args.cpu_sample_terms, tv.cpu().numpy())
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -135,6 +135,67 @@ def batch(args) -> dict:
     }
 
 
+def products(args) -> dict:
+    """64 batched Hamiltonian products of uneven sizes (500-8,000 terms per
+    operand, 64-256 qubits, random density-0.25 terms with complex
+    coefficients), simplified: the batched path (persistent queue-fed product
+    kernel per graph-captured chunk, then per-pair simplify) against the plain
+    per-pair matmul path on the same operands."""
+    import torch
+
+    from .batch import ProductBatch, plain_products
+    from .workloads import product_shapes, random_pauli_masks
+
+    shapes = product_shapes(64, 5001)
+    lhs, rhs = [], []
+    for i, (n, tl, tr) in enumerate(shapes):
+        lhs.append(random_pauli_masks(n, tl, 0.25, 6000 + 2 * i).to_device())
+        rhs.append(random_pauli_masks(n, tr, 0.25, 6001 + 2 * i).to_device())
+    total = sum(tl * tr for _, tl, tr in shapes)
+    t0 = time.perf_counter()
+    pb = ProductBatch(lhs, rhs, max_intermediate_terms=1 << 40, budget_bytes=16 << 30)
+    setup_s = time.perf_counter() - t0
+    # The outputs hold about as many terms as there are products (~100 GB);
+    # keep only the three smallest pairs' results for the bit-exact check.
+    keep = sorted(range(len(shapes)), key=lambda i: shapes[i][1] * shapes[i][2])[:3]
+    out = {"b": {}, "p": {}, "terms": 0}
+
+    def batched():
+        res = pb.run(simplify=True)
+        out["terms"] = sum(r.num_terms for r in res)
+        out["b"] = {i: res[i] for i in keep}
+        del res
+
+    def plain():
+        out["p"] = {}
+        for i, (a, b) in enumerate(zip(lhs, rhs)):
+            r = plain_products([a], [b], simplify=True, max_intermediate_terms=1 << 40)[0]
+            if i in keep:
+                out["p"][i] = r
+            del r
+
+    t_b = _timed(batched, args.steps, args.warmup)
+    t_p = _timed(plain, max(1, args.steps // 2), 1)
+    for i in keep:
+        a, b = out["b"][i].to_host(), out["p"][i].to_host()
+        if not (np.array_equal(a.x_words(), b.x_words()) and np.array_equal(a.z_words(), b.z_words())
+                and np.array_equal(a.coeffs().view(np.uint64), b.coeffs().view(np.uint64))):
+            raise RuntimeError(f"batched product {i} differs from its matmul")
+    return {
+        "metric": "batched Hamiltonian products + simplify: speedup vs plain per-pair matmul",
+        "value": t_p / t_b,
+        "unit": "x",
+        "batched_ms": t_b * 1000,
+        "plain_ms": t_p * 1000,
+        "products_per_s_batched": total / t_b,
+        "setup_s": setup_s,
+        "chunks": pb.chunks,
+        "config": {"workload": "products", "pairs": len(shapes), "seed": 5001,
+                   "terms": [500, 8000], "qubits": [64, 256], "total_products": total,
+                   "output_terms": out["terms"]},
+    }
+
+
 def adjoint(args) -> dict:
     """Forward + fused adjoint (sigma = H psi and per-term <P_k>) on a 24-qubit
     state with a 20,000-term Hamiltonian: the random density-0.25 generator
@@ -198,7 +259,8 @@ def adjoint(args) -> dict:
 
 
 def run(args) -> int:
-    fn = {"commutation": commutation, "matmul": matmul, "batch": batch, "adjoint": adjoint}[args.workload]
+    fn = {"commutation": commutation, "matmul": matmul, "batch": batch, "adjoint": adjoint,
+          "products": products}[args.workload]
     line = fn(args)
     line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
                  "data": "synthetic"})

@@ -397,11 +397,12 @@ struct wg_batch {
 
 namespace {
 
-float time_launch(wg_batch* b, hipStream_t stream) {
+// One timed launch of the batch (after a warm-up launch when `warm`).
+float time_launch(wg_batch* b, hipStream_t stream, bool warm) {
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "create event");
   check_hip(hipEventCreate(&e1), "create event");
-  b->launch(stream);  // warm
+  if (warm) b->launch(stream);
   check_hip(hipEventRecord(e0, stream), "record event");
   b->launch(stream);
   check_hip(hipEventRecord(e1, stream), "record event");
@@ -535,26 +536,28 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
     if (tunables) {
       b->schedule(tunables[0], static_cast<u32>(tunables[1]), static_cast<u32>(tunables[2]), stream);
     } else {
-      // Timing search over the tunables on the real batch.
-      float best = 0.0f;
+      // Timing search on the real batch, one tunable at a time (forward
+      // chunk, then sigma chunk, then grid), each from the best so far.
       int best_f = 16;
       u32 best_s = 4096, best_g = cus * 8;
-      bool have = false;
-      for (int f : {14, 16, 18}) {
-        for (u32 s : {1024u, 8192u}) {
-          for (u32 g : {cus * 4, cus * 8}) {
-            b->schedule(f, s, g, stream);
-            const float ms = time_launch(b.get(), stream);
-            if (!have || ms < best) {
-              best = ms;
-              best_f = f;
-              best_s = s;
-              best_g = g;
-              have = true;
-            }
-          }
+      b->schedule(best_f, best_s, best_g, stream);
+      float best = time_launch(b.get(), stream, true);
+      auto trial = [&](int f, u32 s, u32 g) {
+        if (f == best_f && s == best_s && g == best_g) return;
+        b->schedule(f, s, g, stream);
+        const float ms = time_launch(b.get(), stream, best < 50.0f);
+        if (ms < best) {
+          best = ms;
+          best_f = f;
+          best_s = s;
+          best_g = g;
         }
-      }
+      };
+      for (int f : {14, 18}) trial(f, best_s, best_g);
+      const int f_pick = best_f;
+      for (u32 s : {1024u, 16384u}) trial(f_pick, s, best_g);
+      const u32 s_pick = best_s;
+      for (u32 g : {cus * 4, cus * 16}) trial(f_pick, s_pick, g);
       b->schedule(best_f, best_s, best_g, stream);
     }
     capture(b.get(), stream);

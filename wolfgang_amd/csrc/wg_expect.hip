@@ -55,6 +55,17 @@ constexpr int kTileWaves = kTileBlock / 64;
 // needs more passes (launches) than this.
 constexpr std::size_t kTiledMinQubits = 18;
 constexpr std::size_t kMaxTiledPasses = 256;
+// A pass (one launch, one read of psi) is worth it while its launch cost
+// stays small against the work it replaces: allow up to one pass per
+// 2^kPassPairsLog amplitude pairs of direct-sweep work (G_x * 2^(n-1) pairs),
+// and never fewer than kMaxTiledPasses.
+constexpr int kPassPairsLog = 23;
+
+std::size_t max_tiled_passes(std::size_t n, std::size_t groups) {
+  const int shift = static_cast<int>(n) - 1 - kPassPairsLog;
+  const std::size_t by_work = shift >= 0 ? (groups << std::min(shift, 20)) : (groups >> -shift);
+  return std::max(kMaxTiledPasses, by_work);
+}
 // Diagonal keys use the Walsh-Hadamard sweep beyond this many items' worth.
 constexpr int kDiagWhtMinItems = 4;
 constexpr u32 kDiagWhtBlocks = 1024;
@@ -1078,7 +1089,7 @@ void assign_step_classes(ExpItem& e, int min_d) {
 }  // namespace
 
 void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
-                       bool diag_separate, u32* num_diag_front) {
+                       bool diag_separate, u32* num_diag_front, std::size_t max_passes) {
   const int l = kTileLowLog;
   const int kmax = std::min<int>(kTileMaxLog, static_cast<int>(n) - 3);
   const u64 lowmask = (1ULL << l) - 1;
@@ -1169,6 +1180,12 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
     spans.push_back(sp);
     pass_xs.push_back(std::move(in_pass));
     if (remaining == 0) break;
+    if (spans.size() > max_passes) {
+      // More passes than the caller accepts: report the count only.
+      passes.assign(spans.size(), PassLaunch{});
+      if (num_diag_front) *num_diag_front = 0;
+      return;
+    }
   }
 
   // Emit passes: local coordinates, segment offsets, free positions; items are
@@ -1459,8 +1476,9 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     // diagonal keys take it once they fill more than kDiagWhtMinItems items.
     const bool diag_separate = diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ;
     u32 diag_front = 0;
-    plan_tiled_passes(plan->n, tiled_items, passes, diag_separate, &diag_front);
-    if (passes.size() > kMaxTiledPasses) {
+    const std::size_t max_passes = max_tiled_passes(plan->n, gx.size());
+    plan_tiled_passes(plan->n, tiled_items, passes, diag_separate, &diag_front, max_passes);
+    if (passes.size() > max_passes) {
       plan->tiled = false;  // launch-bound: one launch per pass would dominate
     } else {
       for (const PassLaunch& p : passes) {
@@ -1768,7 +1786,8 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
     std::size_t diag_keys = 0;
     for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
     plan_tiled_passes(num_qubits, items, passes,
-                      diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ, nullptr);
+                      diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ, nullptr,
+                      SIZE_MAX);
     *num_passes = passes.size();
     for (std::size_t p = 0; p < passes.size() && p < max_passes; ++p) {
       int diag = 0;
@@ -1831,7 +1850,7 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
     const bool diag_separate =
         n >= kTileMaxLog + 3 && diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ;
     u32 diag_front = 0;
-    plan_tiled_passes(n, items, passes, diag_separate, &diag_front);
+    plan_tiled_passes(n, items, passes, diag_separate, &diag_front, SIZE_MAX);
     *num_passes = passes.size();
     *num_keys = kx.size();
     const auto* amp = reinterpret_cast<const double2*>(psi);

@@ -122,7 +122,9 @@ std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
 // the planner through wg_expectation_plan_info). With diag_separate the
 // diagonal items are moved to the front of `items` (their count in
 // *num_diag_front) and left out of the passes.
+// Stops early (leaving `items` untouched and max_passes + 1 empty entries in
+// `passes`) once the plan needs more than max_passes passes.
 void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
-                       bool diag_separate, u32* num_diag_front);
+                       bool diag_separate, u32* num_diag_front, std::size_t max_passes);
 
 }  // namespace wg
