@@ -193,6 +193,17 @@ int wg_product_batch_info(const wg_product_batch* batch, size_t* pairs, size_t* 
                           size_t* total_products);
 void wg_product_batch_destroy(wg_product_batch* batch);
 
+/* PauliSum::group_commuting(mode, "largest_first", max_terms_for_graph)
+ * (reference include/wolfgang/pauli_sum.hpp:84-87, src/grouping.cpp:197-290):
+ * greedy largest-first commuting groups of op's terms, full commutation when
+ * full != 0, else qubit-wise. Writes the group count, the group sizes (host,
+ * up to num_terms entries) and the member term indices (host, num_terms
+ * entries; groups concatenated in creation order, members in placement
+ * order). Identical to the reference's graph and streaming paths. Full mode
+ * supports up to 262,144 terms (it builds the packed commutation matrix). */
+int wg_group_commuting(const wg_pauli_sum* op, int full, void* stream, size_t* num_groups,
+                       uint32_t* sizes, uint32_t* members);
+
 /* Expectation plan summary for an operator (built and cached on first use;
  * no reference counterpart): distinct (x, z) keys, distinct x-masks counting
  * the diagonal group (G_x), sweeps over psi per evaluation (tiled passes, or

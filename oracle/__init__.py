@@ -132,6 +132,24 @@ class Oracle(_Lib):
     def has_avx512(self) -> bool:
         return bool(self.fn("has_avx512_popcnt")())
 
+    def group_commuting(self, n, x, z, mode):
+        """Greedy largest-first grouping; returns a list of member-index arrays
+        (groups in order, members in placement order)."""
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        z = np.ascontiguousarray(z, dtype=np.uint64)
+        T = x.shape[0]
+        sizes = np.zeros(max(T, 1), dtype=np.uint32)
+        members = np.zeros(max(T, 1), dtype=np.uint32)
+        g = C.c_size_t()
+        f = self.fn("group_commuting")
+        f.argtypes = [_sz, _sz, _vp, _vp, C.c_int, C.POINTER(C.c_size_t), _vp, _vp]
+        self.check(f(n, T, _arr(x), _arr(z), {"qwc": 0, "full": 1}[mode], C.byref(g), _arr(sizes), _arr(members)))
+        out, k = [], 0
+        for i in range(g.value):
+            out.append(members[k:k + sizes[i]].astype(np.int64))
+            k += int(sizes[i])
+        return out
+
     def simplify_hash_path(self, n, x, z) -> bool:
         x = np.ascontiguousarray(x, dtype=np.uint64)
         z = np.ascontiguousarray(z, dtype=np.uint64)
@@ -184,6 +202,27 @@ class Reference(_Lib):
         f.argtypes = [_sz, _sz, _vp, _vp, _sz, _vp, _vp, _sz, _vp]
         self.check(f(n, L, _arr(lx), _arr(lz), R, _arr(rx), _arr(rz), max_entries, _arr(out)))
         return out.reshape(L, R)
+
+    def group_commuting(self, n, x, z, c, mode, max_terms_for_graph=50_000):
+        """The reference's groups as a list of (x, z, c) arrays."""
+        x, z, c = _prep(n, x, z, c)
+        T = c.shape[0]
+        W = x.shape[1] if x.ndim == 2 else 0
+        sizes = np.zeros(max(T, 1), dtype=np.uint64)
+        gx = np.zeros((max(T, 1), max(W, 1)), dtype=np.uint64)
+        gz = np.zeros_like(gx)
+        gc = np.zeros(max(T, 1), dtype=np.complex128)
+        g = C.c_size_t()
+        f = self.fn("group_commuting")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, C.c_char_p, _sz, C.POINTER(C.c_size_t), _vp, _vp, _vp, _vp]
+        self.check(f(n, T, _arr(x), _arr(z), _arr(c), mode.encode(), max_terms_for_graph, C.byref(g),
+                     _arr(sizes), _arr(gx), _arr(gz), _arr(gc)))
+        out, k = [], 0
+        for i in range(g.value):
+            m = int(sizes[i])
+            out.append((gx[k:k + m, :W], gz[k:k + m, :W], gc[k:k + m]))
+            k += m
+        return out
 
 
 _oracle: Oracle | None = None

@@ -135,4 +135,32 @@ int wgref_expectation_c64(std::size_t n, std::size_t T, const std::uint64_t* x,
   });
 }
 
+// PauliSum::group_commuting (reference src/grouping.cpp:197-290): the groups
+// concatenated into (x, z, c) arrays of num_terms rows, with the group sizes.
+int wgref_group_commuting(std::size_t n, std::size_t T, const std::uint64_t* x,
+                          const std::uint64_t* z, const double* c, const char* mode,
+                          std::size_t max_terms_for_graph, std::size_t* num_groups,
+                          std::size_t* sizes, std::uint64_t* gx, std::uint64_t* gz, double* gc) {
+  return guarded([&] {
+    const auto op = build(n, T, x, z, c);
+    const auto groups = op.group_commuting(mode, "largest_first", max_terms_for_graph);
+    *num_groups = groups.size();
+    std::size_t row = 0;
+    const std::size_t W = op.words();
+    for (std::size_t g = 0; g < groups.size(); ++g) {
+      const auto& grp = groups[g];
+      sizes[g] = grp.num_terms();
+      if (W) {
+        std::memcpy(gx + row * W, grp.x_words().data(), grp.num_terms() * W * sizeof(std::uint64_t));
+        std::memcpy(gz + row * W, grp.z_words().data(), grp.num_terms() * W * sizeof(std::uint64_t));
+      }
+      for (std::size_t t = 0; t < grp.num_terms(); ++t) {
+        gc[2 * (row + t)] = grp.coeffs()[t].real();
+        gc[2 * (row + t) + 1] = grp.coeffs()[t].imag();
+      }
+      row += grp.num_terms();
+    }
+  });
+}
+
 }  // extern "C"

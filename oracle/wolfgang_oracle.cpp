@@ -722,4 +722,91 @@ int orc_apply(std::size_t n, std::size_t T, const u64* x, const u64* z, const do
   });
 }
 
+// PauliSum::group_commuting (reference src/grouping.cpp:36-62 ordering,
+// :64-85 qubit-wise acceptance, :197-290 greedy placement): terms are visited
+// by descending weight |x | z|, then canonical key order, then index; each
+// goes to the first existing group that accepts it, else opens a new group.
+// mode 0 ("qwc"): accepted when (x_t & Zb) ^ (z_t & Xb) = 0 on every word,
+// with Xb/Zb the OR of the group's masks; mode 1 ("full"): accepted when the
+// term commutes with every member. Output: group sizes (in group order) and
+// the member indices, groups concatenated, members in placement order.
+int orc_group_commuting(std::size_t n, std::size_t T, const u64* x, const u64* z, int mode,
+                        std::size_t* num_groups, unsigned* sizes, unsigned* members) {
+  return guarded([&] {
+    const std::size_t W = n == 0 ? 0 : (n + 63) / 64;
+    std::vector<std::size_t> order(T);
+    std::vector<std::size_t> weight(T, 0);
+    for (std::size_t t = 0; t < T; ++t) {
+      order[t] = t;
+      for (std::size_t w = 0; w < W; ++w) weight[t] += std::popcount(x[t * W + w] | z[t * W + w]);
+    }
+    auto key_less = [&](std::size_t a, std::size_t b) {
+      for (std::size_t w = 0; w < W; ++w) {
+        if (x[a * W + w] != x[b * W + w]) return x[a * W + w] < x[b * W + w] ? -1 : 1;
+        if (z[a * W + w] != z[b * W + w]) return z[a * W + w] < z[b * W + w] ? -1 : 1;
+      }
+      return 0;
+    };
+    std::sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
+      if (weight[a] != weight[b]) return weight[a] > weight[b];
+      const int k = key_less(a, b);
+      if (k != 0) return k < 0;
+      return a < b;
+    });
+    auto commute = [&](std::size_t a, std::size_t b) {
+      unsigned parity = 0;
+      for (std::size_t w = 0; w < W; ++w) {
+        parity ^= static_cast<unsigned>(
+                      std::popcount((x[a * W + w] & z[b * W + w]) ^ (z[a * W + w] & x[b * W + w]))) & 1U;
+      }
+      return parity == 0;
+    };
+    std::vector<std::vector<std::size_t>> groups;
+    std::vector<std::vector<u64>> xb, zb;
+    for (std::size_t t : order) {
+      bool placed = false;
+      for (std::size_t g = 0; g < groups.size() && !placed; ++g) {
+        bool ok = true;
+        if (mode == 0) {
+          for (std::size_t w = 0; w < W && ok; ++w) {
+            ok = ((x[t * W + w] & zb[g][w]) ^ (z[t * W + w] & xb[g][w])) == 0;
+          }
+        } else {
+          for (std::size_t m : groups[g]) {
+            if (!commute(t, m)) {
+              ok = false;
+              break;
+            }
+          }
+        }
+        if (ok) {
+          groups[g].push_back(t);
+          if (mode == 0) {
+            for (std::size_t w = 0; w < W; ++w) {
+              xb[g][w] |= x[t * W + w];
+              zb[g][w] |= z[t * W + w];
+            }
+          }
+          placed = true;
+        }
+      }
+      if (!placed) {
+        groups.push_back({t});
+        xb.emplace_back(W, 0);
+        zb.emplace_back(W, 0);
+        for (std::size_t w = 0; w < W; ++w) {
+          xb.back()[w] = x[t * W + w];
+          zb.back()[w] = z[t * W + w];
+        }
+      }
+    }
+    *num_groups = groups.size();
+    std::size_t k = 0;
+    for (std::size_t g = 0; g < groups.size(); ++g) {
+      sizes[g] = static_cast<unsigned>(groups[g].size());
+      for (std::size_t m : groups[g]) members[k++] = static_cast<unsigned>(m);
+    }
+  });
+}
+
 }  // extern "C"

@@ -112,3 +112,16 @@ def test_product_package_does_not_import_the_oracle():
         assert "import oracle" not in text and "from oracle" not in text, path
     for path in (ROOT / "wolfgang_amd" / "csrc").glob("*"):
         assert "oracle" not in path.read_text().lower(), path
+
+
+def test_group_commuting_argument_messages():
+    # Reference tests/test_phase6_commutation_grouping.py:187-194 and
+    # bindings/python/pauli_sum_py.cpp:105-118 (validated before any device work).
+    op = PauliSum.from_labels(["X"], [1.0])
+    with pytest.raises(ValueError, match="mode"):
+        op.group_commuting(mode="diagonal")
+    with pytest.raises(ValueError, match="strategy"):
+        op.group_commuting(strategy="smallest_first")
+    with pytest.raises(ValueError, match="max_terms_for_graph must be a non-negative integer"):
+        op.group_commuting(max_terms_for_graph=-1)
+    assert PauliSum.empty(2).group_commuting(mode="full") == []

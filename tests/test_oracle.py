@@ -244,3 +244,45 @@ def test_chemistry_generator_is_hermitian_and_shaped():
             assert set(stripped[1:-1]) <= {"Z"}
         else:
             assert stripped.count("Z") in (1, 2) and set(stripped) <= {"Z", "I"}
+
+
+# ------------------------------------------------------------ grouping
+
+def test_grouping_reference_fixtures(orc):
+    # Reference tests/test_phase6_commutation_grouping.py:173-184.
+    op = PauliSum.from_labels(["II", "XI", "XX", "IZ"], [1.0, 2.0, 3.0, 4.0])
+    groups = orc.group_commuting(op.num_qubits, op.x_words(), op.z_words(), "qwc")
+    assert [g.tolist() for g in groups] == [[2, 1, 0], [3]]  # [["XX","XI","II"],["IZ"]]
+    # Validity on the reference's 7-term example (:132-158): every group is
+    # internally (qubit-wise) commuting and the groups partition the terms.
+    op = PauliSum.from_labels(["XI", "IX", "XX", "YY", "ZI", "IZ", "II"], [1.0, -2.0, 3.0, 4.0, 5.0, -6.0, 7.0])
+    x, z = op.x_words()[:, 0], op.z_words()[:, 0]
+    for mode in ("qwc", "full"):
+        groups = orc.group_commuting(op.num_qubits, op.x_words(), op.z_words(), mode)
+        assert sorted(i for g in groups for i in g) == list(range(7))
+        for g in groups:
+            for a in g:
+                for b in g:
+                    if mode == "full":
+                        assert bin(int((x[a] & z[b]) ^ (z[a] & x[b]))).count("1") % 2 == 0
+                    else:
+                        both = (x[a] | z[a]) & (x[b] | z[b])
+                        assert ((x[a] ^ x[b]) & both) == 0 and ((z[a] ^ z[b]) & both) == 0
+
+
+@pytest.mark.parametrize("mode", ["qwc", "full"])
+@pytest.mark.parametrize("n,terms", [(1, 5), (6, 40), (64, 300), (70, 500), (130, 200)])
+def test_grouping_equals_compiled_reference(orc, ref_lib, mode, n, terms):
+    pool = random_pauli_masks(n, max(1, terms // 2), 0.3, n * 13 + terms)
+    rng = np.random.default_rng(n + terms)
+    pick = rng.integers(0, pool.num_terms, size=terms)  # duplicates exercise the tie-breaks
+    x, z = pool.x_words()[pick], pool.z_words()[pick]
+    c = rng.normal(size=terms) + 1j * rng.normal(size=terms)
+    groups = orc.group_commuting(n, x, z, mode)
+    for max_graph in (50_000, 0):  # the reference's graph and streaming paths agree
+        want = ref_lib.group_commuting(n, x, z, c, mode, max_graph)
+        assert len(want) == len(groups)
+        for g, (wx, wz, wc) in zip(groups, want):
+            np.testing.assert_array_equal(x[g], wx)
+            np.testing.assert_array_equal(z[g], wz)
+            np.testing.assert_array_equal(c[g].view(np.uint64), wc.view(np.uint64))

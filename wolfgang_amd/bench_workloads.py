@@ -196,6 +196,34 @@ def products(args) -> dict:
     }
 
 
+def grouping(args) -> dict:
+    """Greedy largest-first commuting groups (full commutation and qubit-wise)
+    of a 16,384-term, 128-qubit random operator (density 0.25) on the device."""
+    import torch
+
+    from .workloads import random_pauli_masks
+
+    n, T = 128, 16_384
+    op = random_pauli_masks(n, T, 0.25, 2004)
+    d = op.to_device()
+    res = {}
+    for mode in ("full", "qwc"):
+        out = {}
+
+        def step():
+            out["g"] = d.group_commuting(mode)
+
+        res[mode] = {"ms": _timed(step, max(1, args.steps // 2), 1) * 1000, "groups": len(out["g"])}
+    return {
+        "metric": "greedy commuting groups, 16,384 terms x 128 qubits (full mode time)",
+        "value": res["full"]["ms"],
+        "unit": "ms",
+        "higher_is_better": False,
+        "config": {"workload": "grouping", "num_qubits": n, "num_terms": T, "seed": 2004},
+        "results": res,
+    }
+
+
 def adjoint(args) -> dict:
     """Forward + fused adjoint (sigma = H psi and per-term <P_k>) on a 24-qubit
     state with a 20,000-term Hamiltonian: the random density-0.25 generator
@@ -260,7 +288,7 @@ def adjoint(args) -> dict:
 
 def run(args) -> int:
     fn = {"commutation": commutation, "matmul": matmul, "batch": batch, "adjoint": adjoint,
-          "products": products}[args.workload]
+          "products": products, "grouping": grouping}[args.workload]
     line = fn(args)
     line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
                  "data": "synthetic"})

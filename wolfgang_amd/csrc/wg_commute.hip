@@ -27,22 +27,34 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kWordsPerWave = 4;
 constexpr int kWordsPerBlock = kWaves * kWordsPerWave;
 
+// (a & b) ^ c in one VALU op: v_bitop3_b32 with the truth table of
+// (S0 & S1) ^ S2 (S0 = 0xF0, S1 = 0xCC, S2 = 0xAA).
+__device__ inline u32 and_xor(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x6A); }
+
+// XOR of every 32-bit half of (lx & rz) ^ (lz & rx) over the words: its
+// popcount parity is the parity of the whole conflict popcount.
 template <int WC>
 __device__ inline u32 conflict_fold(const u64* __restrict__ rx, const u64* __restrict__ rz,
                                     std::size_t col, std::size_t W, const u64* lxr,
                                     const u64* lzr) {
-  u64 acc = 0;
   if constexpr (WC > 0) {
+    u32 acc = 0;
 #pragma unroll
     for (int w = 0; w < WC; ++w) {
-      acc ^= (lxr[w] & rz[col * WC + w]) ^ (lzr[w] & rx[col * WC + w]);
+      const u64 rzw = rz[col * WC + w], rxw = rx[col * WC + w];
+      acc = and_xor(static_cast<u32>(lxr[w]), static_cast<u32>(rzw), acc);
+      acc = and_xor(static_cast<u32>(lxr[w] >> 32), static_cast<u32>(rzw >> 32), acc);
+      acc = and_xor(static_cast<u32>(lzr[w]), static_cast<u32>(rxw), acc);
+      acc = and_xor(static_cast<u32>(lzr[w] >> 32), static_cast<u32>(rxw >> 32), acc);
     }
+    return acc;
   } else {
+    u64 acc = 0;
     for (std::size_t w = 0; w < W; ++w) {
       acc ^= (lxr[w] & rz[col * W + w]) ^ (lzr[w] & rx[col * W + w]);
     }
+    return static_cast<u32>(acc) ^ static_cast<u32>(acc >> 32);
   }
-  return static_cast<u32>(acc) ^ static_cast<u32>(acc >> 32);
 }
 
 // WC = compile-time word count (1, 2, 4) or 0 for a runtime count. For WC == 0
@@ -78,18 +90,19 @@ __global__ __launch_bounds__(kBlock) void commute_packed_kernel(
       const std::size_t ncols = (R - c0) < 64 ? (R - c0) : 64;
       if constexpr (WC > 0) {
         if (ncols == 64) {
+          // Odd-parity (anticommuting) bits first, inverted once per word.
           u32 lo = 0, hi = 0;
 #pragma unroll
           for (int b = 0; b < 32; ++b) {
             const u32 f = conflict_fold<WC>(rx, rz, c0 + b, W, lxr, lzr);
-            lo |= ((__popc(f) & 1U) ^ 1U) << b;
+            lo |= (__popc(f) & 1U) << b;
           }
 #pragma unroll
           for (int b = 0; b < 32; ++b) {
             const u32 f = conflict_fold<WC>(rx, rz, c0 + 32 + b, W, lxr, lzr);
-            hi |= ((__popc(f) & 1U) ^ 1U) << b;
+            hi |= (__popc(f) & 1U) << b;
           }
-          bits = (static_cast<u64>(hi) << 32) | lo;
+          bits = ~((static_cast<u64>(hi) << 32) | lo);
         } else {
           for (std::size_t b = 0; b < ncols; ++b) {
             const u32 f = conflict_fold<WC>(rx, rz, c0 + b, W, lxr, lzr);

@@ -318,6 +318,22 @@ class PauliSum:
             raise ValueError("simplify tolerances must be non-negative finite values")
         return self.to_device().simplify(atol, rtol).to_host()
 
+    def group_commuting(
+        self, mode: str = "qwc", strategy: str = "largest_first", max_terms_for_graph: int = 50_000
+    ) -> list["PauliSum"]:
+        """Greedy largest-first commuting groups (reference src/grouping.cpp:197-290),
+        computed on the device. ``max_terms_for_graph`` only selects between the
+        reference's graph and streaming CPU strategies, which give the same
+        groups; it is validated and otherwise unused here."""
+        if strategy != "largest_first":
+            raise ValueError("group_commuting strategy must be 'largest_first'")
+        if mode not in ("qwc", "full"):
+            raise ValueError("group_commuting mode must be 'qwc' or 'full'")
+        _checked_size(max_terms_for_graph, "max_terms_for_graph")
+        if self.num_terms == 0:
+            return []
+        return self.to_device().group_commuting(mode)
+
     def matmul(
         self,
         rhs: "PauliSum",
@@ -507,6 +523,28 @@ class DevicePauliSum:
         if not isinstance(rhs, DevicePauliSum):
             return NotImplemented
         return self.matmul(rhs)
+
+    def group_commuting(self, mode: str = "qwc", stream: Any = None) -> list["PauliSum"]:
+        """Greedy largest-first commuting groups of this operator's terms as
+        host PauliSums (see PauliSum.group_commuting)."""
+        if mode not in ("qwc", "full"):
+            raise ValueError("group_commuting mode must be 'qwc' or 'full'")
+        T = self._t
+        if T == 0:
+            return []
+        sizes = np.zeros(T, dtype=np.uint32)
+        members = np.zeros(T, dtype=np.uint32)
+        count = C.c_size_t()
+        _native.call("wg_group_commuting", self._handle(), int(mode == "full"), _stream_arg(stream),
+                     C.byref(count), sizes.ctypes.data, members.ctypes.data)
+        host = self.to_host()
+        x, z, c = host.x_words(), host.z_words(), host.coeffs()
+        out, k = [], 0
+        for g in range(count.value):
+            idx = members[k:k + sizes[g]].astype(np.int64)
+            out.append(PauliSum._from_arrays(self._n, x[idx], z[idx], c[idx]))
+            k += int(sizes[g])
+        return out
 
     def commutes_with_into(
         self,
