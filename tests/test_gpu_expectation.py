@@ -115,6 +115,50 @@ def test_adjoint_sigma_and_term_values(gpu, orc, n):
     assert abs(np.vdot(psi, sigma.cpu().numpy()) - e) <= RTOL * s
 
 
+@pytest.mark.parametrize("gen,n,terms", [("random", 18, 300), ("chemistry", 19, 2000), ("diagonal", 18, 120)])
+def test_tiled_adjoint_sigma_matches_oracle(gpu, orc, gen, n, terms):
+    # States with n >= 18 take the tiled path: forward passes for E and the
+    # per-term values, then sigma over the same passes (the chemistry case has
+    # more than 32 diagonal keys, swept separately in the forward and carried
+    # by the first pass in the sigma sweep).
+    import torch
+
+    from wolfgang_amd import PauliSum
+
+    if gen == "chemistry":
+        op = chemistry_masks(n, terms, 70 + n)
+    elif gen == "random":
+        op = random_pauli_masks(n, terms, 0.3, 71 + n)
+    else:
+        base = chemistry_masks(n, 4 * terms, 72 + n)
+        keep = base.x_words()[:, 0] == 0
+        op = PauliSum.from_packed(n, base.x_words()[keep][:terms], base.z_words()[keep][:terms],
+                                  base.coeffs()[keep][:terms])
+    psi = random_statevector(n, 73 + n)
+    dop = op.to_device()
+    assert dop.expectation_plan_info()["tiled"]
+    t = torch.from_numpy(psi).to("cuda:0")
+    rng = np.random.default_rng(n)
+    coeffs = op.coeffs() * (1 + 0.5j * rng.normal(size=op.num_terms))  # complex override
+    ct = torch.from_numpy(coeffs).to("cuda:0")
+    sigma, sigma_dag = torch.empty_like(t), torch.empty_like(t)
+    tv = torch.empty(op.num_terms, dtype=torch.complex128, device="cuda:0")
+    for override in (None, ct):
+        c = op.coeffs() if override is None else coeffs
+        e = dop.expectation_adjoint(t.data_ptr(), t.numel(), tv.data_ptr(), sigma.data_ptr(),
+                                    sigma_dag.data_ptr(),
+                                    coeffs_ptr=None if override is None else override.data_ptr())
+        want_sigma = orc.apply(n, op.x_words(), op.z_words(), c, psi, threads=8)
+        want_dag = orc.apply(n, op.x_words(), op.z_words(), c.conj(), psi, threads=8)
+        s = float(np.abs(c).sum()) * float(np.vdot(psi, psi).real)
+        np.testing.assert_allclose(sigma.cpu().numpy(), want_sigma, rtol=0, atol=RTOL * s)
+        np.testing.assert_allclose(sigma_dag.cpu().numpy(), want_dag, rtol=0, atol=RTOL * s)
+        want_tv = orc.term_values(n, op.x_words(), op.z_words(), psi, threads=8)
+        np.testing.assert_allclose(tv.cpu().numpy().real, want_tv, rtol=0, atol=RTOL)
+        assert abs(e - np.sum(c * want_tv)) <= RTOL * s
+        assert abs(np.vdot(psi, sigma.cpu().numpy()) - e) <= RTOL * s
+
+
 @pytest.mark.parametrize("gen,n,terms", [("chemistry", 18, 2000), ("chemistry", 20, 3000),
                                          ("random", 18, 400), ("diagonal", 19, 300)])
 def test_tiled_sweep_matches_oracle(gpu, orc, gen, n, terms):

@@ -68,6 +68,8 @@ std::size_t max_tiled_passes(std::size_t n, std::size_t groups) {
 }
 // Diagonal keys use the Walsh-Hadamard sweep beyond this many items' worth.
 constexpr int kDiagWhtMinItems = 4;
+// Grid of the tiled sigma launches.
+constexpr u64 kSigmaBlocks = 2048;
 constexpr u32 kDiagWhtBlocks = 1024;
 
 __device__ inline u64 insert_zero(u64 v, u64 pivot) {
@@ -614,6 +616,164 @@ constexpr std::array<TiledKernel, sizeof...(V)> tiled_kernel_table(std::integer_
 
 const std::array<TiledKernel, kVariants> kTiledKernels =
     tiled_kernel_table(std::make_integer_sequence<int, kVariants>{});
+
+// ------------------------------------------------------------ tiled sigma
+
+// sigma = H psi (and H^dagger psi) over the tiled passes: for output o and
+// x-group g, (P_k psi)[o] = i^{y_k} (-1)^{|z_k & (o ^ x)|} psi[o ^ x], so
+// sigma[o] = sum_g psi[o ^ x_g] * sum_{k in g} cy_k (-1)^{|z_k & (o ^ x_g)|} with
+// cy_k = c_k i^{y_k} (H^dagger: conj(c_k) i^{y_k}). A block stages a tile and,
+// for every item of the pass, each thread adds to its 8 staged points'
+// accumulators one partner read from LDS times the item's signed coefficient
+// sum; the accumulators are then added into sigma in HBM (one read-modify-write
+// per point per pass, sigma cleared before the first pass). For the thread's
+// point u = tid + 512 i the sign of key j splits into a lane part
+// parity(zl_j & tid), the uniform parts parity(zl_j & 512 i) and
+// parity(zl_j & xl), and the tile part parity(z_j & rep).
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+
+template <bool kDagger>
+__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void sigma_tiled_kernel(const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
+                        const ExpItem* __restrict__ items, u32 num_items,
+                        const double2* __restrict__ cy, const double2* __restrict__ cy_dag,
+                        double2* __restrict__ sigma, double2* __restrict__ sigma_dag) {
+  __shared__ double2 tile[kTilePoints];
+  const u32 tid = threadIdx.x;
+  const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const u64 free_mask = pd->free_mask;
+  const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[8], sb_e1 = pd->seg_off[16],
+            sb_e2 = pd->seg_off[32];
+  const u32 lo = tid & ((1U << kTileLowLog) - 1U);
+  // Point i of this thread in tile rep: (rep ^ seg_off[wave + 8 i]) | lo, the
+  // segment offset being linear in the segment index. Named registers (not
+  // arrays) keep the eight points out of scratch memory.
+#define WG_G(rep, I)                                                                  \
+  ((((rep) ^ sb_w ^ (((I) & 1) ? sb_e0 : 0) ^ (((I) & 2) ? sb_e1 : 0) ^ (((I) & 4) ? sb_e2 : 0))) | lo)
+  double2 st0, st1, st2, st3, st4, st5, st6, st7;
+  u64 t = blockIdx.x;
+  if (t < ntiles) {
+    const u64 rep0 = tile_rep(t, free_mask);
+    st0 = psi[WG_G(rep0, 0)];
+    st1 = psi[WG_G(rep0, 1)];
+    st2 = psi[WG_G(rep0, 2)];
+    st3 = psi[WG_G(rep0, 3)];
+    st4 = psi[WG_G(rep0, 4)];
+    st5 = psi[WG_G(rep0, 5)];
+    st6 = psi[WG_G(rep0, 6)];
+    st7 = psi[WG_G(rep0, 7)];
+  }
+  for (; t < ntiles; t += gridDim.x) {
+    __syncthreads();
+    tile[tid + 0 * kTileBlock] = st0;
+    tile[tid + 1 * kTileBlock] = st1;
+    tile[tid + 2 * kTileBlock] = st2;
+    tile[tid + 3 * kTileBlock] = st3;
+    tile[tid + 4 * kTileBlock] = st4;
+    tile[tid + 5 * kTileBlock] = st5;
+    tile[tid + 6 * kTileBlock] = st6;
+    tile[tid + 7 * kTileBlock] = st7;
+    __syncthreads();
+    const u64 rep = tile_rep(t, free_mask);
+    const u64 tn = t + gridDim.x;
+    if (tn < ntiles) {
+      const u64 repn = tile_rep(tn, free_mask);
+      st0 = psi[WG_G(repn, 0)];
+      st1 = psi[WG_G(repn, 1)];
+      st2 = psi[WG_G(repn, 2)];
+      st3 = psi[WG_G(repn, 3)];
+      st4 = psi[WG_G(repn, 4)];
+      st5 = psi[WG_G(repn, 5)];
+      st6 = psi[WG_G(repn, 6)];
+      st7 = psi[WG_G(repn, 7)];
+    }
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0), a2 = make_double2(0.0, 0.0), a3 = make_double2(0.0, 0.0), a4 = make_double2(0.0, 0.0), a5 = make_double2(0.0, 0.0), a6 = make_double2(0.0, 0.0), a7 = make_double2(0.0, 0.0);
+    double2 d0 = make_double2(0.0, 0.0), d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0), d3 = make_double2(0.0, 0.0), d4 = make_double2(0.0, 0.0), d5 = make_double2(0.0, 0.0), d6 = make_double2(0.0, 0.0), d7 = make_double2(0.0, 0.0);
+    for (u32 m = 0; m < num_items; ++m) {
+      const ExpItem& it = items[m];
+      const u32 xl = static_cast<u32>(it.xl);
+      double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0), c2 = make_double2(0.0, 0.0), c3 = make_double2(0.0, 0.0), c4 = make_double2(0.0, 0.0), c5 = make_double2(0.0, 0.0), c6 = make_double2(0.0, 0.0), c7 = make_double2(0.0, 0.0);
+      double2 e0 = make_double2(0.0, 0.0), e1 = make_double2(0.0, 0.0), e2 = make_double2(0.0, 0.0), e3 = make_double2(0.0, 0.0), e4 = make_double2(0.0, 0.0), e5 = make_double2(0.0, 0.0), e6 = make_double2(0.0, 0.0), e7 = make_double2(0.0, 0.0);
+      for (int j = 0; j < it.nz; ++j) {
+        const u32 zlj = static_cast<u32>(it.zl[j]);
+        const u32 base = static_cast<u32>((__popc(zlj & xl) + __popcll(it.z[j] & rep)) & 1);
+        const u32 lanep = static_cast<u32>(__popc(zlj & tid) & 1) ^ base;
+        const double2 k = cy[it.key[j]];
+        const double2 kd = kDagger ? cy_dag[it.key[j]] : make_double2(0.0, 0.0);
+#define WG_ADD(I)                                                                       \
+  {                                                                                     \
+    const u32 neg = lanep ^ static_cast<u32>(__popc(zlj & static_cast<u32>((I) * kTileBlock)) & 1); \
+    c##I.x += neg ? -k.x : k.x;                                                          \
+    c##I.y += neg ? -k.y : k.y;                                                          \
+    if (kDagger) {                                                                      \
+      e##I.x += neg ? -kd.x : kd.x;                                                      \
+      e##I.y += neg ? -kd.y : kd.y;                                                      \
+    }                                                                                   \
+  }
+        WG_ADD(0) WG_ADD(1) WG_ADD(2) WG_ADD(3) WG_ADD(4) WG_ADD(5) WG_ADD(6) WG_ADD(7)
+#undef WG_ADD
+      }
+#define WG_MAC(I)                                                                       \
+  {                                                                                     \
+    const double2 b = tile[(tid + (I) * kTileBlock) ^ xl];                              \
+    const double2 pr = cmul(c##I, b);                                                   \
+    a##I.x += pr.x;                                                                     \
+    a##I.y += pr.y;                                                                     \
+    if (kDagger) {                                                                      \
+      const double2 pq = cmul(e##I, b);                                                 \
+      d##I.x += pq.x;                                                                   \
+      d##I.y += pq.y;                                                                   \
+    }                                                                                   \
+  }
+      WG_MAC(0) WG_MAC(1) WG_MAC(2) WG_MAC(3) WG_MAC(4) WG_MAC(5) WG_MAC(6) WG_MAC(7)
+#undef WG_MAC
+    }
+#define WG_OUT(I)                                                                       \
+  {                                                                                     \
+    const u64 g = WG_G(rep, I);                                                         \
+    double2 v = sigma[g];                                                               \
+    v.x += a##I.x;                                                                      \
+    v.y += a##I.y;                                                                      \
+    sigma[g] = v;                                                                       \
+    if (kDagger) {                                                                      \
+      double2 w = sigma_dag[g];                                                         \
+      w.x += d##I.x;                                                                    \
+      w.y += d##I.y;                                                                    \
+      sigma_dag[g] = w;                                                                 \
+    }                                                                                   \
+  }
+    WG_OUT(0) WG_OUT(1) WG_OUT(2) WG_OUT(3) WG_OUT(4) WG_OUT(5) WG_OUT(6) WG_OUT(7)
+#undef WG_OUT
+  }
+#undef WG_G
+}
+
+// cy[id] = c_key * i^{y_key}, cy_dag[id] = conj(c_key) * i^{y_key}, by distinct-key id.
+__global__ __launch_bounds__(256) void key_cy_kernel(const double2* __restrict__ key_c_flat,
+                                                     const int* __restrict__ key_y_flat,
+                                                     const u32* __restrict__ flat_key,
+                                                     std::size_t num_keys,
+                                                     double2* __restrict__ cy,
+                                                     double2* __restrict__ cy_dag) {
+  const std::size_t p = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= num_keys) return;
+  const double2 c = key_c_flat[p];
+  const double2 cc = make_double2(c.x, -c.y);
+  const int y = key_y_flat[p] & 3;
+  auto times_iy = [y](double2 v) {
+    switch (y) {
+      case 0: return v;
+      case 1: return make_double2(-v.y, v.x);
+      case 2: return make_double2(-v.x, -v.y);
+      default: return make_double2(v.y, -v.x);
+    }
+  };
+  cy[flat_key[p]] = times_iy(c);
+  cy_dag[flat_key[p]] = times_iy(cc);
+}
 
 // ------------------------------------------------------------ diagonal keys
 
@@ -1258,6 +1418,18 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
       return out;
     };
     pl.first_item = static_cast<u32>(ordered.size());
+    if (p == 0 && diag_separate) {
+      // The separately swept diagonal items lead the list; their local masks
+      // in pass 0's coordinates serve the tiled sigma sweep.
+      for (u32 i = 0; i < (num_diag_front ? *num_diag_front : 0); ++i) {
+        ExpItem& e = ordered[i];
+        e.xl = 0;
+        e.pbit = 0;
+        for (int j = 0; j < e.nz; ++j) e.zl[j] = z_local(e.z[j]);
+        e.cls_log = kNoClasses;
+        e.variant = static_cast<unsigned char>(8 + e.nz - 1);
+      }
+    }
     for (int diag_round = 1; diag_round >= 0; --diag_round) {
       for (const ExpItem& it : items) {
         const bool is_diag = it.x == 0;
@@ -1700,7 +1872,50 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
     const double2* dpsi = reinterpret_cast<const double2*>(psi);
     double2* dout = scratch.take<double2>(1);
     const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
-    if (sigma || sigma_dagger) {
+    if ((sigma || sigma_dagger) && plan->tiled) {
+      // Forward sweeps for E and the per-key values, then sigma over the
+      // same passes (one read of psi and one sigma update per pass).
+      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+      const double2* key_c = plan->d_key_c;
+      if (coeffs) {
+        double2* kc = scratch.take<double2>(plan->num_keys);
+        key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+            c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
+        check_launch("key_coeff");
+        key_c = kc;
+      }
+      double2* cy = scratch.take<double2>(plan->num_keys);
+      double2* cy_dag = scratch.take<double2>(plan->num_keys);
+      key_cy_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+          key_c, plan->d_key_y, plan->d_flat_key, plan->num_keys, cy, cy_dag);
+      check_launch("key_cy");
+      if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
+      if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
+      const u64 ntiles = u64{1} << (plan->n - kTileMaxLog);
+      const unsigned grid = static_cast<unsigned>(std::min<u64>(ntiles, kSigmaBlocks));
+      for (std::size_t p = 0; p < plan->passes.size(); ++p) {
+        const PassLaunch& pl = plan->passes[p];
+        // Pass 0 also carries the diagonal items swept separately in the forward.
+        const u32 first = p == 0 ? 0 : pl.first_item;
+        const u32 count = pl.first_item + pl.num_items - first;
+        if (count == 0) continue;
+        ProfileScope prof("sigma_tiled", stream);
+        // sigma and sigma^dagger (the same sweep with conj(c)) are separate
+        // launches: both accumulator sets would not fit the register budget.
+        if (sigma) {
+          sigma_tiled_kernel<false><<<grid, kTileBlock, 0, stream>>>(
+              dpsi, plan->d_passes + p, plan->d_items + first, count, cy, nullptr,
+              reinterpret_cast<double2*>(sigma), nullptr);
+          check_launch("sigma_tiled");
+        }
+        if (sigma_dagger) {
+          sigma_tiled_kernel<false><<<grid, kTileBlock, 0, stream>>>(
+              dpsi, plan->d_passes + p, plan->d_items + first, count, cy_dag, nullptr,
+              reinterpret_cast<double2*>(sigma_dagger), nullptr);
+          check_launch("sigma_tiled");
+        }
+      }
+    } else if (sigma || sigma_dagger) {
       // One fused sweep: sigma, sigma^dagger and the per-key values.
       const double2* key_c = plan->d_key_c;
       if (coeffs) {
