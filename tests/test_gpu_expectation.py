@@ -64,6 +64,20 @@ def test_expectation_complex64_and_device_tensor(gpu, orc):
     t64 = torch.from_numpy(psi.astype(np.complex64)).to("cuda:0")
     assert dop.expectation_statevector(t64) == got64
 
+    class DLPackOnly:
+        # A producer that exposes only the DLPack protocol (kDLROCM device memory).
+        def __init__(self, tensor):
+            self._t = tensor
+
+        def __dlpack__(self, stream=None):
+            return self._t.__dlpack__()
+
+        def __dlpack_device__(self):
+            return self._t.__dlpack_device__()
+
+    assert dop.expectation_statevector(DLPackOnly(t)) == dop.expectation_statevector(t)
+    assert int(t.__dlpack_device__()[0]) in (2, 10)
+
 
 @pytest.mark.parametrize("case", KATS["expectation"], ids=lambda c: c["name"])
 def test_expectation_reference_results(gpu, case):

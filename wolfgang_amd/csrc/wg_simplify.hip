@@ -146,64 +146,41 @@ __device__ inline u64 match_digit(u32 d) {
   return peers;
 }
 
-// Stable scatter of one 4096-record tile per block. Wave w owns records
-// [tile + 1024 w, tile + 1024 (w + 1)), visited in 16 rounds of 64 consecutive
-// records: a record's rank is the wave's running count of its digit (kept in
-// the wave's own LDS row, so no block barrier per round) plus its rank among
-// the round's equal digits (ballot match). One barrier then turns the per-wave
-// counts into global positions: block offset of the digit plus the counts of
-// the earlier waves.
-constexpr int kRounds = static_cast<int>(kTile / kBlock);
-
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const u64* __restrict__ keys_in, const u32* __restrict__ idx_in, u64* __restrict__ keys_out,
     u32* __restrict__ idx_out, std::size_t T, int KW, int word, int shift, std::size_t nblocks,
     const u32* __restrict__ offsets) {
-  __shared__ u32 wave_cnt[kBlock / 64][257];
+  __shared__ u32 base[256];
+  __shared__ u32 counts[kBlock / 64][256];
   const unsigned lane = threadIdx.x & 63;
   const unsigned wave = threadIdx.x >> 6;
-  for (unsigned k = lane; k < 257; k += 64) wave_cnt[wave][k] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  base[threadIdx.x] = offsets[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x];
+  for (int w = 0; w < kBlock / 64; ++w) counts[w][threadIdx.x] = 0;
+  __syncthreads();
   const u64 lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-  const std::size_t first = static_cast<std::size_t>(blockIdx.x) * kTile +
-                            static_cast<std::size_t>(wave) * (kRounds * 64);
-  u32 rank[kRounds], dig[kRounds];
-#pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    const std::size_t i = first + static_cast<std::size_t>(r) * 64 + lane;
+  const std::size_t tile = static_cast<std::size_t>(blockIdx.x) * kTile;
+  for (int r = 0; r < kItems; ++r) {
+    const std::size_t i = tile + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
     const bool valid = i < T;
     const u32 d = valid ? (static_cast<u32>(keys_in[i * KW + word] >> shift) & 0xFFU) : 256U;
     const u64 peers = match_digit(d);
-    const u32 within = __popcll(peers & lt_mask);
-    const u32 run = wave_cnt[wave][d];
-    __builtin_amdgcn_wave_barrier();
-    if (within == 0) wave_cnt[wave][d] = run + __popcll(peers);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    rank[r] = run + within;
-    dig[r] = d;
-  }
-  __syncthreads();
-  // Digit threadIdx.x: global start of each wave's run of that digit.
-  {
-    u32 pos = offsets[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x];
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-      const u32 c = wave_cnt[w][threadIdx.x];
-      wave_cnt[w][threadIdx.x] = pos;
-      pos += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kRounds; ++r) {
-    const std::size_t i = first + static_cast<std::size_t>(r) * 64 + lane;
-    if (i < T) {
-      const std::size_t pos = wave_cnt[wave][dig[r]] + rank[r];
-      for (int k = 0; k < KW; ++k) keys_out[pos * KW + k] = keys_in[i * KW + k];
+    const u32 rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) counts[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      u32 pos = base[d] + rank;
+      for (unsigned w = 0; w < wave; ++w) pos += counts[w][d];
+      for (int k = 0; k < KW; ++k) keys_out[static_cast<std::size_t>(pos) * KW + k] = keys_in[i * KW + k];
       idx_out[pos] = idx_in[i];
     }
+    __syncthreads();
+    u32 add = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      add += counts[w][threadIdx.x];
+      counts[w][threadIdx.x] = 0;
+    }
+    base[threadIdx.x] += add;
+    __syncthreads();
   }
 }
 

@@ -657,6 +657,18 @@ def _statevector_arg(psi: Any, device: int, stream: Any):
             s = stream if stream is not None else torch.cuda.current_stream(psi.device)
             return psi.data_ptr() or None, dtype, int(psi.numel()), 1, _stream_arg(s)
         psi = psi.numpy()
+    if (hasattr(psi, "__dlpack__") and hasattr(psi, "__dlpack_device__")
+            and not isinstance(psi, np.ndarray)):
+        # DLPack producers: kDLROCM / kDLCUDA (device type 10 / 2) are viewed
+        # in place through torch, kDLCPU (1) through NumPy.
+        dev_type = int(psi.__dlpack_device__()[0])
+        if dev_type in (2, 10):
+            import torch
+
+            return _statevector_arg(torch.from_dlpack(psi), device, stream)
+        if dev_type == 1:
+            return _statevector_arg(np.from_dlpack(psi), device, stream)
+        raise TypeError("psi must be a NumPy array or a device tensor")
     if hasattr(psi, "__cuda_array_interface__") and not isinstance(psi, np.ndarray):
         iface = psi.__cuda_array_interface__
         shape = tuple(iface["shape"])
