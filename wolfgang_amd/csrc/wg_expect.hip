@@ -71,6 +71,9 @@ constexpr int kDiagWhtMinItems = 4;
 // Grid of the tiled sigma launches.
 constexpr u64 kSigmaBlocks = 2048;
 constexpr u32 kDiagWhtBlocks = 1024;
+// Off-diagonal items of tiled passes use the owner sweep (one LDS read per
+// amplitude pair) instead of the per-wave item sweep.
+constexpr bool kOwnerSweep = true;
 
 __device__ inline u64 insert_zero(u64 v, u64 pivot) {
   const u64 low = pivot - 1;
@@ -616,6 +619,233 @@ constexpr std::array<TiledKernel, sizeof...(V)> tiled_kernel_table(std::integer_
 
 const std::array<TiledKernel, kVariants> kTiledKernels =
     tiled_kernel_table(std::make_integer_sequence<int, kVariants>{});
+
+// ------------------------------------------------------------ owner sweep
+
+// Off-diagonal items of a pass in "bundles" of item slots (2 or 4 keys per
+// slot); one launch per pass runs all its bundles, so the bundles that share
+// a tile read it from L2. A 512-thread block stages a tile and thread tid
+// keeps its points u = tid + 512 i (i < 8) in registers; for every slot it
+// evaluates the 4 pairs whose representative it owns, with one LDS read (the
+// partner) per pair:
+//  * register pivot (local bit 11 of x set): representatives are registers
+//    i = j < 4, partners (tid ^ x[0..8], j ^ x[9..11]);
+//  * lane pivot (x inside local bits 0..5): representatives are registers
+//    i = j + 4 h, h = bit `hbit` of tid, partners (tid ^ x, i).
+// For representative j and key k the pair's sign is (-1)^{parity(z & rep(t))}
+// (-1)^{c_k . j} times a per-thread sign applied once at the end (c_k: z's
+// local bits 9, 10). Each key needs one component of conj(psi[b ^ x]) psi[b]
+// (real for even |x & z|, imaginary for odd), which is all 2 Re(i^y S) uses.
+// The bundle's per-slot and per-key words live in scalar registers for the
+// whole sweep, so the tile loop issues no scalar loads (their completion
+// counter is shared with the LDS reads).
+//
+// Bundle kinds: bit 0 odd |x & z|, bit 1 lane pivot, bits 2.. the bundle
+// shape (slots, keys per slot).
+constexpr int kOwnerBlock = kTileBlock;
+constexpr int kOwnerWaves = kOwnerBlock / 64;
+constexpr int kOwnerShapes = 6;
+constexpr int kOwnerShapeSlots[kOwnerShapes] = {8, 4, 2, 4, 2, 1};
+constexpr int kOwnerShapeKeys[kOwnerShapes] = {2, 2, 2, 4, 4, 4};
+constexpr int kOwnerKinds = 4 * kOwnerShapes;
+constexpr int kOwnerMaxAcc = 16;
+// Lane-pivot slots also hold their representatives' selects: their largest
+// shapes would not fit the 128-register budget, so lane-pivot bundles stop at
+// 4 slots x 2 keys and 2 slots x 4 keys.
+constexpr int owner_first_shape(bool wide, bool lanex) { return (wide ? 3 : 0) + (lanex ? 1 : 0); }
+constexpr bool owner_kind_used(int kind) {
+  const int shape = kind >> 2;
+  const bool lanex = (kind >> 1) & 1, wide = shape >= 3;
+  return shape >= owner_first_shape(wide, lanex);
+}
+// Owner passes split the tiles over 512 blocks per bundle (two blocks per CU).
+constexpr u32 kOwnerTileBlocks = 512;
+static_assert(kTilePoints / kOwnerBlock == 8 && kOwnerWaves == 8,
+              "owner staging is written for 8 points per thread and 8 waves");
+
+__device__ __forceinline__ double unit_sign(u32 hi) { return __hiloint2double(static_cast<int>(hi), 0); }
+
+template <int M, int NZ, bool kLaneX, bool kOdd>
+__device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ psi,
+                                                   const PassDesc* __restrict__ pd,
+                                                   const OwnerBundle& B, u32 tb, u32 tile_blocks,
+                                                   u32 partial_base, double2* __restrict__ tile,
+                                                   double (*red)[kOwnerMaxAcc],
+                                                   double2* __restrict__ partials) {
+  static_assert(M * NZ <= kOwnerMaxAcc && M <= kOwnerMaxSlots && NZ <= kOwnerMaxKeys,
+                "owner bundle shape");
+  const u32 tid = threadIdx.x;
+  const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const u32 lane = tid & 63U;
+  const u64 free_mask = pd->free_mask;
+  const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  u64 sb_w = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if ((wave >> i) & 1U) sb_w ^= B.seg_basis[i];
+  }
+  const u64 sb_e0 = B.seg_basis[3], sb_e1 = B.seg_basis[4], sb_e2 = B.seg_basis[5];
+  const u32 tid16 = tid << 4;
+  u32 addr0[M], kw[M][NZ];
+  u32 hmask = 0;  // lane pivot: bit m = bit hbit_m of tid
+  double acc[M][NZ];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    addr0[m] = __builtin_amdgcn_readfirstlane(B.slot[m].addr0);
+    if constexpr (kLaneX) hmask |= ((tid >> B.slot[m].hbit) & 1U) << m;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      kw[m][k] = __builtin_amdgcn_readfirstlane(B.slot[m].key[k].w);
+      acc[m][k] = 0.0;
+    }
+  }
+  double2 st0, st1, st2, st3, st4, st5, st6, st7;
+  u64 t = tb;
+#define WG_OWNER_FETCH(rep)                                                 \
+  do {                                                                      \
+    const u64 r0_ = (rep) ^ sb_w, r1_ = r0_ ^ sb_e0, r2_ = r0_ ^ sb_e1;      \
+    const u64 r3_ = r1_ ^ sb_e1;                                            \
+    st0 = psi[r0_ | lane];                                                  \
+    st1 = psi[r1_ | lane];                                                  \
+    st2 = psi[r2_ | lane];                                                  \
+    st3 = psi[r3_ | lane];                                                  \
+    st4 = psi[(r0_ ^ sb_e2) | lane];                                        \
+    st5 = psi[(r1_ ^ sb_e2) | lane];                                        \
+    st6 = psi[(r2_ ^ sb_e2) | lane];                                        \
+    st7 = psi[(r3_ ^ sb_e2) | lane];                                        \
+  } while (0)
+  if (t < ntiles) WG_OWNER_FETCH(tile_rep(t, free_mask));
+  for (; t < ntiles; t += tile_blocks) {
+    __syncthreads();
+    tile[tid + 0 * kOwnerBlock] = st0;
+    tile[tid + 1 * kOwnerBlock] = st1;
+    tile[tid + 2 * kOwnerBlock] = st2;
+    tile[tid + 3 * kOwnerBlock] = st3;
+    tile[tid + 4 * kOwnerBlock] = st4;
+    tile[tid + 5 * kOwnerBlock] = st5;
+    tile[tid + 6 * kOwnerBlock] = st6;
+    tile[tid + 7 * kOwnerBlock] = st7;
+    const double2 own0 = st0, own1 = st1, own2 = st2, own3 = st3;
+    const double2 own4 = st4, own5 = st5, own6 = st6, own7 = st7;
+    __syncthreads();
+    const u64 tn = t + tile_blocks;
+    if (tn < ntiles) WG_OWNER_FETCH(tile_rep(tn, free_mask));
+    const u32 t32 = static_cast<u32>(t);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double2 a[4], p[4];
+      if constexpr (kLaneX) {
+        const bool h = (hmask >> m) & 1U;
+        a[0] = h ? own4 : own0;
+        a[1] = h ? own5 : own1;
+        a[2] = h ? own6 : own2;
+        a[3] = h ? own7 : own3;
+        const u32 base = (tid16 ^ addr0[m]) | (h ? (4U << 13) : 0U);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
+      } else {
+        a[0] = own0;
+        a[1] = own1;
+        a[2] = own2;
+        a[3] = own3;
+        const u32 base = tid16 ^ addr0[m];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
+      }
+      double v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = kOdd ? fma(p[j].x, a[j].y, -(p[j].y * a[j].x)) : fma(p[j].x, a[j].x, p[j].y * a[j].y);
+      }
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        const u32 w = kw[m][k];
+        const u32 h0 = 0x3FF00000U ^ (static_cast<u32>(__popc(w & t32)) << 31);
+        const u32 c1 = w & 0x80000000U, c0 = (w << 1) & 0x80000000U;
+        double s = acc[m][k];
+        s = fma(unit_sign(h0), v[0], s);
+        s = fma(unit_sign(h0 ^ c0), v[1], s);
+        s = fma(unit_sign(h0 ^ c1), v[2], s);
+        s = fma(unit_sign(h0 ^ c0 ^ c1), v[3], s);
+        acc[m][k] = s;
+      }
+    }
+  }
+#undef WG_OWNER_FETCH
+  // Per-thread sign, then a fixed-order block sum per slot key.
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const OwnerKey& K = B.slot[m].key[k];
+      u32 par = static_cast<u32>(__popc(K.zlt & tid)) & 1U;
+      if constexpr (kLaneX) par ^= (K.info >> 2) & (hmask >> m) & 1U;
+      double r = par ? -acc[m][k] : acc[m][k];
+      for (int d = 32; d > 0; d >>= 1) r += __shfl_xor(r, d, 64);
+      if (lane == 0) red[wave][m * NZ + k] = r;
+    }
+  }
+  __syncthreads();
+  if (tid < static_cast<u32>(M * NZ)) {
+    const int m = static_cast<int>(tid) / NZ, k = static_cast<int>(tid) % NZ;
+    const OwnerSlot& S = B.slot[m];
+    const u32 info = S.key[k].info;
+    if ((info >> 15) & 1U) {
+      double r = 0.0;
+      for (int w = 0; w < kOwnerWaves; ++w) r += red[w][tid];
+      const std::size_t row = static_cast<std::size_t>(partial_base) +
+                              static_cast<std::size_t>(S.item) * tile_blocks + tb;
+      partials[row * kZ + ((info >> 4) & 15U)] = make_double2(r, r);
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __restrict__ psi,
+                                                   const PassDesc* __restrict__ pd,
+                                                   const OwnerBundle& B, u32 tb, u32 tile_blocks,
+                                                   u32 partial_base, double2* __restrict__ tile,
+                                                   double (*red)[kOwnerMaxAcc],
+                                                   double2* __restrict__ partials) {
+  if constexpr (owner_kind_used(K)) {
+    if (kind != K) return;
+    constexpr int shape = K >> 2;
+    owner_bundle_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], ((K >> 1) & 1) != 0,
+                       (K & 1) != 0>(psi, pd, B, tb, tile_blocks, partial_base, tile, red,
+                                     partials);
+  }
+}
+
+template <int... K>
+__device__ __forceinline__ void owner_dispatch(int kind, std::integer_sequence<int, K...>,
+                                               const double2* __restrict__ psi,
+                                               const PassDesc* __restrict__ pd,
+                                               const OwnerBundle& B, u32 tb, u32 tile_blocks,
+                                               u32 partial_base, double2* __restrict__ tile,
+                                               double (*red)[kOwnerMaxAcc],
+                                               double2* __restrict__ partials) {
+  (owner_dispatch_one<K>(kind, psi, pd, B, tb, tile_blocks, partial_base, tile, red, partials), ...);
+}
+
+// Grid: bundles * tile_blocks blocks; the blocks of one tile block are
+// consecutive multiples of 8 apart (one XCD, close in time), as in
+// expect_tiled_kernel. 128 VGPRs per lane: two blocks per CU.
+__global__ __launch_bounds__(kOwnerBlock) __attribute__((amdgpu_waves_per_eu(4)))
+void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
+                         const OwnerBundle* __restrict__ bundles, u32 groups, u32 tile_blocks,
+                         u32 partial_base, double2* __restrict__ partials) {
+  __shared__ double2 tile[kTilePoints];
+  __shared__ double red[kOwnerWaves][kOwnerMaxAcc];
+  const u32 b = blockIdx.x;
+  const u32 xcd = b & 7U;
+  const u32 slot_id = b >> 3;
+  const u32 ig = slot_id % groups;
+  const u32 tb = (slot_id / groups) * 8U + xcd;
+  const OwnerBundle& B = bundles[ig];
+  const int kind = static_cast<int>(__builtin_amdgcn_readfirstlane(B.kind));
+  owner_dispatch(kind, std::make_integer_sequence<int, kOwnerKinds>{}, psi, pd, B, tb, tile_blocks,
+                 partial_base, tile, red, partials);
+}
 
 // ------------------------------------------------------------ tiled sigma
 
@@ -1502,6 +1732,223 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
   items = std::move(ordered);
 }
 
+namespace {
+
+// Bundle layout for a register-pivot bundle: new segment coordinate 5 is the
+// functional lam (a set bit of every item's x lands at local bit 11); the
+// others are the old coordinates except lam's top bit p, in order. With
+// s = A s' the basis images are A e'_i = e_{pi(i)} (+ e_p if lam has bit
+// pi(i)) for i < 5 and A e'_5 = e_p.
+struct SegLayout {
+  u32 img[6];  // A e'_i as 6-bit old segment vectors
+  u32 lam;
+  int p;
+  int pi[5];
+};
+
+SegLayout seg_layout(u32 lam) {
+  SegLayout L{};
+  L.lam = lam;
+  if (lam == 0) {
+    for (int i = 0; i < 6; ++i) L.img[i] = 1U << i;
+    L.p = -1;
+    return L;
+  }
+  L.p = 31 - __builtin_clz(lam);
+  int c = 0;
+  for (int i = 0; i < 6; ++i) {
+    if (i != L.p) L.pi[c++] = i;
+  }
+  for (int i = 0; i < 5; ++i) {
+    L.img[i] = (1U << L.pi[i]) | (((lam >> L.pi[i]) & 1U) << L.p);
+  }
+  L.img[5] = 1U << L.p;
+  return L;
+}
+
+// x (12-bit pass-local) in the bundle layout.
+u32 layout_x(const SegLayout& L, u32 xl) {
+  if (L.p < 0) return xl;
+  const u32 s = (xl >> 6) & 63U;
+  u32 s2 = 0;
+  for (int i = 0; i < 5; ++i) s2 |= ((s >> L.pi[i]) & 1U) << i;
+  s2 |= static_cast<u32>(host_parity(L.lam & s)) << 5;
+  return (xl & 63U) | (s2 << 6);
+}
+
+// z character (12-bit pass-local) in the bundle layout: bit i of the new
+// segment part is parity(zl_seg & A e'_i).
+u32 layout_z(const SegLayout& L, u32 zl) {
+  const u32 s = (zl >> 6) & 63U;
+  u32 s2 = 0;
+  for (int i = 0; i < 6; ++i) s2 |= static_cast<u32>(host_parity(s & L.img[i])) << i;
+  return (zl & 63U) | (s2 << 6);
+}
+
+u64 host_compress(u64 v, u64 mask) {
+  u64 out = 0;
+  int k = 0;
+  while (mask) {
+    const u64 low = mask & (0 - mask);
+    if (v & low) out |= 1ULL << k;
+    ++k;
+    mask ^= low;
+  }
+  return out;
+}
+
+}  // namespace
+
+void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
+                        std::vector<OwnerBundle>& bundles) {
+  struct Cand {
+    u32 item;  // relative to the pass
+    u32 xl;
+    int odd;
+    int ncols;
+    int cols[kOwnerMaxKeys];
+    int wide;
+    int lanex;
+  };
+  for (PassLaunch& pl : passes) {
+    const PassDesc& d = pl.desc;
+    // The diagonal items keep their sweep variants; the off-diagonal ones move
+    // to owner bundles.
+    std::vector<SubLaunch> keep;
+    u32 groups = 0;
+    for (const SubLaunch& sub : pl.subs) {
+      if (sub.variant >= 8 && sub.variant < 16) {
+        keep.push_back(sub);
+        groups += sub.groups;
+      }
+    }
+    pl.subs = keep;
+    pl.owner_first = static_cast<u32>(bundles.size());
+    pl.owner_count = 0;
+    std::vector<Cand> cands;
+    for (u32 r = 0; r < pl.num_items; ++r) {
+      const ExpItem& e = items[pl.first_item + r];
+      if (e.x == 0) continue;
+      for (int odd = 0; odd < 2; ++odd) {
+        std::vector<int> cols;
+        for (int j = 0; j < e.nz; ++j) {
+          if ((e.y[j] & 1) == odd) cols.push_back(j);
+        }
+        const int wide = cols.size() > 2 ? 1 : 0;
+        const std::size_t per = wide ? 4 : 2;
+        for (std::size_t s = 0; s < cols.size(); s += per) {
+          Cand c{};
+          c.item = r;
+          c.xl = static_cast<u32>(e.xl);
+          c.odd = odd;
+          c.ncols = static_cast<int>(std::min(per, cols.size() - s));
+          for (int q = 0; q < c.ncols; ++q) c.cols[q] = cols[s + q];
+          c.wide = wide;
+          c.lanex = ((c.xl >> 6) & 63U) == 0 ? 1 : 0;
+          cands.push_back(c);
+        }
+      }
+    }
+    // Bundles of a group: full bundles of the largest shape, then the
+    // smallest shape that holds the rest.
+    auto emit = [&](const std::vector<Cand>& group, const SegLayout& L, int base_kind, bool wide) {
+      const int first_shape = owner_first_shape(wide, (base_kind >> 1) & 1);
+      const int last_shape = wide ? 5 : 2;
+      static_assert(kOwnerShapes == 6, "three narrow and three wide bundle shapes");
+      for (std::size_t s = 0; s < group.size();) {
+        const std::size_t left = group.size() - s;
+        int shape = first_shape;
+        for (int q = last_shape; q >= first_shape; --q) {
+          if (static_cast<std::size_t>(kOwnerShapeSlots[q]) >= left) {
+            shape = q;
+            break;
+          }
+        }
+        const int M = kOwnerShapeSlots[shape];
+        const int kind = shape * 4 + base_kind;
+        const bool lanex = (base_kind >> 1) & 1;
+        OwnerBundle bd{};
+        for (int i = 0; i < 6; ++i) bd.seg_basis[i] = d.seg_off[L.img[i]];
+        bd.kind = static_cast<u32>(kind);
+        for (int m = 0; m < M; ++m) {
+          OwnerSlot& S = bd.slot[m];
+          if (s + m >= group.size()) {
+            // Padding: in-bounds reads, no keys written.
+            S.addr0 = lanex ? 0U : (4U << 13);
+            continue;
+          }
+          const Cand& c = group[s + m];
+          const ExpItem& e = items[pl.first_item + c.item];
+          const u32 xl = layout_x(L, c.xl);
+          S.item = c.item;
+          const u32 xt16 = (xl & 511U) << 4, xr = (xl >> 9) & 7U;
+          if (c.lanex) {
+            if (xl >= 64U) fail_runtime("lane-pivot item has x outside the lane bits");
+            S.hbit = static_cast<u32>(31 - __builtin_clz(xl));
+            S.addr0 = xt16;
+          } else {
+            if (xr < 4U) fail_runtime("owner bundle layout misses an item's x");
+            S.addr0 = xt16 | (xr << 13);
+          }
+          for (int q = 0; q < c.ncols; ++q) {
+            const int j = c.cols[q];
+            const u32 zl = layout_z(L, static_cast<u32>(e.zl[j]));
+            OwnerKey& K = S.key[q];
+            const u64 zt = host_compress(e.z[j], d.free_mask);
+            if (zt >> 30) fail_runtime("owner sweep supports at most 42 qubits");
+            K.w = static_cast<u32>(zt) | (((zl >> 9) & 1U) << 30) | (((zl >> 10) & 1U) << 31);
+            K.zlt = zl & 511U;
+            K.info = (((zl >> 11) & 1U) << 2) | (static_cast<u32>(j) << 4) | (1U << 15);
+          }
+        }
+        bundles.push_back(bd);
+        ++pl.owner_count;
+        s += static_cast<std::size_t>(M);
+      }
+    };
+    for (int wide = 0; wide < 2; ++wide) {
+      for (int base_kind = 0; base_kind < 4; ++base_kind) {
+        std::vector<Cand> pool;
+        for (const Cand& c : cands) {
+          if (c.odd == (base_kind & 1) && c.lanex == ((base_kind >> 1) & 1) && c.wide == wide) {
+            pool.push_back(c);
+          }
+        }
+        if (pool.empty()) continue;
+        if ((base_kind >> 1) & 1) {
+          emit(pool, seg_layout(0), base_kind, wide);
+          continue;
+        }
+        // Greedy: the functional covering the most remaining items, repeatedly.
+        while (!pool.empty()) {
+          u32 best = 1;
+          std::size_t best_n = 0;
+          for (u32 lam = 1; lam < 64; ++lam) {
+            std::size_t cnt = 0;
+            for (const Cand& c : pool) cnt += host_parity(lam & ((c.xl >> 6) & 63U));
+            if (cnt > best_n) {
+              best_n = cnt;
+              best = lam;
+            }
+          }
+          std::vector<Cand> take, rest;
+          for (const Cand& c : pool) {
+            (host_parity(best & ((c.xl >> 6) & 63U)) ? take : rest).push_back(c);
+          }
+          emit(take, seg_layout(best), base_kind, wide);
+          pool = std::move(rest);
+        }
+      }
+    }
+    groups += pl.owner_count;
+    pl.item_groups = std::max<u32>(1, groups);
+    if (pl.owner_count) {
+      const u64 ntiles = 1ULL << (d.n - d.k);
+      pl.tile_blocks = static_cast<u32>(std::min<u64>(kOwnerTileBlocks, std::max<u64>(8, ntiles / 8 * 8)));
+    }
+  }
+}
+
 // ------------------------------------------------------------ plan
 
 ExpectationPlan::~ExpectationPlan() {
@@ -1521,6 +1968,7 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_passes);
   device_free(d_flat_key);
   device_free(d_key_terms);
+  device_free(d_bundles);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -1656,6 +2104,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
       for (const PassLaunch& p : passes) {
         if (p.desc.k != kTileMaxLog) fail_runtime("tiled expectation plan has a short tile");
       }
+      if (kOwnerSweep) plan_owner_bundles(tiled_items, passes, plan->bundles);
       items = std::move(tiled_items);
       plan->passes = std::move(passes);
       plan->diag_items = diag_front;
@@ -1724,6 +2173,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     upload(plan->d_passes, descs);
   }
   upload(plan->d_key_terms, key_terms);
+  upload(plan->d_bundles, plan->bundles);
   sync(stream);
   return plan;
 }
@@ -1763,6 +2213,14 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
               sub.count, sub.groups, p.tile_blocks, p.partial_base + sub.first * p.tile_blocks,
               partials);
           check_launch("expect_tiled");
+        }
+        if (p.owner_count) {
+          const unsigned grid = p.owner_count * p.tile_blocks;
+          ProfileScope prof("expect_owner", stream);
+          expect_owner_kernel<<<grid, kOwnerBlock, 0, stream>>>(
+              psi, plan.d_passes + (&p - plan.passes.data()), plan.d_bundles + p.owner_first,
+              p.owner_count, p.tile_blocks, p.partial_base, partials);
+          check_launch("expect_owner");
         }
       }
     } else {
@@ -2003,6 +2461,8 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
     plan_tiled_passes(num_qubits, items, passes,
                       diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ, nullptr,
                       SIZE_MAX);
+    std::vector<OwnerBundle> bundles;
+    if (kOwnerSweep) plan_owner_bundles(items, passes, bundles);
     *num_passes = passes.size();
     for (std::size_t p = 0; p < passes.size() && p < max_passes; ++p) {
       int diag = 0;
@@ -2066,6 +2526,8 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
         n >= kTileMaxLog + 3 && diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ;
     u32 diag_front = 0;
     plan_tiled_passes(n, items, passes, diag_separate, &diag_front, SIZE_MAX);
+    std::vector<OwnerBundle> bundles;
+    if (kOwnerSweep) plan_owner_bundles(items, passes, bundles);
     *num_passes = passes.size();
     *num_keys = kx.size();
     const auto* amp = reinterpret_cast<const double2*>(psi);
@@ -2110,6 +2572,7 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
       std::fill(seen.begin(), seen.end(), 0);
       tile.assign(tile_size, make_double2(0, 0));
       std::vector<double2> acc(static_cast<std::size_t>(p.num_items) * kZ, make_double2(0, 0));
+      std::vector<double2> owner_acc(acc.size(), make_double2(0, 0));
       for (u64 t = 0; t < ntiles; ++t) {
         u64 rep = 0;
         for (int j = 0; j < d.nfree; ++j) rep |= ((t >> j) & 1ULL) << d.free_pos[j];
@@ -2119,8 +2582,59 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
           seen[b] = 1;
           tile[u] = amp[b];
         }
+        // Owner bundles: the kernel's representatives, partners and signs,
+        // with the points addressed through each bundle's own layout.
+        for (u32 bi = p.owner_first; bi < p.owner_first + p.owner_count; ++bi) {
+          const OwnerBundle& bd = bundles[bi];
+          const int M = kOwnerShapeSlots[bd.kind >> 2], NZ = kOwnerShapeKeys[bd.kind >> 2];
+          const bool lanex = (bd.kind >> 1) & 1, odd = bd.kind & 1;
+          auto point = [&](u32 u) {
+            u64 off = 0;
+            for (int i = 0; i < 6; ++i) {
+              if ((u >> (6 + i)) & 1U) off ^= bd.seg_basis[i];
+            }
+            return (rep ^ off) | (u & 63U);
+          };
+          if (t == 0) {
+            std::vector<u64> pts(kTilePoints), ref(kTilePoints);
+            for (u32 u = 0; u < kTilePoints; ++u) {
+              pts[u] = point(u);
+              ref[u] = (rep ^ d.seg_off[u >> d.l]) | (u & 63U);
+            }
+            std::sort(pts.begin(), pts.end());
+            std::sort(ref.begin(), ref.end());
+            if (pts != ref) fail_runtime("owner bundle layout does not span the pass's tile");
+          }
+          for (int m = 0; m < M; ++m) {
+            const OwnerSlot& S = bd.slot[m];
+            for (u32 tid = 0; tid < static_cast<u32>(kOwnerBlock); ++tid) {
+              const u32 h = lanex ? (tid >> S.hbit) & 1U : 0U;
+              for (u32 j = 0; j < 4; ++j) {
+                const u32 i_own = lanex ? j + 4 * h : j;
+                const u32 par = lanex ? (((tid << 4) ^ S.addr0) | (h << 15)) + (j << 13)
+                                      : (tid << 4) ^ S.addr0 ^ (j << 13);
+                if (par >= kTilePoints * 16U) fail_runtime("owner partner outside the tile");
+                const double2 A = amp[point(tid + kOwnerBlock * i_own)];
+                const double2 P = amp[point(par >> 4)];
+                const double v = odd ? P.x * A.y - P.y * A.x : P.x * A.x + P.y * A.y;
+                for (int q = 0; q < NZ; ++q) {
+                  const OwnerKey& K = S.key[q];
+                  if (!((K.info >> 15) & 1U)) continue;
+                  u32 sb = (__builtin_popcount(K.w & 0x3FFFFFFFU & static_cast<u32>(t)) & 1U) ^
+                           (__builtin_popcount(K.zlt & tid) & 1U) ^ ((K.w >> 30) & j & 1U) ^
+                           ((K.w >> 31) & (j >> 1) & 1U);
+                  if (lanex) sb ^= (K.info >> 2) & h & 1U;
+                  double2& a2 = owner_acc[static_cast<std::size_t>(S.item) * kZ + ((K.info >> 4) & 15U)];
+                  a2.x += sb ? -v : v;
+                  a2.y += sb ? -v : v;
+                }
+              }
+            }
+          }
+        }
         for (u32 i = 0; i < p.num_items; ++i) {
           const ExpItem& it = items[p.first_item + i];
+          if (kOwnerSweep && it.x != 0) continue;
           const u32 xl = static_cast<u32>(it.xl), pbit = static_cast<u32>(it.pbit);
           auto ins0 = [&](u32 v) {
             const u32 low = pbit - 1;
@@ -2192,7 +2706,7 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
       for (u32 i = 0; i < p.num_items; ++i) {
         const ExpItem& it = items[p.first_item + i];
         for (int j = 0; j < it.nz; ++j) {
-          const double2 S = acc[static_cast<std::size_t>(i) * kZ + j];
+          const double2 S = (kOwnerSweep && it.x != 0 ? owner_acc : acc)[static_cast<std::size_t>(i) * kZ + j];
           double v;
           if (it.x == 0) {
             v = S.x;

@@ -57,6 +57,42 @@ struct PassDesc {
   unsigned char free_pos[64];
 };
 
+// Owner sweep of a tiled pass's off-diagonal items: thread tid of a
+// 512-thread block owns the tile points u = tid + 512 i (i < 8, local bits
+// 9..11 = i) in registers and evaluates every pair {u, u ^ x} of an item once,
+// from the point whose pivot bit is clear, reading only the partner from LDS.
+// Each bundle may change the basis of the 6 segment coordinates (local bits
+// 6..11) so that every item of a register-pivot bundle has local bit 11 set;
+// items inside the low 6 bits use a lane pivot.
+constexpr int kOwnerMaxSlots = 8;
+constexpr int kOwnerMaxKeys = 4;
+
+struct OwnerKey {
+  u32 w;     // z compressed to the tile-index bits (bits 0..29: parity(z & rep(t)) =
+             // parity(w & t)) | (z local bit 9) << 30 | (z local bit 10) << 31
+  u32 zlt;   // z in bundle-local coordinates, bits 0..8 (the thread part of the sign)
+  u32 info;  // bit 2: z local bit 11; bits 4..7: partial column (key index in the
+             // item); bit 15: slot key in use
+  u32 pad;
+};
+
+struct OwnerSlot {
+  u32 item;   // item index relative to the pass's first item
+  u32 hbit;   // lane pivot: the pivot bit of x (< 6)
+  u32 addr0;  // (x local bits 0..8) << 4 | (x local bits 9..11) << 13: the partner of
+              // representative j of thread tid is at byte (tid << 4) ^ addr0 ^ (j << 13)
+              // (register pivot) or ((tid << 4) ^ addr0 | h << 15) + (j << 13) (lane pivot)
+  u32 pad;
+  OwnerKey key[kOwnerMaxKeys];
+};
+
+struct OwnerBundle {
+  u64 seg_basis[6];  // global offsets of the 6 segment coordinates in this layout
+  u32 kind;          // bit 0 odd |x & z|, bit 1 lane pivot, bits 2.. shape
+  u32 pad;
+  OwnerSlot slot[kOwnerMaxSlots];
+};
+
 // Items of one sweep variant inside a pass: one launch.
 struct SubLaunch {
   int variant;
@@ -73,6 +109,8 @@ struct PassLaunch {
   u32 tile_blocks;   // blocks per item group (multiple of 8)
   u32 partial_base;  // first partial row of this pass
   std::vector<SubLaunch> subs;
+  u32 owner_first = 0;  // owner bundles of the pass (plan-wide indices), one launch
+  u32 owner_count = 0;
 };
 
 struct ExpectationPlan {
@@ -86,6 +124,8 @@ struct ExpectationPlan {
   bool tiled = false;
   u32 blocks_per_item = 1;           // direct path
   std::vector<PassLaunch> passes;    // tiled path
+  std::vector<OwnerBundle> bundles;  // owner sweep bundles of all passes
+  OwnerBundle* d_bundles = nullptr;
   u32 diag_items = 0;                // tiled path: leading diagonal items swept by the
                                      // Walsh-Hadamard kernel (0: they ride in pass 0)
   u32 diag_keys = 0;
@@ -126,5 +166,10 @@ std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
 // `passes`) once the plan needs more than max_passes passes.
 void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
                        bool diag_separate, u32* num_diag_front, std::size_t max_passes);
+
+// Owner-sweep bundles for the off-diagonal items of every pass (appended to
+// `bundles`; each pass records its range).
+void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
+                        std::vector<OwnerBundle>& bundles);
 
 }  // namespace wg
