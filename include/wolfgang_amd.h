@@ -220,7 +220,8 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
 
 /* Host-only: per-pass summary of the tiled plan of a W = 1 operator (test and
  * tuning hook): rows of {k, items, item groups, tile blocks, diagonal items,
- * off-diagonal items}, up to max_passes rows; *num_passes gets the count. */
+ * off-diagonal items, owner bundles, owner slots}, up to max_passes rows;
+ * *num_passes gets the count. */
 int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                          int* rows, size_t max_passes, size_t* num_passes);
 

@@ -658,8 +658,15 @@ constexpr bool owner_kind_used(int kind) {
   const bool lanex = (kind >> 1) & 1, wide = shape >= 3;
   return shape >= owner_first_shape(wide, lanex);
 }
-// Owner passes split the tiles over 512 blocks per bundle (two blocks per CU).
-constexpr u32 kOwnerTileBlocks = 512;
+// Owner passes give each block about kOwnerTilesPerBlock tiles: the blocks of
+// the bundles that share a tile start together and stay close, so all but the
+// first read of each tile hit L2 (long runs drift apart by hundreds of tiles).
+// A launch keeps at least kOwnerMinBlocks blocks (two per CU).
+constexpr u64 kOwnerTilesPerBlock = 64;
+constexpr u64 kOwnerMinBlocks = 512;
+// Partial rows of one owner pass (items x tile blocks, kZ values each) stay
+// within this many rows.
+constexpr u64 kOwnerMaxPassRows = u64{1} << 20;
 static_assert(kTilePoints / kOwnerBlock == 8 && kOwnerWaves == 8,
               "owner staging is written for 8 points per thread and 8 waves");
 
@@ -731,31 +738,50 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
     const u64 tn = t + tile_blocks;
     if (tn < ntiles) WG_OWNER_FETCH(tile_rep(tn, free_mask));
     const u32 t32 = static_cast<u32>(t);
+    // Register-pivot slots: slot m's partners are read while slot m - 1 is
+    // evaluated (two partner sets in flight; lane-pivot slots also hold all 8
+    // points and read just in time). The partner base is formed inside the
+    // loop so the compiler keeps one address per slot instead of hoisting four.
+    constexpr bool kPipe = !kLaneX;
+    double2 p[2][4];
+    auto fetch = [&](int m, double2 (&q)[4]) {
+      u32 base;
+      asm volatile("v_xor_b32 %0, %1, %2" : "=v"(base) : "s"(addr0[m]), "v"(tid16));
+      if constexpr (kLaneX) {
+        base |= ((hmask >> m) & 1U) ? (4U << 13) : 0U;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
+      }
+    };
+    if constexpr (kPipe) fetch(0, p[0]);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      double2 a[4], p[4];
+      if constexpr (kPipe) {
+        if (m + 1 < M) fetch(m + 1, p[(m + 1) & 1]);
+      } else {
+        fetch(m, p[m & 1]);
+      }
+      const double2(&q)[4] = p[m & 1];
+      double2 a[4];
       if constexpr (kLaneX) {
         const bool h = (hmask >> m) & 1U;
         a[0] = h ? own4 : own0;
         a[1] = h ? own5 : own1;
         a[2] = h ? own6 : own2;
         a[3] = h ? own7 : own3;
-        const u32 base = (tid16 ^ addr0[m]) | (h ? (4U << 13) : 0U);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) p[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
       } else {
         a[0] = own0;
         a[1] = own1;
         a[2] = own2;
         a[3] = own3;
-        const u32 base = tid16 ^ addr0[m];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) p[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
       }
       double v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        v[j] = kOdd ? fma(p[j].x, a[j].y, -(p[j].y * a[j].x)) : fma(p[j].x, a[j].x, p[j].y * a[j].y);
+        v[j] = kOdd ? fma(q[j].x, a[j].y, -(q[j].y * a[j].x)) : fma(q[j].x, a[j].x, q[j].y * a[j].y);
       }
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
@@ -1110,39 +1136,67 @@ __global__ __launch_bounds__(kTileBlock) void expect_diag_wht_kernel(
   }
 }
 
-// One thread per (item, slot): ordered sum over the item's partial rows, then
-// the key value <P> = 2 Re(i^y S) (x != 0) or Re S (x == 0).
-__global__ __launch_bounds__(256) void expect_reduce_kernel(const ExpItem* __restrict__ items,
-                                                            u32 num_items,
-                                                            const u32* __restrict__ offset,
-                                                            const u32* __restrict__ count,
-                                                            const double2* __restrict__ partials,
-                                                            double* __restrict__ key_vals) {
-  const std::size_t gid = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::size_t item_id = gid / kZ;
-  const int j = static_cast<int>(gid % kZ);
-  if (item_id >= num_items) return;
+// One block per item: thread r sums the item's partial rows r, r + 256, ...
+// for every key column, a fixed-shape tree adds the 256 sums, and the key
+// value is <P> = 2 Re(i^y S) (x != 0) or Re S (x == 0).
+constexpr int kReduceBlock = 256;
+
+__global__ __launch_bounds__(kReduceBlock) void expect_reduce_kernel(
+    const ExpItem* __restrict__ items, const u32* __restrict__ offset,
+    const u32* __restrict__ count, const double2* __restrict__ partials,
+    double* __restrict__ key_vals) {
+  __shared__ double2 lds[kReduceBlock / 64][kZ];
+  const u32 item_id = blockIdx.x;
   const ExpItem& it = items[item_id];
-  if (j >= it.nz) return;
-  double r = 0.0, i = 0.0;
+  const int nz = it.nz;
   const std::size_t base = offset[item_id];
-  for (u32 c = 0; c < count[item_id]; ++c) {
-    const double2 p = partials[(base + c) * kZ + j];
-    r += p.x;
-    i += p.y;
-  }
-  double val;
-  if (it.x == 0) {
-    val = r;
-  } else {
-    switch (it.y[j] & 3) {
-      case 0: val = 2.0 * r; break;
-      case 1: val = -2.0 * i; break;
-      case 2: val = -2.0 * r; break;
-      default: val = 2.0 * i; break;
+  const u32 rows = count[item_id];
+  double r[kZ], i[kZ];
+#pragma unroll
+  for (int j = 0; j < kZ; ++j) r[j] = i[j] = 0.0;
+  for (u32 c = threadIdx.x; c < rows; c += kReduceBlock) {
+    const double2* row = partials + (base + c) * kZ;
+#pragma unroll
+    for (int j = 0; j < kZ; ++j) {
+      if (j < nz) {
+        const double2 p = row[j];
+        r[j] += p.x;
+        i[j] += p.y;
+      }
     }
   }
-  key_vals[it.key[j]] = val;
+  const u32 wave = threadIdx.x >> 6, lane = threadIdx.x & 63U;
+#pragma unroll
+  for (int j = 0; j < kZ; ++j) {
+    if (j < nz) {
+      for (int d = 32; d > 0; d >>= 1) {
+        r[j] += __shfl_xor(r[j], d, 64);
+        i[j] += __shfl_xor(i[j], d, 64);
+      }
+      if (lane == 0) lds[wave][j] = make_double2(r[j], i[j]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < static_cast<u32>(nz)) {
+    const int j = static_cast<int>(threadIdx.x);
+    double sr = 0.0, si = 0.0;
+    for (int w = 0; w < kReduceBlock / 64; ++w) {
+      sr += lds[w][j].x;
+      si += lds[w][j].y;
+    }
+    double val;
+    if (it.x == 0) {
+      val = sr;
+    } else {
+      switch (it.y[j] & 3) {
+        case 0: val = 2.0 * sr; break;
+        case 1: val = -2.0 * si; break;
+        case 2: val = -2.0 * sr; break;
+        default: val = 2.0 * si; break;
+      }
+    }
+    key_vals[it.key[j]] = val;
+  }
 }
 
 // E = sum_t c_t <P_key(t)>: one block, strided per-thread sums in term order,
@@ -1944,7 +1998,12 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
     pl.item_groups = std::max<u32>(1, groups);
     if (pl.owner_count) {
       const u64 ntiles = 1ULL << (d.n - d.k);
-      pl.tile_blocks = static_cast<u32>(std::min<u64>(kOwnerTileBlocks, std::max<u64>(8, ntiles / 8 * 8)));
+      u64 tb = std::max<u64>(ntiles / kOwnerTilesPerBlock,
+                             (kOwnerMinBlocks + pl.item_groups - 1) / pl.item_groups);
+      u64 pow2 = 8;
+      while (pow2 < tb) pow2 <<= 1;
+      while (pow2 > 8 && pow2 * pl.num_items > kOwnerMaxPassRows) pow2 >>= 1;
+      pl.tile_blocks = static_cast<u32>(std::min<u64>(pow2, std::max<u64>(8, ntiles)));
     }
   }
 }
@@ -2130,6 +2189,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
       }
       slot += static_cast<std::size_t>(p.num_items) * p.tile_blocks;
     }
+    if (slot > 0xFFFFFFFFULL) fail_runtime("tiled expectation plan has too many partial rows");
     plan->partial_slots = slot;
   } else {
     const u64 range = plan->n == 0 ? 1 : (u64{1} << (plan->n - 1));
@@ -2230,9 +2290,8 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
                              stream>>>(psi, len, plan.d_items, plan.blocks_per_item, partials);
       check_launch("expect_direct");
     }
-    expect_reduce_kernel<<<grid_for(plan.num_items * kZ, 256), 256, 0, stream>>>(
-        plan.d_items, static_cast<u32>(plan.num_items), plan.d_partial_offset,
-        plan.d_partial_count, partials, key_vals);
+    expect_reduce_kernel<<<static_cast<unsigned>(plan.num_items), kReduceBlock, 0, stream>>>(
+        plan.d_items, plan.d_partial_offset, plan.d_partial_count, partials, key_vals);
     check_launch("expect_reduce");
   }
   expect_finalize_kernel<<<1, 1024, 0, stream>>>(coeffs, plan.d_term_key, plan.T, key_vals,
@@ -2431,7 +2490,8 @@ int wg_expectation_plan_info(const wg_pauli_sum* op_c, size_t* distinct_keys, si
 
 // Host-only: the tiled pass plan of a W = 1 operator, without replay. Writes
 // up to max_passes rows of {k, items, item_groups, tile_blocks, diagonal
-// items, off-diagonal items} and the pass count.
+// items, off-diagonal items, owner bundles, owner slots} and the pass count.
+constexpr int kDebugPassColumns = 8;
 int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                          int* rows, size_t max_passes, size_t* num_passes) {
   return guard([&] {
@@ -2467,12 +2527,19 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
     for (std::size_t p = 0; p < passes.size() && p < max_passes; ++p) {
       int diag = 0;
       for (u32 i = 0; i < passes[p].num_items; ++i) diag += items[passes[p].first_item + i].x == 0;
-      rows[6 * p + 0] = passes[p].desc.k;
-      rows[6 * p + 1] = static_cast<int>(passes[p].num_items);
-      rows[6 * p + 2] = static_cast<int>(passes[p].item_groups);
-      rows[6 * p + 3] = static_cast<int>(passes[p].tile_blocks);
-      rows[6 * p + 4] = diag;
-      rows[6 * p + 5] = static_cast<int>(passes[p].num_items) - diag;
+      int slots = 0;
+      for (u32 b = passes[p].owner_first; b < passes[p].owner_first + passes[p].owner_count; ++b) {
+        slots += kOwnerShapeSlots[bundles[b].kind >> 2];
+      }
+      int* row = rows + kDebugPassColumns * p;
+      row[0] = passes[p].desc.k;
+      row[1] = static_cast<int>(passes[p].num_items);
+      row[2] = static_cast<int>(passes[p].item_groups);
+      row[3] = static_cast<int>(passes[p].tile_blocks);
+      row[4] = diag;
+      row[5] = static_cast<int>(passes[p].num_items) - diag;
+      row[6] = static_cast<int>(passes[p].owner_count);
+      row[7] = slots;
     }
   });
 }
