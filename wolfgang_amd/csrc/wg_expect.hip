@@ -678,6 +678,7 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
                                                    const OwnerBundle& B, u32 tb, u32 tile_blocks,
                                                    u32 partial_base, double2* __restrict__ tile,
                                                    double (*red)[kOwnerMaxAcc],
+                                                   u64* __restrict__ step_mask,
                                                    double2* __restrict__ partials) {
   static_assert(M * NZ <= kOwnerMaxAcc && M <= kOwnerMaxSlots && NZ <= kOwnerMaxKeys,
                 "owner bundle shape");
@@ -706,8 +707,15 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
       acc[m][k] = 0.0;
     }
   }
+  // The block's tiles t_s = tb + tile_blocks * gray(s), s < run, in Gray-code
+  // order: the coset representative of t_{s+1} is that of t_s XOR one free
+  // index bit, step_mask[ctz(s + 1)] (a software pdep per tile would cost
+  // ~70 scalar instructions).
+  const u64 run = ntiles / tile_blocks;
+  const int log_tb = __builtin_ctz(tile_blocks);
+  if (tid < 32U) step_mask[tid] = tile_rep(static_cast<u64>(tile_blocks) << tid, free_mask);
   double2 st0, st1, st2, st3, st4, st5, st6, st7;
-  u64 t = tb;
+  u64 rep = tile_rep(tb, free_mask);
 #define WG_OWNER_FETCH(rep)                                                 \
   do {                                                                      \
     const u64 r0_ = (rep) ^ sb_w, r1_ = r0_ ^ sb_e0, r2_ = r0_ ^ sb_e1;      \
@@ -721,8 +729,8 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
     st6 = psi[(r2_ ^ sb_e2) | lane];                                        \
     st7 = psi[(r3_ ^ sb_e2) | lane];                                        \
   } while (0)
-  if (t < ntiles) WG_OWNER_FETCH(tile_rep(t, free_mask));
-  for (; t < ntiles; t += tile_blocks) {
+  WG_OWNER_FETCH(rep);
+  for (u64 step = 0; step < run; ++step) {
     __syncthreads();
     tile[tid + 0 * kOwnerBlock] = st0;
     tile[tid + 1 * kOwnerBlock] = st1;
@@ -735,9 +743,13 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
     const double2 own0 = st0, own1 = st1, own2 = st2, own3 = st3;
     const double2 own4 = st4, own5 = st5, own6 = st6, own7 = st7;
     __syncthreads();
-    const u64 tn = t + tile_blocks;
-    if (tn < ntiles) WG_OWNER_FETCH(tile_rep(tn, free_mask));
-    const u32 t32 = static_cast<u32>(t);
+    const u32 t32 = tb | (static_cast<u32>(step ^ (step >> 1)) << log_tb);
+    if (step + 1 < run) {
+      const u64 m = step_mask[__builtin_ctzll(step + 1)];
+      rep ^= (static_cast<u64>(__builtin_amdgcn_readfirstlane(static_cast<u32>(m >> 32))) << 32) |
+             __builtin_amdgcn_readfirstlane(static_cast<u32>(m));
+      WG_OWNER_FETCH(rep);
+    }
     // Register-pivot slots: slot m's partners are read while slot m - 1 is
     // evaluated (two partner sets in flight; lane-pivot slots also hold all 8
     // points and read just in time). The partner base is formed inside the
@@ -832,13 +844,14 @@ __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __re
                                                    const OwnerBundle& B, u32 tb, u32 tile_blocks,
                                                    u32 partial_base, double2* __restrict__ tile,
                                                    double (*red)[kOwnerMaxAcc],
+                                                   u64* __restrict__ step_mask,
                                                    double2* __restrict__ partials) {
   if constexpr (owner_kind_used(K)) {
     if (kind != K) return;
     constexpr int shape = K >> 2;
     owner_bundle_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], ((K >> 1) & 1) != 0,
                        (K & 1) != 0>(psi, pd, B, tb, tile_blocks, partial_base, tile, red,
-                                     partials);
+                                     step_mask, partials);
   }
 }
 
@@ -849,8 +862,11 @@ __device__ __forceinline__ void owner_dispatch(int kind, std::integer_sequence<i
                                                const OwnerBundle& B, u32 tb, u32 tile_blocks,
                                                u32 partial_base, double2* __restrict__ tile,
                                                double (*red)[kOwnerMaxAcc],
+                                               u64* __restrict__ step_mask,
                                                double2* __restrict__ partials) {
-  (owner_dispatch_one<K>(kind, psi, pd, B, tb, tile_blocks, partial_base, tile, red, partials), ...);
+  (owner_dispatch_one<K>(kind, psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask,
+                         partials),
+   ...);
 }
 
 // Grid: bundles * tile_blocks blocks; the blocks of one tile block are
@@ -862,6 +878,7 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
                          u32 partial_base, double2* __restrict__ partials) {
   __shared__ double2 tile[kTilePoints];
   __shared__ double red[kOwnerWaves][kOwnerMaxAcc];
+  __shared__ u64 step_mask[32];
   const u32 b = blockIdx.x;
   const u32 xcd = b & 7U;
   const u32 slot_id = b >> 3;
@@ -870,7 +887,7 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
   const OwnerBundle& B = bundles[ig];
   const int kind = static_cast<int>(__builtin_amdgcn_readfirstlane(B.kind));
   owner_dispatch(kind, std::make_integer_sequence<int, kOwnerKinds>{}, psi, pd, B, tb, tile_blocks,
-                 partial_base, tile, red, partials);
+                 partial_base, tile, red, step_mask, partials);
 }
 
 // ------------------------------------------------------------ tiled sigma
