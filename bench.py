@@ -199,9 +199,12 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     info = dop.expectation_plan_info()
-    # The sweep kernels over psi: the tiled passes plus, when the diagonal keys
-    # use it, the Walsh-Hadamard sweep; the direct kernel for small states.
-    kernels = ["expect_tiled", "expect_diag_wht"] if info["tiled"] else ["expect_direct"]
+    # The sweep kernels over psi: the tiled passes (owner sweep for off-diagonal
+    # items, item sweep for diagonal items riding in a pass) plus, when the
+    # diagonal keys use it, the Walsh-Hadamard sweep; the direct kernel for
+    # small states.
+    kernels = (["expect_owner", "expect_tiled", "expect_diag_wht"] if info["tiled"]
+               else ["expect_direct"])
     per_kernel = {k: _native.profile_get(k) for k in kernels}
     kernel_ms = sum(ms for ms, _ in per_kernel.values())
     launches = sum(n_ for _, n_ in per_kernel.values())
