@@ -1991,6 +1991,11 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
           continue;
         }
         // Greedy: the functional covering the most remaining items, repeatedly.
+        // While it covers a full bundle, only whole bundles are taken (the
+        // rest of its items may fill bundles of other layouts); the final
+        // partial bundles take everything their functional covers.
+        const std::size_t full = static_cast<std::size_t>(
+            kOwnerShapeSlots[owner_first_shape(wide, false)]);
         while (!pool.empty()) {
           u32 best = 1;
           std::size_t best_n = 0;
@@ -2002,9 +2007,11 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
               best = lam;
             }
           }
+          const std::size_t quota = best_n >= full ? best_n / full * full : best_n;
           std::vector<Cand> take, rest;
           for (const Cand& c : pool) {
-            (host_parity(best & ((c.xl >> 6) & 63U)) ? take : rest).push_back(c);
+            const bool covered = host_parity(best & ((c.xl >> 6) & 63U));
+            (covered && take.size() < quota ? take : rest).push_back(c);
           }
           emit(take, seg_layout(best), base_kind, wide);
           pool = std::move(rest);
