@@ -644,19 +644,25 @@ const std::array<TiledKernel, kVariants> kTiledKernels =
 // shape (slots, keys per slot).
 constexpr int kOwnerBlock = kTileBlock;
 constexpr int kOwnerWaves = kOwnerBlock / 64;
-constexpr int kOwnerShapes = 6;
-constexpr int kOwnerShapeSlots[kOwnerShapes] = {8, 4, 2, 4, 2, 1};
-constexpr int kOwnerShapeKeys[kOwnerShapes] = {2, 2, 2, 4, 4, 4};
+// Shapes 0..5 are register-pivot bundles (half-tile sweep), 6..9 lane-pivot
+// bundles (full-tile sweep: their partners may sit in any register, and they
+// also hold all 8 points and the representatives' selects, so they are smaller).
+constexpr int kOwnerShapes = 10;
+constexpr int kOwnerShapeSlots[kOwnerShapes] = {9, 4, 2, 4, 2, 1, 4, 2, 2, 1};
+constexpr int kOwnerShapeKeys[kOwnerShapes] = {2, 2, 2, 4, 4, 4, 2, 2, 4, 4};
 constexpr int kOwnerKinds = 4 * kOwnerShapes;
-constexpr int kOwnerMaxAcc = 16;
-// Lane-pivot slots also hold their representatives' selects: their largest
-// shapes would not fit the 128-register budget, so lane-pivot bundles stop at
-// 4 slots x 2 keys and 2 slots x 4 keys.
-constexpr int owner_first_shape(bool wide, bool lanex) { return (wide ? 3 : 0) + (lanex ? 1 : 0); }
+constexpr int kOwnerMaxAcc = 18;
+// Shapes available to a group of slots, largest first.
+constexpr int owner_first_shape(bool wide, bool lanex) {
+  return lanex ? (wide ? 8 : 6) : (wide ? 3 : 0);
+}
+constexpr int owner_last_shape(bool wide, bool lanex) {
+  return lanex ? (wide ? 9 : 7) : (wide ? 5 : 2);
+}
 constexpr bool owner_kind_used(int kind) {
   const int shape = kind >> 2;
-  const bool lanex = (kind >> 1) & 1, wide = shape >= 3;
-  return shape >= owner_first_shape(wide, lanex);
+  const bool lanex = (kind >> 1) & 1;
+  return lanex ? shape >= 6 : shape < 6;
 }
 // Owner passes give each block about kOwnerTilesPerBlock tiles: the blocks of
 // the bundles that share a tile start together and stay close, so all but the
@@ -669,6 +675,7 @@ constexpr u64 kOwnerMinBlocks = 512;
 constexpr u64 kOwnerMaxPassRows = u64{1} << 20;
 static_assert(kTilePoints / kOwnerBlock == 8 && kOwnerWaves == 8,
               "owner staging is written for 8 points per thread and 8 waves");
+static_assert(kOwnerMaxSlots == 9, "bundle slots match the largest shape");
 
 __device__ __forceinline__ double unit_sign(u32 hi) { return __hiloint2double(static_cast<int>(hi), 0); }
 
@@ -838,6 +845,134 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
   }
 }
 
+// Register-pivot bundles: representatives are registers 0..3 of every thread
+// and every partner lies in registers 4..7 (local bit 11 set), so LDS holds
+// only that half of the tile (32 KiB), double-buffered: while tile s is
+// evaluated, global_load_lds fills the other half-buffer with tile s + 1 and
+// the own half of tile s + 1 is loaded into registers. One barrier per tile.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int M, int NZ, bool kOdd>
+__device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi,
+                                                 const PassDesc* __restrict__ pd,
+                                                 const OwnerBundle& B, u32 tb, u32 tile_blocks,
+                                                 u32 partial_base, double2* __restrict__ lds,
+                                                 double (*red)[kOwnerMaxAcc],
+                                                 u64* __restrict__ step_mask,
+                                                 double2* __restrict__ partials) {
+  static_assert(M * NZ <= kOwnerMaxAcc && M <= kOwnerMaxSlots && NZ <= kOwnerMaxKeys,
+                "owner bundle shape");
+  constexpr u32 kHalf = kTilePoints / 2;
+  const u32 tid = threadIdx.x;
+  const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const u32 lane = tid & 63U;
+  const u64 free_mask = pd->free_mask;
+  const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  u64 sb_w = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if ((wave >> i) & 1U) sb_w ^= B.seg_basis[i];
+  }
+  const u64 sb_e0 = B.seg_basis[3], sb_e1 = B.seg_basis[4], sb_e2 = B.seg_basis[5];
+  const u32 tid16 = tid << 4;
+  // Partner of representative j: byte (tid << 4) ^ haddr[m] ^ (j << 13) of the
+  // half-buffer (register j ^ x[9..11] minus 4; x bit 11 is set).
+  u32 haddr[M], kw[M][NZ];
+  double acc[M][NZ];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    haddr[m] = __builtin_amdgcn_readfirstlane(B.slot[m].addr0 ^ (1U << 15));
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      kw[m][k] = __builtin_amdgcn_readfirstlane(B.slot[m].key[k].w);
+      acc[m][k] = 0.0;
+    }
+  }
+  const u64 run = ntiles / tile_blocks;
+  const int log_tb = __builtin_ctz(tile_blocks);
+  if (tid < 32U) step_mask[tid] = tile_rep(static_cast<u64>(tile_blocks) << tid, free_mask);
+  u64 rep = tile_rep(tb, free_mask);
+  double2 nx0, nx1, nx2, nx3;
+  auto issue = [&](u64 r, u32 buf) {
+    const u64 r0 = r ^ sb_w, r1 = r0 ^ sb_e0, r2 = r0 ^ sb_e1, r3 = r1 ^ sb_e1;
+    nx0 = psi[r0 | lane];
+    nx1 = psi[r1 | lane];
+    nx2 = psi[r2 | lane];
+    nx3 = psi[r3 | lane];
+    double2* half = lds + buf * kHalf + 64U * wave;
+    __builtin_amdgcn_global_load_lds(psi + ((r0 ^ sb_e2) | lane), (lds_void*)(half + 0 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r1 ^ sb_e2) | lane), (lds_void*)(half + 1 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r2 ^ sb_e2) | lane), (lds_void*)(half + 2 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r3 ^ sb_e2) | lane), (lds_void*)(half + 3 * kOwnerBlock), 16, 0, 0);
+  };
+  issue(rep, 0);
+  for (u64 step = 0; step < run; ++step) {
+    // This wave's loads of tile `step` have landed; the barrier makes every
+    // wave's half-buffer writes visible and retires tile step - 1's reads.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const double2 own0 = nx0, own1 = nx1, own2 = nx2, own3 = nx3;
+    if (step + 1 < run) {
+      const u64 msk = step_mask[__builtin_ctzll(step + 1)];
+      rep ^= (static_cast<u64>(__builtin_amdgcn_readfirstlane(static_cast<u32>(msk >> 32))) << 32) |
+             __builtin_amdgcn_readfirstlane(static_cast<u32>(msk));
+      issue(rep, static_cast<u32>(step + 1) & 1U);
+    }
+    const double2* tile = lds + (static_cast<u32>(step) & 1U) * kHalf;
+    const u32 t32 = tb | (static_cast<u32>(step ^ (step >> 1)) << log_tb);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const u32 base = tid16 ^ haddr[m];
+      double2 q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
+      const double2 a[4] = {own0, own1, own2, own3};
+      double v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = kOdd ? fma(q[j].x, a[j].y, -(q[j].y * a[j].x)) : fma(q[j].x, a[j].x, q[j].y * a[j].y);
+      }
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        const u32 w = kw[m][k];
+        const u32 h0 = 0x3FF00000U ^ (static_cast<u32>(__popc(w & t32)) << 31);
+        const u32 c1 = w & 0x80000000U, c0 = (w << 1) & 0x80000000U;
+        double sacc = acc[m][k];
+        sacc = fma(unit_sign(h0), v[0], sacc);
+        sacc = fma(unit_sign(h0 ^ c0), v[1], sacc);
+        sacc = fma(unit_sign(h0 ^ c1), v[2], sacc);
+        sacc = fma(unit_sign(h0 ^ c0 ^ c1), v[3], sacc);
+        acc[m][k] = sacc;
+      }
+    }
+  }
+  // Per-thread sign, then a fixed-order block sum per slot key.
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const OwnerKey& K = B.slot[m].key[k];
+      const u32 par = static_cast<u32>(__popc(K.zlt & tid)) & 1U;
+      double r = par ? -acc[m][k] : acc[m][k];
+      for (int d = 32; d > 0; d >>= 1) r += __shfl_xor(r, d, 64);
+      if (lane == 0) red[wave][m * NZ + k] = r;
+    }
+  }
+  __syncthreads();
+  if (tid < static_cast<u32>(M * NZ)) {
+    const int m = static_cast<int>(tid) / NZ, k = static_cast<int>(tid) % NZ;
+    const OwnerSlot& S = B.slot[m];
+    const u32 info = S.key[k].info;
+    if ((info >> 15) & 1U) {
+      double r = 0.0;
+      for (int w = 0; w < kOwnerWaves; ++w) r += red[w][tid];
+      const std::size_t row = static_cast<std::size_t>(partial_base) +
+                              static_cast<std::size_t>(S.item) * tile_blocks + tb;
+      partials[row * kZ + ((info >> 4) & 15U)] = make_double2(r, r);
+    }
+  }
+}
+
 template <int K>
 __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __restrict__ psi,
                                                    const PassDesc* __restrict__ pd,
@@ -849,9 +984,13 @@ __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __re
   if constexpr (owner_kind_used(K)) {
     if (kind != K) return;
     constexpr int shape = K >> 2;
-    owner_bundle_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], ((K >> 1) & 1) != 0,
-                       (K & 1) != 0>(psi, pd, B, tb, tile_blocks, partial_base, tile, red,
-                                     step_mask, partials);
+    if constexpr ((K >> 1) & 1) {
+      owner_bundle_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], true, (K & 1) != 0>(
+          psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask, partials);
+    } else {
+      owner_half_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], (K & 1) != 0>(
+          psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask, partials);
+    }
   }
 }
 
@@ -1924,8 +2063,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
     // smallest shape that holds the rest.
     auto emit = [&](const std::vector<Cand>& group, const SegLayout& L, int base_kind, bool wide) {
       const int first_shape = owner_first_shape(wide, (base_kind >> 1) & 1);
-      const int last_shape = wide ? 5 : 2;
-      static_assert(kOwnerShapes == 6, "three narrow and three wide bundle shapes");
+      const int last_shape = owner_last_shape(wide, (base_kind >> 1) & 1);
       for (std::size_t s = 0; s < group.size();) {
         const std::size_t left = group.size() - s;
         int shape = first_shape;

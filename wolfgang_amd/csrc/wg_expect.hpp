@@ -64,7 +64,7 @@ struct PassDesc {
 // Each bundle may change the basis of the 6 segment coordinates (local bits
 // 6..11) so that every item of a register-pivot bundle has local bit 11 set;
 // items inside the low 6 bits use a lane pivot.
-constexpr int kOwnerMaxSlots = 8;
+constexpr int kOwnerMaxSlots = 9;
 constexpr int kOwnerMaxKeys = 4;
 
 struct OwnerKey {
