@@ -624,14 +624,17 @@ const std::array<TiledKernel, kVariants> kTiledKernels =
 
 // Off-diagonal items of a pass in "bundles" of item slots (2 or 4 keys per
 // slot); one launch per pass runs all its bundles, so the bundles that share
-// a tile read it from L2. A 512-thread block stages a tile and thread tid
-// keeps its points u = tid + 512 i (i < 8) in registers; for every slot it
-// evaluates the 4 pairs whose representative it owns, with one LDS read (the
-// partner) per pair:
-//  * register pivot (local bit 11 of x set): representatives are registers
-//    i = j < 4, partners (tid ^ x[0..8], j ^ x[9..11]);
+// a tile read it from L2. A 512-thread block owns one bundle; thread tid keeps
+// its points u = tid + 512 i (i < 8) of each tile in registers and, for every
+// slot, evaluates the 4 pairs whose representative it owns, with one LDS read
+// (the partner) per pair:
+//  * register pivot (local bit 11 of x set in the bundle's layout):
+//    representatives are registers i = j < 4 and partners
+//    (tid ^ x[0..8], j ^ x[9..11]) lie in registers 4..7, so only that half
+//    of the tile goes to LDS (owner_half_sweep);
 //  * lane pivot (x inside local bits 0..5): representatives are registers
-//    i = j + 4 h, h = bit `hbit` of tid, partners (tid ^ x, i).
+//    i = j + 4 h, h = bit `hbit` of tid, partners (tid ^ x, i); the whole tile
+//    is staged (owner_lane_sweep).
 // For representative j and key k the pair's sign is (-1)^{parity(z & rep(t))}
 // (-1)^{c_k . j} times a per-thread sign applied once at the end (c_k: z's
 // local bits 9, 10). Each key needs one component of conj(psi[b ^ x]) psi[b]
@@ -679,8 +682,10 @@ static_assert(kOwnerMaxSlots == 9, "bundle slots match the largest shape");
 
 __device__ __forceinline__ double unit_sign(u32 hi) { return __hiloint2double(static_cast<int>(hi), 0); }
 
-template <int M, int NZ, bool kLaneX, bool kOdd>
-__device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ psi,
+// Lane-pivot bundles: the whole tile is staged in LDS (partners may lie in
+// either half), the next tile prefetched into registers; two barriers per tile.
+template <int M, int NZ, bool kOdd>
+__device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi,
                                                    const PassDesc* __restrict__ pd,
                                                    const OwnerBundle& B, u32 tb, u32 tile_blocks,
                                                    u32 partial_base, double2* __restrict__ tile,
@@ -702,12 +707,12 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
   const u64 sb_e0 = B.seg_basis[3], sb_e1 = B.seg_basis[4], sb_e2 = B.seg_basis[5];
   const u32 tid16 = tid << 4;
   u32 addr0[M], kw[M][NZ];
-  u32 hmask = 0;  // lane pivot: bit m = bit hbit_m of tid
+  u32 hmask = 0;  // bit m = bit hbit_m of tid
   double acc[M][NZ];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     addr0[m] = __builtin_amdgcn_readfirstlane(B.slot[m].addr0);
-    if constexpr (kLaneX) hmask |= ((tid >> B.slot[m].hbit) & 1U) << m;
+    hmask |= ((tid >> B.slot[m].hbit) & 1U) << m;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       kw[m][k] = __builtin_amdgcn_readfirstlane(B.slot[m].key[k].w);
@@ -757,46 +762,14 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
              __builtin_amdgcn_readfirstlane(static_cast<u32>(m));
       WG_OWNER_FETCH(rep);
     }
-    // Register-pivot slots: slot m's partners are read while slot m - 1 is
-    // evaluated (two partner sets in flight; lane-pivot slots also hold all 8
-    // points and read just in time). The partner base is formed inside the
-    // loop so the compiler keeps one address per slot instead of hoisting four.
-    constexpr bool kPipe = !kLaneX;
-    double2 p[2][4];
-    auto fetch = [&](int m, double2 (&q)[4]) {
-      u32 base;
-      asm volatile("v_xor_b32 %0, %1, %2" : "=v"(base) : "s"(addr0[m]), "v"(tid16));
-      if constexpr (kLaneX) {
-        base |= ((hmask >> m) & 1U) ? (4U << 13) : 0U;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
-      }
-    };
-    if constexpr (kPipe) fetch(0, p[0]);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      if constexpr (kPipe) {
-        if (m + 1 < M) fetch(m + 1, p[(m + 1) & 1]);
-      } else {
-        fetch(m, p[m & 1]);
-      }
-      const double2(&q)[4] = p[m & 1];
-      double2 a[4];
-      if constexpr (kLaneX) {
-        const bool h = (hmask >> m) & 1U;
-        a[0] = h ? own4 : own0;
-        a[1] = h ? own5 : own1;
-        a[2] = h ? own6 : own2;
-        a[3] = h ? own7 : own3;
-      } else {
-        a[0] = own0;
-        a[1] = own1;
-        a[2] = own2;
-        a[3] = own3;
-      }
+      const bool h = (hmask >> m) & 1U;
+      const double2 a[4] = {h ? own4 : own0, h ? own5 : own1, h ? own6 : own2, h ? own7 : own3};
+      const u32 base = (tid16 ^ addr0[m]) | (h ? (4U << 13) : 0U);
+      double2 q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
       double v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -824,7 +797,7 @@ __device__ __forceinline__ void owner_bundle_sweep(const double2* __restrict__ p
     for (int k = 0; k < NZ; ++k) {
       const OwnerKey& K = B.slot[m].key[k];
       u32 par = static_cast<u32>(__popc(K.zlt & tid)) & 1U;
-      if constexpr (kLaneX) par ^= (K.info >> 2) & (hmask >> m) & 1U;
+      par ^= (K.info >> 2) & (hmask >> m) & 1U;
       double r = par ? -acc[m][k] : acc[m][k];
       for (int d = 32; d > 0; d >>= 1) r += __shfl_xor(r, d, 64);
       if (lane == 0) red[wave][m * NZ + k] = r;
@@ -985,7 +958,7 @@ __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __re
     if (kind != K) return;
     constexpr int shape = K >> 2;
     if constexpr ((K >> 1) & 1) {
-      owner_bundle_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], true, (K & 1) != 0>(
+      owner_lane_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], (K & 1) != 0>(
           psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask, partials);
     } else {
       owner_half_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], (K & 1) != 0>(
