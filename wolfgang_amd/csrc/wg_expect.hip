@@ -68,8 +68,6 @@ std::size_t max_tiled_passes(std::size_t n, std::size_t groups) {
 }
 // Diagonal keys use the Walsh-Hadamard sweep beyond this many items' worth.
 constexpr int kDiagWhtMinItems = 4;
-// Grid of the tiled sigma launches.
-constexpr u64 kSigmaBlocks = 2048;
 constexpr u32 kDiagWhtBlocks = 1024;
 // Off-diagonal items of tiled passes use the owner sweep (one LDS read per
 // amplitude pair) instead of the per-wave item sweep.
@@ -1007,133 +1005,207 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
 // sigma = H psi (and H^dagger psi) over the tiled passes: for output o and
 // x-group g, (P_k psi)[o] = i^{y_k} (-1)^{|z_k & (o ^ x)|} psi[o ^ x], so
 // sigma[o] = sum_g psi[o ^ x_g] * sum_{k in g} cy_k (-1)^{|z_k & (o ^ x_g)|} with
-// cy_k = c_k i^{y_k} (H^dagger: conj(c_k) i^{y_k}). A block stages a tile and,
-// for every item of the pass, each thread adds to its 8 staged points'
-// accumulators one partner read from LDS times the item's signed coefficient
-// sum; the accumulators are then added into sigma in HBM (one read-modify-write
-// per point per pass, sigma cleared before the first pass). For the thread's
-// point u = tid + 512 i the sign of key j splits into a lane part
-// parity(zl_j & tid), the uniform parts parity(zl_j & 512 i) and
-// parity(zl_j & xl), and the tile part parity(z_j & rep).
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+// cy_k = c_k i^{y_k} (H^dagger: conj(c_k) i^{y_k}).
+//
+// One 1024-thread block per CU walks its tiles in Gray-code order (as the
+// owner sweep). The tile is double-buffered in LDS (2 x 64 KiB): global_load_lds
+// fills the other buffer with the next tile while this one is evaluated, so a
+// tile costs one barrier. Thread tid owns the points u = tid + 1024 I (I < 4),
+// which are segment wave + 16 I, lane `lane`: the same addresses it copies into
+// LDS, reads sigma from and writes sigma back to (one read-modify-write per
+// point per pass; the first launched pass writes without reading). The current
+// sigma values are loaded at the start of a tile and the sums are stored after
+// the next tile's barrier, so neither latency sits in front of a barrier.
+//
+// Keys are a flat list (SigmaKey), read through scalar loads with the next
+// record loaded ahead. The sign of key j at point u is
+// parity(z_j & rep) ^ parity(zl_j & (u ^ xl)), i.e. a uniform part, a lane part
+// parity(zl_j & tid) and the register part parity(((zl_j >> 10) & 3) & I).
+constexpr int kSigmaBlock = 1024;
+constexpr int kSigmaPoints = kTilePoints / kSigmaBlock;
+static_assert(kSigmaPoints == 4 && (kSigmaBlock >> kTileLowLog) == 16,
+              "sigma sweep is written for 4 points per thread and 16 waves");
+
+__device__ __forceinline__ double flip_sign(double v, u32 m) {
+  return __hiloint2double(__double2hiint(v) ^ static_cast<int>(m), __double2loint(v));
 }
 
-template <bool kDagger>
-__global__ __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4)))
-void sigma_tiled_kernel(const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
-                        const ExpItem* __restrict__ items, u32 num_items,
-                        const double2* __restrict__ cy, const double2* __restrict__ cy_dag,
-                        double2* __restrict__ sigma, double2* __restrict__ sigma_dag) {
-  __shared__ double2 tile[kTilePoints];
+// c += s * k for a uniform sign s (hi word 0x3FF00000 or 0xBFF00000).
+__device__ __forceinline__ void add_signed(double2& c, u32 s_hi, double2 k) {
+  const double s = unit_sign(s_hi);
+  c.x = fma(s, k.x, c.x);
+  c.y = fma(s, k.y, c.y);
+}
+
+// a += c * b (complex).
+__device__ __forceinline__ void cmac(double2& a, double2 c, double2 b) {
+  a.x = fma(c.x, b.x, a.x);
+  a.x = fma(-c.y, b.y, a.x);
+  a.y = fma(c.x, b.y, a.y);
+  a.y = fma(c.y, b.x, a.y);
+}
+
+template <bool kPair>
+__global__ __launch_bounds__(kSigmaBlock) void sigma_tiled_kernel(
+    const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
+    const SigmaKey* __restrict__ keys, u32 key_begin, u32 key_end, u32 log_blocks, u32 init,
+    double2* __restrict__ out, double2* __restrict__ out_dag) {
+  __shared__ double2 tile[2 * kTilePoints];
+  __shared__ u64 step_mask[32];
   const u32 tid = threadIdx.x;
   const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const u32 lane = tid & 63U;
   const u64 free_mask = pd->free_mask;
-  const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
-  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[8], sb_e1 = pd->seg_off[16],
-            sb_e2 = pd->seg_off[32];
-  const u32 lo = tid & ((1U << kTileLowLog) - 1U);
-  // Point i of this thread in tile rep: (rep ^ seg_off[wave + 8 i]) | lo, the
-  // segment offset being linear in the segment index. Named registers (not
-  // arrays) keep the eight points out of scratch memory.
-#define WG_G(rep, I)                                                                  \
-  ((((rep) ^ sb_w ^ (((I) & 1) ? sb_e0 : 0) ^ (((I) & 2) ? sb_e1 : 0) ^ (((I) & 4) ? sb_e2 : 0))) | lo)
-  double2 st0, st1, st2, st3, st4, st5, st6, st7;
-  u64 t = blockIdx.x;
-  if (t < ntiles) {
-    const u64 rep0 = tile_rep(t, free_mask);
-    st0 = psi[WG_G(rep0, 0)];
-    st1 = psi[WG_G(rep0, 1)];
-    st2 = psi[WG_G(rep0, 2)];
-    st3 = psi[WG_G(rep0, 3)];
-    st4 = psi[WG_G(rep0, 4)];
-    st5 = psi[WG_G(rep0, 5)];
-    st6 = psi[WG_G(rep0, 6)];
-    st7 = psi[WG_G(rep0, 7)];
-  }
-  for (; t < ntiles; t += gridDim.x) {
+  const u64 run = (1ULL << (pd->n - kTileMaxLog)) >> log_blocks;
+  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[16], sb_e1 = pd->seg_off[32];
+  if (tid < 32U) step_mask[tid] = tile_rep(1ULL << (tid + log_blocks), free_mask);
+  u64 rep = tile_rep(blockIdx.x, free_mask);
+  // Global index of point I of this thread in the tile with representative r.
+#define WG_PT(r, I) \
+  ((((r) ^ sb_w ^ (((I) & 1) ? sb_e0 : 0) ^ (((I) & 2) ? sb_e1 : 0))) | lane)
+  auto issue = [&](u64 r, u32 buf) {
+    double2* seg = tile + buf * kTilePoints + 64U * wave;
+    __builtin_amdgcn_global_load_lds(psi + WG_PT(r, 0), (lds_void*)(seg + 0 * kSigmaBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + WG_PT(r, 1), (lds_void*)(seg + 1 * kSigmaBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + WG_PT(r, 2), (lds_void*)(seg + 2 * kSigmaBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + WG_PT(r, 3), (lds_void*)(seg + 3 * kSigmaBlock), 16, 0, 0);
+  };
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 o0 = zero, o1 = zero, o2 = zero, o3 = zero;  // sums waiting to be stored
+  double2 q0 = zero, q1 = zero, q2 = zero, q3 = zero;  // sigma^dagger sums
+  u64 prev = 0;
+  issue(rep, 0);
+  for (u64 step = 0; step < run; ++step) {
+    // This wave's copies of tile `step` have landed; the barrier makes every
+    // wave's copies visible and retires the reads of the other buffer.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    tile[tid + 0 * kTileBlock] = st0;
-    tile[tid + 1 * kTileBlock] = st1;
-    tile[tid + 2 * kTileBlock] = st2;
-    tile[tid + 3 * kTileBlock] = st3;
-    tile[tid + 4 * kTileBlock] = st4;
-    tile[tid + 5 * kTileBlock] = st5;
-    tile[tid + 6 * kTileBlock] = st6;
-    tile[tid + 7 * kTileBlock] = st7;
-    __syncthreads();
-    const u64 rep = tile_rep(t, free_mask);
-    const u64 tn = t + gridDim.x;
-    if (tn < ntiles) {
-      const u64 repn = tile_rep(tn, free_mask);
-      st0 = psi[WG_G(repn, 0)];
-      st1 = psi[WG_G(repn, 1)];
-      st2 = psi[WG_G(repn, 2)];
-      st3 = psi[WG_G(repn, 3)];
-      st4 = psi[WG_G(repn, 4)];
-      st5 = psi[WG_G(repn, 5)];
-      st6 = psi[WG_G(repn, 6)];
-      st7 = psi[WG_G(repn, 7)];
-    }
-    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0), a2 = make_double2(0.0, 0.0), a3 = make_double2(0.0, 0.0), a4 = make_double2(0.0, 0.0), a5 = make_double2(0.0, 0.0), a6 = make_double2(0.0, 0.0), a7 = make_double2(0.0, 0.0);
-    double2 d0 = make_double2(0.0, 0.0), d1 = make_double2(0.0, 0.0), d2 = make_double2(0.0, 0.0), d3 = make_double2(0.0, 0.0), d4 = make_double2(0.0, 0.0), d5 = make_double2(0.0, 0.0), d6 = make_double2(0.0, 0.0), d7 = make_double2(0.0, 0.0);
-    for (u32 m = 0; m < num_items; ++m) {
-      const ExpItem& it = items[m];
-      const u32 xl = static_cast<u32>(it.xl);
-      double2 c0 = make_double2(0.0, 0.0), c1 = make_double2(0.0, 0.0), c2 = make_double2(0.0, 0.0), c3 = make_double2(0.0, 0.0), c4 = make_double2(0.0, 0.0), c5 = make_double2(0.0, 0.0), c6 = make_double2(0.0, 0.0), c7 = make_double2(0.0, 0.0);
-      double2 e0 = make_double2(0.0, 0.0), e1 = make_double2(0.0, 0.0), e2 = make_double2(0.0, 0.0), e3 = make_double2(0.0, 0.0), e4 = make_double2(0.0, 0.0), e5 = make_double2(0.0, 0.0), e6 = make_double2(0.0, 0.0), e7 = make_double2(0.0, 0.0);
-      for (int j = 0; j < it.nz; ++j) {
-        const u32 zlj = static_cast<u32>(it.zl[j]);
-        const u32 base = static_cast<u32>((__popc(zlj & xl) + __popcll(it.z[j] & rep)) & 1);
-        const u32 lanep = static_cast<u32>(__popc(zlj & tid) & 1) ^ base;
-        const double2 k = cy[it.key[j]];
-        const double2 kd = kDagger ? cy_dag[it.key[j]] : make_double2(0.0, 0.0);
-#define WG_ADD(I)                                                                       \
-  {                                                                                     \
-    const u32 neg = lanep ^ static_cast<u32>(__popc(zlj & static_cast<u32>((I) * kTileBlock)) & 1); \
-    c##I.x += neg ? -k.x : k.x;                                                          \
-    c##I.y += neg ? -k.y : k.y;                                                          \
-    if (kDagger) {                                                                      \
-      e##I.x += neg ? -kd.x : kd.x;                                                      \
-      e##I.y += neg ? -kd.y : kd.y;                                                      \
-    }                                                                                   \
-  }
-        WG_ADD(0) WG_ADD(1) WG_ADD(2) WG_ADD(3) WG_ADD(4) WG_ADD(5) WG_ADD(6) WG_ADD(7)
-#undef WG_ADD
+    if (step > 0) {
+      out[WG_PT(prev, 0)] = o0;
+      out[WG_PT(prev, 1)] = o1;
+      out[WG_PT(prev, 2)] = o2;
+      out[WG_PT(prev, 3)] = o3;
+      if (kPair) {
+        out_dag[WG_PT(prev, 0)] = q0;
+        out_dag[WG_PT(prev, 1)] = q1;
+        out_dag[WG_PT(prev, 2)] = q2;
+        out_dag[WG_PT(prev, 3)] = q3;
       }
-#define WG_MAC(I)                                                                       \
-  {                                                                                     \
-    const double2 b = tile[(tid + (I) * kTileBlock) ^ xl];                              \
-    const double2 pr = cmul(c##I, b);                                                   \
-    a##I.x += pr.x;                                                                     \
-    a##I.y += pr.y;                                                                     \
-    if (kDagger) {                                                                      \
-      const double2 pq = cmul(e##I, b);                                                 \
-      d##I.x += pq.x;                                                                   \
-      d##I.y += pq.y;                                                                   \
-    }                                                                                   \
-  }
-      WG_MAC(0) WG_MAC(1) WG_MAC(2) WG_MAC(3) WG_MAC(4) WG_MAC(5) WG_MAC(6) WG_MAC(7)
-#undef WG_MAC
     }
-#define WG_OUT(I)                                                                       \
-  {                                                                                     \
-    const u64 g = WG_G(rep, I);                                                         \
-    double2 v = sigma[g];                                                               \
-    v.x += a##I.x;                                                                      \
-    v.y += a##I.y;                                                                      \
-    sigma[g] = v;                                                                       \
-    if (kDagger) {                                                                      \
-      double2 w = sigma_dag[g];                                                         \
-      w.x += d##I.x;                                                                    \
-      w.y += d##I.y;                                                                    \
-      sigma_dag[g] = w;                                                                 \
-    }                                                                                   \
+    const u64 cur = rep;
+    double2 s0 = zero, s1 = zero, s2 = zero, s3 = zero;
+    double2 t0 = zero, t1 = zero, t2 = zero, t3 = zero;
+    if (!init) {
+      s0 = out[WG_PT(cur, 0)];
+      s1 = out[WG_PT(cur, 1)];
+      s2 = out[WG_PT(cur, 2)];
+      s3 = out[WG_PT(cur, 3)];
+      if (kPair) {
+        t0 = out_dag[WG_PT(cur, 0)];
+        t1 = out_dag[WG_PT(cur, 1)];
+        t2 = out_dag[WG_PT(cur, 2)];
+        t3 = out_dag[WG_PT(cur, 3)];
+      }
+    }
+    if (step + 1 < run) {
+      const u64 msk = step_mask[__builtin_ctzll(step + 1)];
+      rep ^= (static_cast<u64>(__builtin_amdgcn_readfirstlane(static_cast<u32>(msk >> 32))) << 32) |
+             __builtin_amdgcn_readfirstlane(static_cast<u32>(msk));
+      issue(rep, static_cast<u32>(step + 1) & 1U);
+    }
+    const double2* tl = tile + (static_cast<u32>(step) & 1U) * kTilePoints;
+    double2 a0 = zero, a1 = zero, a2 = zero, a3 = zero;
+    double2 d0 = zero, d1 = zero, d2 = zero, d3 = zero;
+    double2 c0 = zero, c1 = zero, c2 = zero, c3 = zero;
+    double2 e0 = zero, e1 = zero, e2 = zero, e3 = zero;
+    SigmaKey nk = keys[key_begin];
+    for (u32 r = key_begin; r < key_end; ++r) {
+      const SigmaKey k = nk;
+      nk = keys[r + 1];  // the array ends with a zero record
+      const u32 xl = k.xl & ~kSigmaLast;
+      const u32 s = static_cast<u32>(__popcll(k.z & cur) + __popc(k.zl & xl)) & 1U;
+      const u32 m = ((static_cast<u32>(__popc(k.zl & tid)) ^ s) & 1U) << 31;
+      const u32 h = (k.zl >> 10) & 3U;
+      const u32 s1h = 0x3FF00000U ^ ((h & 1U) << 31);
+      const u32 s2h = 0x3FF00000U ^ ((h & 2U) << 30);
+      const u32 s3h = s1h ^ s2h ^ 0x3FF00000U;
+      const double2 kp = make_double2(flip_sign(k.c.x, m), flip_sign(k.c.y, m));
+      c0.x += kp.x;
+      c0.y += kp.y;
+      add_signed(c1, s1h, kp);
+      add_signed(c2, s2h, kp);
+      add_signed(c3, s3h, kp);
+      if (kPair) {
+        const double2 kq = make_double2(flip_sign(k.cd.x, m), flip_sign(k.cd.y, m));
+        e0.x += kq.x;
+        e0.y += kq.y;
+        add_signed(e1, s1h, kq);
+        add_signed(e2, s2h, kq);
+        add_signed(e3, s3h, kq);
+      }
+      if (k.xl & kSigmaLast) {
+        const double2 b0 = tl[(tid + 0 * kSigmaBlock) ^ xl];
+        const double2 b1 = tl[(tid + 1 * kSigmaBlock) ^ xl];
+        const double2 b2 = tl[(tid + 2 * kSigmaBlock) ^ xl];
+        const double2 b3 = tl[(tid + 3 * kSigmaBlock) ^ xl];
+        cmac(a0, c0, b0);
+        cmac(a1, c1, b1);
+        cmac(a2, c2, b2);
+        cmac(a3, c3, b3);
+        c0 = c1 = c2 = c3 = zero;
+        if (kPair) {
+          cmac(d0, e0, b0);
+          cmac(d1, e1, b1);
+          cmac(d2, e2, b2);
+          cmac(d3, e3, b3);
+          e0 = e1 = e2 = e3 = zero;
+        }
+      }
+    }
+    o0 = make_double2(s0.x + a0.x, s0.y + a0.y);
+    o1 = make_double2(s1.x + a1.x, s1.y + a1.y);
+    o2 = make_double2(s2.x + a2.x, s2.y + a2.y);
+    o3 = make_double2(s3.x + a3.x, s3.y + a3.y);
+    if (kPair) {
+      q0 = make_double2(t0.x + d0.x, t0.y + d0.y);
+      q1 = make_double2(t1.x + d1.x, t1.y + d1.y);
+      q2 = make_double2(t2.x + d2.x, t2.y + d2.y);
+      q3 = make_double2(t3.x + d3.x, t3.y + d3.y);
+    }
+    prev = cur;
   }
-    WG_OUT(0) WG_OUT(1) WG_OUT(2) WG_OUT(3) WG_OUT(4) WG_OUT(5) WG_OUT(6) WG_OUT(7)
-#undef WG_OUT
+  if (run > 0) {
+    out[WG_PT(prev, 0)] = o0;
+    out[WG_PT(prev, 1)] = o1;
+    out[WG_PT(prev, 2)] = o2;
+    out[WG_PT(prev, 3)] = o3;
+    if (kPair) {
+      out_dag[WG_PT(prev, 0)] = q0;
+      out_dag[WG_PT(prev, 1)] = q1;
+      out_dag[WG_PT(prev, 2)] = q2;
+      out_dag[WG_PT(prev, 3)] = q3;
+    }
   }
-#undef WG_G
+#undef WG_PT
+}
+
+// Per-call sigma key records: the plan's records with c = cy[key] (or
+// cy_dag[key] when only sigma^dagger is wanted) and cd = cy_dag[key]; the
+// closing zero record is copied as well.
+__global__ __launch_bounds__(256) void sigma_keys_kernel(const SigmaKey* __restrict__ plan_keys,
+                                                         std::size_t count,
+                                                         const double2* __restrict__ cy,
+                                                         const double2* __restrict__ cy_dag,
+                                                         int dagger_only,
+                                                         SigmaKey* __restrict__ keys) {
+  const std::size_t r = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r > count) return;
+  SigmaKey k = plan_keys[r];
+  if (r < count) {
+    k.c = dagger_only ? cy_dag[k.key] : cy[k.key];
+    k.cd = cy_dag[k.key];
+  }
+  keys[r] = k;
 }
 
 // cy[id] = c_key * i^{y_key}, cy_dag[id] = conj(c_key) * i^{y_key}, by distinct-key id.
@@ -2163,6 +2235,7 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_flat_key);
   device_free(d_key_terms);
   device_free(d_bundles);
+  device_free(d_sigma_keys);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
 
@@ -2281,6 +2354,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   plan->h_term_key = term_key;
   plan->tiled = plan->n >= kTiledMinQubits && !items.empty();
   std::vector<u32> offset, count;
+  std::vector<SigmaKey> sigma_keys;
   if (plan->tiled) {
     std::vector<ExpItem> tiled_items = items;
     std::vector<PassLaunch> passes;
@@ -2326,6 +2400,34 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     }
     if (slot > 0xFFFFFFFFULL) fail_runtime("tiled expectation plan has too many partial rows");
     plan->partial_slots = slot;
+    // Sigma key records, pass by pass (pass 0 from item 0: it also carries
+    // the diagonal items), then the zero record the sweep reads ahead.
+    for (std::size_t p = 0; p < plan->passes.size(); ++p) {
+      PassLaunch& pl = plan->passes[p];
+      const u32 first = p == 0 ? 0 : pl.first_item;
+      pl.sigma_begin = static_cast<u32>(sigma_keys.size());
+      for (u32 i = first; i < pl.first_item + pl.num_items; ++i) {
+        const ExpItem& it = items[i];
+        for (int j = 0; j < it.nz; ++j) {
+          SigmaKey k{};
+          k.z = it.z[j];
+          k.zl = static_cast<u32>(it.zl[j]);
+          k.xl = static_cast<u32>(it.xl) | (j + 1 == it.nz ? kSigmaLast : 0U);
+          k.key = it.key[j];
+          sigma_keys.push_back(k);
+        }
+      }
+      pl.sigma_end = static_cast<u32>(sigma_keys.size());
+    }
+    plan->sigma_keys = sigma_keys.size();
+    sigma_keys.push_back(SigmaKey{});
+    int cus = 0;
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, op->device),
+              "query compute units");
+    const u64 ntiles = u64{1} << (plan->n - kTileMaxLog);
+    u32 lb = 0;
+    while ((u64{2} << lb) <= std::min<u64>(ntiles, static_cast<u64>(std::max(cus, 1)))) ++lb;
+    plan->sigma_log_blocks = lb;
   } else {
     const u64 range = plan->n == 0 ? 1 : (u64{1} << (plan->n - 1));
     u64 bpi = (range + kBlock * 64 - 1) / (kBlock * 64);
@@ -2369,6 +2471,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   }
   upload(plan->d_key_terms, key_terms);
   upload(plan->d_bundles, plan->bundles);
+  upload(plan->d_sigma_keys, sigma_keys);
   sync(stream);
   return plan;
 }
@@ -2541,31 +2644,36 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
       key_cy_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
           key_c, plan->d_key_y, plan->d_flat_key, plan->num_keys, cy, cy_dag);
       check_launch("key_cy");
-      if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
-      if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
-      const u64 ntiles = u64{1} << (plan->n - kTileMaxLog);
-      const unsigned grid = static_cast<unsigned>(std::min<u64>(ntiles, kSigmaBlocks));
+      // Per-call key records, then one sweep per pass: sigma and sigma^dagger
+      // together when both are wanted. The first launched pass writes sigma
+      // without reading it.
+      SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys + 1);
+      sigma_keys_kernel<<<grid_for(plan->sigma_keys + 1, 256), 256, 0, stream>>>(
+          plan->d_sigma_keys, plan->sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
+      check_launch("sigma_keys");
+      double2* out = reinterpret_cast<double2*>(sigma ? sigma : sigma_dagger);
+      double2* out_dag = sigma && sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger) : nullptr;
+      const unsigned grid = 1U << plan->sigma_log_blocks;
+      u32 init = 1;
       for (std::size_t p = 0; p < plan->passes.size(); ++p) {
         const PassLaunch& pl = plan->passes[p];
-        // Pass 0 also carries the diagonal items swept separately in the forward.
-        const u32 first = p == 0 ? 0 : pl.first_item;
-        const u32 count = pl.first_item + pl.num_items - first;
-        if (count == 0) continue;
+        if (pl.sigma_end == pl.sigma_begin) continue;
         ProfileScope prof("sigma_tiled", stream);
-        // sigma and sigma^dagger (the same sweep with conj(c)) are separate
-        // launches: both accumulator sets would not fit the register budget.
-        if (sigma) {
-          sigma_tiled_kernel<false><<<grid, kTileBlock, 0, stream>>>(
-              dpsi, plan->d_passes + p, plan->d_items + first, count, cy, nullptr,
-              reinterpret_cast<double2*>(sigma), nullptr);
-          check_launch("sigma_tiled");
+        if (out_dag) {
+          sigma_tiled_kernel<true><<<grid, kSigmaBlock, 0, stream>>>(
+              dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
+              init, out, out_dag);
+        } else {
+          sigma_tiled_kernel<false><<<grid, kSigmaBlock, 0, stream>>>(
+              dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
+              init, out, nullptr);
         }
-        if (sigma_dagger) {
-          sigma_tiled_kernel<false><<<grid, kTileBlock, 0, stream>>>(
-              dpsi, plan->d_passes + p, plan->d_items + first, count, cy_dag, nullptr,
-              reinterpret_cast<double2*>(sigma_dagger), nullptr);
-          check_launch("sigma_tiled");
-        }
+        check_launch("sigma_tiled");
+        init = 0;
+      }
+      if (init) {
+        if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
+        if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
       }
     } else if (sigma || sigma_dagger) {
       // One fused sweep: sigma, sigma^dagger and the per-key values.

@@ -93,6 +93,21 @@ struct OwnerBundle {
   OwnerSlot slot[kOwnerMaxSlots];
 };
 
+// One key of the tiled sigma sweep, in the pass's item order: the keys of an
+// item are consecutive and the last one carries kSigmaLast. The plan holds
+// z, zl, xl and key; every call writes c and cd from its coefficients.
+constexpr u32 kSigmaLast = 1U << 31;
+
+struct SigmaKey {
+  u64 z;       // global z mask (tile part of the sign: parity(z & rep))
+  u32 zl;      // z in tile-local coordinates
+  u32 xl;      // the item's x in tile-local coordinates | kSigmaLast on its last key
+  u32 key;     // distinct-key id
+  u32 pad[3];
+  double2 c;   // c_key i^{y_key} (sigma), per call
+  double2 cd;  // conj(c_key) i^{y_key} (sigma^dagger), per call
+};
+
 // Items of one sweep variant inside a pass: one launch.
 struct SubLaunch {
   int variant;
@@ -111,6 +126,8 @@ struct PassLaunch {
   std::vector<SubLaunch> subs;
   u32 owner_first = 0;  // owner bundles of the pass (plan-wide indices), one launch
   u32 owner_count = 0;
+  u32 sigma_begin = 0;  // the pass's sigma keys [sigma_begin, sigma_end); pass 0 also
+  u32 sigma_end = 0;    // holds the diagonal items the forward sweeps separately
 };
 
 struct ExpectationPlan {
@@ -126,6 +143,9 @@ struct ExpectationPlan {
   std::vector<PassLaunch> passes;    // tiled path
   std::vector<OwnerBundle> bundles;  // owner sweep bundles of all passes
   OwnerBundle* d_bundles = nullptr;
+  u32 sigma_log_blocks = 0;          // tiled sigma grid: 2^sigma_log_blocks blocks (<= CUs)
+  std::size_t sigma_keys = 0;        // tiled sigma key records (the device array holds
+  SigmaKey* d_sigma_keys = nullptr;  // one zero record more, read ahead by the sweep)
   u32 diag_items = 0;                // tiled path: leading diagonal items swept by the
                                      // Walsh-Hadamard kernel (0: they ride in pass 0)
   u32 diag_keys = 0;
