@@ -1017,12 +1017,13 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
 // sigma values are loaded at the start of a tile and the sums are stored after
 // the next tile's barrier, so neither latency sits in front of a barrier.
 //
-// Keys are a flat list (SigmaKey), read through scalar loads with the next
-// record loaded ahead. The sign of key j at point u is
+// Keys are a flat list (SigmaKey), read through scalar loads a chunk of
+// records at a time. The sign of key j at point u is
 // parity(z_j & rep) ^ parity(zl_j & (u ^ xl)), i.e. a uniform part, a lane part
 // parity(zl_j & tid) and the register part parity(((zl_j >> 10) & 3) & I).
 constexpr int kSigmaBlock = 1024;
 constexpr int kSigmaPoints = kTilePoints / kSigmaBlock;
+
 static_assert(kSigmaPoints == 4 && (kSigmaBlock >> kTileLowLog) == 16,
               "sigma sweep is written for 4 points per thread and 16 waves");
 
@@ -1043,6 +1044,13 @@ __device__ __forceinline__ void cmac(double2& a, double2 c, double2 b) {
   a.x = fma(-c.y, b.y, a.x);
   a.y = fma(c.x, b.y, a.y);
   a.y = fma(c.y, b.x, a.y);
+}
+
+template <bool kPair>
+__device__ __forceinline__ void hold_key(const SigmaKey& k) {
+  asm volatile("" ::"s"(k.z), "s"(k.zl), "s"(k.xl), "s"(k.sh[0]), "s"(k.sh[1]), "s"(k.sh[2]),
+               "s"(k.c.x), "s"(k.c.y));
+  if (kPair) asm volatile("" ::"s"(k.cd.x), "s"(k.cd.y));
 }
 
 template <bool kPair>
@@ -1118,17 +1126,13 @@ __global__ __launch_bounds__(kSigmaBlock) void sigma_tiled_kernel(
     double2 d0 = zero, d1 = zero, d2 = zero, d3 = zero;
     double2 c0 = zero, c1 = zero, c2 = zero, c3 = zero;
     double2 e0 = zero, e1 = zero, e2 = zero, e3 = zero;
-    SigmaKey nk = keys[key_begin];
-    for (u32 r = key_begin; r < key_end; ++r) {
-      const SigmaKey k = nk;
-      nk = keys[r + 1];  // the array ends with a zero record
+    auto key_step = [&](const SigmaKey& k) {
       const u32 xl = k.xl & ~kSigmaLast;
-      const u32 s = static_cast<u32>(__popcll(k.z & cur) + __popc(k.zl & xl)) & 1U;
-      const u32 m = ((static_cast<u32>(__popc(k.zl & tid)) ^ s) & 1U) << 31;
-      const u32 h = (k.zl >> 10) & 3U;
-      const u32 s1h = 0x3FF00000U ^ ((h & 1U) << 31);
-      const u32 s2h = 0x3FF00000U ^ ((h & 2U) << 30);
-      const u32 s3h = s1h ^ s2h ^ 0x3FF00000U;
+      // Bit 31 of m: parity(z & rep) ^ parity(zl & tid) (the parity of zl & xl
+      // is folded into the coefficients).
+      const u32 m = (static_cast<u32>(__popc(k.zl & tid)) + static_cast<u32>(__popcll(k.z & cur)))
+                    << 31;
+      const u32 s1h = k.sh[0], s2h = k.sh[1], s3h = k.sh[2];
       const double2 kp = make_double2(flip_sign(k.c.x, m), flip_sign(k.c.y, m));
       c0.x += kp.x;
       c0.y += kp.y;
@@ -1161,6 +1165,24 @@ __global__ __launch_bounds__(kSigmaBlock) void sigma_tiled_kernel(
           e0 = e1 = e2 = e3 = zero;
         }
       }
+    };
+    // Key records are loaded kSigmaChunk at a time (one wait on the scalar
+    // loads per chunk); every pass's list is padded with zero records (zero
+    // coefficient, not an item's last key) to a whole number of chunks.
+    // (Pairs of records when sigma^dagger rides along: four would not fit
+    // the scalar registers.)
+    constexpr u32 kChunk = kPair ? kSigmaChunk / 2 : kSigmaChunk;
+    for (u32 r = key_begin; r < key_end; r += kChunk) {
+      SigmaKey k[kChunk];
+#pragma unroll
+      for (u32 i = 0; i < kChunk; ++i) k[i] = keys[r + i];
+      // Empty asm statements that take the records' fields as scalar
+      // operands keep the compiler from moving a record's loads down to its
+      // own step (which would wait on the scalar loads once per key again).
+#pragma unroll
+      for (u32 i = 0; i < kChunk; ++i) hold_key<kPair>(k[i]);
+#pragma unroll
+      for (u32 i = 0; i < kChunk; ++i) key_step(k[i]);
     }
     o0 = make_double2(s0.x + a0.x, s0.y + a0.y);
     o1 = make_double2(s1.x + a1.x, s1.y + a1.y);
@@ -1190,8 +1212,8 @@ __global__ __launch_bounds__(kSigmaBlock) void sigma_tiled_kernel(
 }
 
 // Per-call sigma key records: the plan's records with c = cy[key] (or
-// cy_dag[key] when only sigma^dagger is wanted) and cd = cy_dag[key]; the
-// closing zero record is copied as well.
+// cy_dag[key] when only sigma^dagger is wanted) and cd = cy_dag[key], both
+// times (-1)^{parity(zl & xl)}; padding records keep zero coefficients.
 __global__ __launch_bounds__(256) void sigma_keys_kernel(const SigmaKey* __restrict__ plan_keys,
                                                          std::size_t count,
                                                          const double2* __restrict__ cy,
@@ -1199,11 +1221,13 @@ __global__ __launch_bounds__(256) void sigma_keys_kernel(const SigmaKey* __restr
                                                          int dagger_only,
                                                          SigmaKey* __restrict__ keys) {
   const std::size_t r = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r > count) return;
+  if (r >= count) return;
   SigmaKey k = plan_keys[r];
-  if (r < count) {
-    k.c = dagger_only ? cy_dag[k.key] : cy[k.key];
-    k.cd = cy_dag[k.key];
+  if (k.key != kSigmaPadKey) {
+    const bool neg = __popc(k.zl & k.xl & ~kSigmaLast) & 1;
+    const double2 a = dagger_only ? cy_dag[k.key] : cy[k.key], b = cy_dag[k.key];
+    k.c = neg ? make_double2(-a.x, -a.y) : a;
+    k.cd = neg ? make_double2(-b.x, -b.y) : b;
   }
   keys[r] = k;
 }
@@ -2413,14 +2437,19 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
           k.z = it.z[j];
           k.zl = static_cast<u32>(it.zl[j]);
           k.xl = static_cast<u32>(it.xl) | (j + 1 == it.nz ? kSigmaLast : 0U);
+          for (u32 i = 1; i < 4; ++i) {
+            k.sh[i - 1] = 0x3FF00000U | (host_parity(k.zl & (i << 10)) ? 0x80000000U : 0U);
+          }
           k.key = it.key[j];
           sigma_keys.push_back(k);
         }
       }
+      SigmaKey pad{};
+      pad.key = kSigmaPadKey;
+      while ((sigma_keys.size() - pl.sigma_begin) % kSigmaChunk != 0) sigma_keys.push_back(pad);
       pl.sigma_end = static_cast<u32>(sigma_keys.size());
     }
     plan->sigma_keys = sigma_keys.size();
-    sigma_keys.push_back(SigmaKey{});
     int cus = 0;
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, op->device),
               "query compute units");
@@ -2647,8 +2676,8 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
       // Per-call key records, then one sweep per pass: sigma and sigma^dagger
       // together when both are wanted. The first launched pass writes sigma
       // without reading it.
-      SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys + 1);
-      sigma_keys_kernel<<<grid_for(plan->sigma_keys + 1, 256), 256, 0, stream>>>(
+      SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys);
+      sigma_keys_kernel<<<grid_for(plan->sigma_keys, 256), 256, 0, stream>>>(
           plan->d_sigma_keys, plan->sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
       check_launch("sigma_keys");
       double2* out = reinterpret_cast<double2*>(sigma ? sigma : sigma_dagger);

@@ -97,15 +97,18 @@ struct OwnerBundle {
 // item are consecutive and the last one carries kSigmaLast. The plan holds
 // z, zl, xl and key; every call writes c and cd from its coefficients.
 constexpr u32 kSigmaLast = 1U << 31;
+constexpr u32 kSigmaPadKey = 0xFFFFFFFFU;  // key of a padding record (zero coefficients)
+constexpr u32 kSigmaChunk = 4;             // records per scalar-load chunk; every pass's
+                                           // list is padded to a multiple of it
 
 struct SigmaKey {
   u64 z;       // global z mask (tile part of the sign: parity(z & rep))
   u32 zl;      // z in tile-local coordinates
   u32 xl;      // the item's x in tile-local coordinates | kSigmaLast on its last key
+  u32 sh[3];   // high words of (-1)^{parity(zl & (I << 10))} as a double, I = 1, 2, 3
   u32 key;     // distinct-key id
-  u32 pad[3];
-  double2 c;   // c_key i^{y_key} (sigma), per call
-  double2 cd;  // conj(c_key) i^{y_key} (sigma^dagger), per call
+  double2 c;   // (-1)^{parity(zl & xl)} c_key i^{y_key} (sigma), per call
+  double2 cd;  // the same with conj(c_key) (sigma^dagger), per call
 };
 
 // Items of one sweep variant inside a pass: one launch.
@@ -144,8 +147,8 @@ struct ExpectationPlan {
   std::vector<OwnerBundle> bundles;  // owner sweep bundles of all passes
   OwnerBundle* d_bundles = nullptr;
   u32 sigma_log_blocks = 0;          // tiled sigma grid: 2^sigma_log_blocks blocks (<= CUs)
-  std::size_t sigma_keys = 0;        // tiled sigma key records (the device array holds
-  SigmaKey* d_sigma_keys = nullptr;  // one zero record more, read ahead by the sweep)
+  std::size_t sigma_keys = 0;        // tiled sigma key records, all passes
+  SigmaKey* d_sigma_keys = nullptr;
   u32 diag_items = 0;                // tiled path: leading diagonal items swept by the
                                      // Walsh-Hadamard kernel (0: they ride in pass 0)
   u32 diag_keys = 0;
