@@ -157,16 +157,20 @@ int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t len
  * optional coeffs[i] (num_terms complex128, device; NULL = the operator's own),
  * optional outputs term_values[i], sigma[i] = H psi, sigma_dagger[i] =
  * H^dagger psi; energies holds 2*count doubles (device) for E_i. All pointers
- * are fixed at creation: the work is cut into units served by one persistent
- * kernel from a device work queue and the launch sequence is captured into a
- * HIP graph. tunables = {log2 forward chunk, sigma outputs per unit,
- * persistent grid} or NULL to pick them by timing the batch. */
+ * are fixed at creation: the work of the instances whose plan is direct (small
+ * states or few x-groups) is cut into units served by one persistent kernel from
+ * a device work queue and that launch sequence is captured into a HIP graph;
+ * instances with a tiled plan run the tiled sweeps of wg_expectation_adjoint
+ * after it. tunables = {log2 forward chunk, sigma outputs per unit, persistent
+ * grid} or NULL to pick them by timing the queued part. */
 typedef struct wg_batch wg_batch;
 int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* const* psi,
                     const double* const* coeffs, double* const* term_values, double* const* sigma,
                     double* const* sigma_dagger, double* energies, const int* tunables,
                     void* stream, wg_batch** out);
-/* Enqueues one graph replay on `stream` (does not synchronize). */
+/* Enqueues one graph replay on `stream`, then runs the tiled instances on the
+ * same stream, synchronizing after each (returns without waiting only when
+ * the batch has no tiled instance). */
 int wg_batch_run(wg_batch* batch, void* stream);
 int wg_batch_sync(wg_batch* batch, void* stream);
 int wg_batch_info(const wg_batch* batch, int* tunables, size_t* units, size_t* items);

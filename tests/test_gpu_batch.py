@@ -76,6 +76,47 @@ def test_batch_with_coefficient_overrides(gpu, orc):
             rtol=0, atol=1e-12 * scale)
 
 
+@pytest.mark.parametrize("all_tiled", [False, True])
+def test_batch_with_tiled_instances(gpu, orc, all_tiled):
+    # Instances with a tiled plan (18 qubits and up) bypass the queue and run
+    # the per-instance tiled sweeps after the graph; the rest stay queued.
+    import torch
+
+    from wolfgang_amd.batch import BatchedExpectation, plain_forward_adjoint
+
+    shapes = [(18, 400), (19, 150)] + ([] if all_tiled else [(9, 60), (12, 500), (5, 20)])
+    ops = [random_pauli_masks(n, t, 0.25, 300 + i) if i % 2 else chemistry_masks(n, t, 300 + i)
+           for i, (n, t) in enumerate(shapes)]
+    psis = [random_statevector(n, 400 + i) for i, (n, _) in enumerate(shapes)]
+    rng = np.random.default_rng(9)
+    new_c = [rng.normal(size=op.num_terms) + 1j * rng.normal(size=op.num_terms) for op in ops]
+    b = BatchedExpectation([op.to_device() for op in ops], [torch.from_numpy(p).to("cuda:0") for p in psis],
+                           coeffs=[torch.from_numpy(c) for c in new_c], sigma=True, sigma_dagger=True,
+                           use_cache=False)
+    assert (b.units == 0) == all_tiled
+    for _ in range(2):
+        b.run()
+        torch.cuda.synchronize()
+        for i, (op, psi) in enumerate(zip(ops, psis)):
+            n, x, z = op.num_qubits, op.x_words(), op.z_words()
+            scale = float(np.abs(new_c[i]).sum())
+            want = orc.expectation_compensated(n, x, z, new_c[i], psi)
+            assert abs(complex(b.energies[i].item()) - want) <= 1e-12 * scale
+            np.testing.assert_allclose(b.sigma[i].cpu().numpy(), orc.apply(n, x, z, new_c[i], psi, threads=8),
+                                       rtol=0, atol=1e-12 * scale)
+            np.testing.assert_allclose(b.sigma_dagger[i].cpu().numpy(),
+                                       orc.apply(n, x, z, new_c[i].conj(), psi, threads=8),
+                                       rtol=0, atol=1e-12 * scale)
+            tv = orc.term_values(n, x, z, psi)
+            np.testing.assert_allclose(b.term_values[i].cpu().numpy().real, tv, rtol=0, atol=1e-12)
+    e_batch = b.energies.clone()
+    plain_forward_adjoint(b)
+    torch.cuda.synchronize()
+    for i in range(len(ops)):
+        scale = float(np.abs(new_c[i]).sum())
+        assert abs(complex(b.energies[i].item()) - complex(e_batch[i].item())) <= 1e-12 * scale
+
+
 # ------------------------------------------------------------ batched products
 
 def make_product_pairs(seed, count=9):

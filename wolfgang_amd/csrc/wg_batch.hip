@@ -13,6 +13,11 @@
 // into a HIP graph and replayed; the queue counter is reset by a one-thread
 // kernel inside the graph (no memset node).
 //
+// Instances whose expectation plan is tiled (large states with many x-groups)
+// stay out of the queue: the direct sweeps of a unit read psi once per x-group,
+// the tiled passes once per pass. They run the per-instance tiled path on the
+// same stream after the graph.
+//
 // Tunables (picked by a timing search at creation unless given): forward chunk
 // (amplitude pairs per unit, 2^fwd_log), sigma chunk (outputs per unit) and the
 // persistent grid size.
@@ -45,7 +50,7 @@ struct BatchInst {
   u32 term_base;
   u32 num_terms;
   u32 key_base;
-  u32 pad;
+  u32 slot;  // the instance's position in the batch (its energy entry)
 };
 
 struct BatchUnit {
@@ -234,7 +239,7 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchInst* __
       sr += lds[w].x;
       si += lds[w].y;
     }
-    energies[blockIdx.x] = make_double2(sr, si);
+    energies[in.slot] = make_double2(sr, si);
   }
 }
 
@@ -380,6 +385,7 @@ struct wg_batch {
   }
 
   void launch(hipStream_t stream) {
+    if (insts.empty()) return;
     reset_counter_kernel<<<1, 1, 0, stream>>>(d_counter);
     batch_persistent_kernel<<<grid, kBlock, 0, stream>>>(
         d_units, static_cast<u32>(num_units), d_counter, d_insts, d_items, d_group_x,
@@ -393,6 +399,20 @@ struct wg_batch {
   }
 
   std::vector<std::size_t> item_first;  // first item of each instance
+
+  // Instances whose plan is tiled (the largest ones) run the tiled sweeps of
+  // the per-instance path after the graph: the queue's direct sweeps would
+  // read psi once per x-group.
+  struct TiledInst {
+    wg_pauli_sum* op;
+    const double* psi;
+    const double* coeffs;
+    double* term_values;
+    double* sigma;
+    double* sigma_dag;
+    u32 slot;
+  };
+  std::vector<TiledInst> tiled;
 };
 
 namespace {
@@ -461,7 +481,14 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
       auto* op = const_cast<wg_pauli_sum*>(ops[i]);
       auto plan = get_plan(op, stream);
       b->plans.push_back(plan);
+      if (plan->tiled && op->num_terms > 0) {
+        b->tiled.push_back({op, psi[i], coeffs ? coeffs[i] : nullptr,
+                            term_values ? term_values[i] : nullptr, sigma ? sigma[i] : nullptr,
+                            sigma_dagger ? sigma_dagger[i] : nullptr, static_cast<u32>(i)});
+        continue;
+      }
       BatchInst in{};
+      in.slot = static_cast<u32>(i);
       in.psi = reinterpret_cast<const double2*>(psi[i]);
       in.len = u64{1} << op->num_qubits;
       in.coeffs = coeffs && coeffs[i] ? reinterpret_cast<const double2*>(coeffs[i]) : op->coeffs;
@@ -533,7 +560,9 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
     hipDeviceProp_t prop{};
     check_hip(hipGetDeviceProperties(&prop, device), "get device properties");
     const u32 cus = static_cast<u32>(prop.multiProcessorCount);
-    if (tunables) {
+    if (b->insts.empty()) {
+      // Every instance is tiled: nothing for the queue.
+    } else if (tunables) {
       b->schedule(tunables[0], static_cast<u32>(tunables[1]), static_cast<u32>(tunables[2]), stream);
     } else {
       // Timing search on the real batch, one tunable at a time (forward
@@ -560,7 +589,7 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
       for (u32 g : {cus * 4, cus * 16}) trial(f_pick, s_pick, g);
       b->schedule(best_f, best_s, best_g, stream);
     }
-    capture(b.get(), stream);
+    if (!b->insts.empty()) capture(b.get(), stream);
     *out = b.release();
   });
 }
@@ -570,8 +599,18 @@ int wg_batch_run(wg_batch* b, void* stream_ptr) {
     if (b == nullptr) fail_runtime("batch is empty or destroyed");
     DeviceScope scope(b->device);
     hipStream_t stream = as_stream(stream_ptr);
-    ProfileScope prof("batch_graph", stream);
-    check_hip(hipGraphLaunch(b->exec, stream), "launch graph");
+    if (b->exec) {
+      ProfileScope prof("batch_graph", stream);
+      check_hip(hipGraphLaunch(b->exec, stream), "launch graph");
+    }
+    // Tiled instances behind the graph on the same stream, one at a time
+    // (each synchronizes before its scratch buffers are released).
+    for (const auto& t : b->tiled) {
+      Scratch scratch(b->device);
+      adjoint_enqueue(t.op, t.psi, std::size_t{1} << t.op->num_qubits, t.coeffs, t.term_values,
+                      t.sigma, t.sigma_dag, stream, scratch, b->d_energies + t.slot);
+      sync(stream);
+    }
   });
 }
 

@@ -2634,6 +2634,103 @@ int wg_expectation(const wg_pauli_sum* op_c, const void* psi, int dtype, size_t 
   });
 }
 
+}  // extern "C"
+
+namespace wg {
+
+void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const double* coeffs,
+                     double* term_values, double* sigma, double* sigma_dagger, hipStream_t stream,
+                     Scratch& scratch, double2* dout) {
+  auto plan = get_plan(op, stream);
+  const double2* dpsi = reinterpret_cast<const double2*>(psi);
+  const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
+  if ((sigma || sigma_dagger) && plan->tiled) {
+    // Forward sweeps for E and the per-key values, then sigma over the
+    // same passes (one read of psi and one sigma update per pass).
+    run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+    const double2* key_c = plan->d_key_c;
+    if (coeffs) {
+      double2* kc = scratch.take<double2>(plan->num_keys);
+      key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+          c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
+      check_launch("key_coeff");
+      key_c = kc;
+    }
+    double2* cy = scratch.take<double2>(plan->num_keys);
+    double2* cy_dag = scratch.take<double2>(plan->num_keys);
+    key_cy_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+        key_c, plan->d_key_y, plan->d_flat_key, plan->num_keys, cy, cy_dag);
+    check_launch("key_cy");
+    // Per-call key records, then one sweep per pass: sigma and sigma^dagger
+    // together when both are wanted. The first launched pass writes sigma
+    // without reading it.
+    SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys);
+    sigma_keys_kernel<<<grid_for(plan->sigma_keys, 256), 256, 0, stream>>>(
+        plan->d_sigma_keys, plan->sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
+    check_launch("sigma_keys");
+    double2* out = reinterpret_cast<double2*>(sigma ? sigma : sigma_dagger);
+    double2* out_dag = sigma && sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger) : nullptr;
+    const unsigned grid = 1U << plan->sigma_log_blocks;
+    u32 init = 1;
+    for (std::size_t p = 0; p < plan->passes.size(); ++p) {
+      const PassLaunch& pl = plan->passes[p];
+      if (pl.sigma_end == pl.sigma_begin) continue;
+      ProfileScope prof("sigma_tiled", stream);
+      if (out_dag) {
+        sigma_tiled_kernel<true><<<grid, kSigmaBlock, 0, stream>>>(
+            dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
+            init, out, out_dag);
+      } else {
+        sigma_tiled_kernel<false><<<grid, kSigmaBlock, 0, stream>>>(
+            dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
+            init, out, nullptr);
+      }
+      check_launch("sigma_tiled");
+      init = 0;
+    }
+    if (init) {
+      if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
+      if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
+    }
+  } else if (sigma || sigma_dagger) {
+    // One fused sweep: sigma, sigma^dagger and the per-key values.
+    const double2* key_c = plan->d_key_c;
+    if (coeffs) {
+      double2* kc = scratch.take<double2>(plan->num_keys);
+      key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
+          c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
+      check_launch("key_coeff");
+      key_c = kc;
+    }
+    const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
+    const u64 nchunks = (len + chunk_out - 1) / chunk_out;
+    const u32 rows = static_cast<u32>(std::min<u64>(nchunks, 1024));
+    const u32 K = static_cast<u32>(plan->num_keys);
+    double2* rowbuf = scratch.take<double2>(checked_mul(rows, K, "adjoint partials"));
+    double* key_vals = scratch.take<double>(K);
+    {
+      ProfileScope prof("adjoint_fused", stream);
+      adjoint_fused_kernel<<<rows, kFuseBlock, 0, stream>>>(
+          dpsi, len, plan->d_group_x, plan->d_group_begin, static_cast<u32>(plan->num_groups),
+          plan->d_key_z, plan->d_key_y, key_c, K, reinterpret_cast<double2*>(sigma),
+          reinterpret_cast<double2*>(sigma_dagger), rowbuf);
+      check_launch("adjoint_fused");
+    }
+    fused_reduce_kernel<<<grid_for(K, 256), 256, 0, stream>>>(rowbuf, rows, K, plan->d_flat_key,
+                                                              key_vals);
+    check_launch("fused_reduce");
+    expect_finalize_kernel<<<1, 1024, 0, stream>>>(c, plan->d_term_key, plan->T, key_vals,
+                                                   reinterpret_cast<double2*>(term_values), dout);
+    check_launch("expect_finalize");
+  } else {
+    run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+  }
+}
+
+}  // namespace wg
+
+extern "C" {
+
 int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t length,
                            const double* coeffs, double* term_values, double* sigma,
                            double* sigma_dagger, void* stream_ptr, double* out2) {
@@ -2651,92 +2748,9 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
       sync(stream);
       return;
     }
-    auto plan = get_plan(op, stream);
     Scratch scratch(op->device);
-    const double2* dpsi = reinterpret_cast<const double2*>(psi);
     double2* dout = scratch.take<double2>(1);
-    const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
-    if ((sigma || sigma_dagger) && plan->tiled) {
-      // Forward sweeps for E and the per-key values, then sigma over the
-      // same passes (one read of psi and one sigma update per pass).
-      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
-      const double2* key_c = plan->d_key_c;
-      if (coeffs) {
-        double2* kc = scratch.take<double2>(plan->num_keys);
-        key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
-            c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
-        check_launch("key_coeff");
-        key_c = kc;
-      }
-      double2* cy = scratch.take<double2>(plan->num_keys);
-      double2* cy_dag = scratch.take<double2>(plan->num_keys);
-      key_cy_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
-          key_c, plan->d_key_y, plan->d_flat_key, plan->num_keys, cy, cy_dag);
-      check_launch("key_cy");
-      // Per-call key records, then one sweep per pass: sigma and sigma^dagger
-      // together when both are wanted. The first launched pass writes sigma
-      // without reading it.
-      SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys);
-      sigma_keys_kernel<<<grid_for(plan->sigma_keys, 256), 256, 0, stream>>>(
-          plan->d_sigma_keys, plan->sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
-      check_launch("sigma_keys");
-      double2* out = reinterpret_cast<double2*>(sigma ? sigma : sigma_dagger);
-      double2* out_dag = sigma && sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger) : nullptr;
-      const unsigned grid = 1U << plan->sigma_log_blocks;
-      u32 init = 1;
-      for (std::size_t p = 0; p < plan->passes.size(); ++p) {
-        const PassLaunch& pl = plan->passes[p];
-        if (pl.sigma_end == pl.sigma_begin) continue;
-        ProfileScope prof("sigma_tiled", stream);
-        if (out_dag) {
-          sigma_tiled_kernel<true><<<grid, kSigmaBlock, 0, stream>>>(
-              dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
-              init, out, out_dag);
-        } else {
-          sigma_tiled_kernel<false><<<grid, kSigmaBlock, 0, stream>>>(
-              dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
-              init, out, nullptr);
-        }
-        check_launch("sigma_tiled");
-        init = 0;
-      }
-      if (init) {
-        if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
-        if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
-      }
-    } else if (sigma || sigma_dagger) {
-      // One fused sweep: sigma, sigma^dagger and the per-key values.
-      const double2* key_c = plan->d_key_c;
-      if (coeffs) {
-        double2* kc = scratch.take<double2>(plan->num_keys);
-        key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
-            c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
-        check_launch("key_coeff");
-        key_c = kc;
-      }
-      const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
-      const u64 nchunks = (len + chunk_out - 1) / chunk_out;
-      const u32 rows = static_cast<u32>(std::min<u64>(nchunks, 1024));
-      const u32 K = static_cast<u32>(plan->num_keys);
-      double2* rowbuf = scratch.take<double2>(checked_mul(rows, K, "adjoint partials"));
-      double* key_vals = scratch.take<double>(K);
-      {
-        ProfileScope prof("adjoint_fused", stream);
-        adjoint_fused_kernel<<<rows, kFuseBlock, 0, stream>>>(
-            dpsi, len, plan->d_group_x, plan->d_group_begin, static_cast<u32>(plan->num_groups),
-            plan->d_key_z, plan->d_key_y, key_c, K, reinterpret_cast<double2*>(sigma),
-            reinterpret_cast<double2*>(sigma_dagger), rowbuf);
-        check_launch("adjoint_fused");
-      }
-      fused_reduce_kernel<<<grid_for(K, 256), 256, 0, stream>>>(rowbuf, rows, K, plan->d_flat_key,
-                                                                key_vals);
-      check_launch("fused_reduce");
-      expect_finalize_kernel<<<1, 1024, 0, stream>>>(c, plan->d_term_key, plan->T, key_vals,
-                                                     reinterpret_cast<double2*>(term_values), dout);
-      check_launch("expect_finalize");
-    } else {
-      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
-    }
+    adjoint_enqueue(op, psi, len, coeffs, term_values, sigma, sigma_dagger, stream, scratch, dout);
     double2 h;
     check_hip(hipMemcpyAsync(&h, dout, sizeof(double2), hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");

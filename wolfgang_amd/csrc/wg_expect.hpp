@@ -181,6 +181,15 @@ struct ExpectationPlan {
 
 std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
 
+// Forward + adjoint of one operator with at least one term (the work of
+// wg_expectation_adjoint), enqueued on `stream`: E goes to the device location
+// `dout`, per-term values and sigma / sigma^dagger where their pointers are
+// non-null. Temporary buffers come from `scratch`; the caller synchronizes the
+// stream before `scratch` is destroyed.
+void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const double* coeffs,
+                     double* term_values, double* sigma, double* sigma_dagger, hipStream_t stream,
+                     Scratch& scratch, double2* dout);
+
 // Builds the tiled passes for the given items (host only; exposed for tests of
 // the planner through wg_expectation_plan_info). With diag_separate the
 // diagonal items are moved to the front of `items` (their count in
