@@ -15,8 +15,10 @@
 //      [run rank, word]), written back into the tied positions;
 //   3. each LSD radix pass uses 8-bit digits (constant digits skipped) and is
 //      histogram -> exclusive scan -> stable scatter. Stability within a
-//      4096-key tile comes from wave-level digit matching (ballots) and
-//      per-wave digit counts in LDS, so equal keys keep construction order;
+//      4096-key tile comes from wave-level digit matching (ballots), each
+//      wave ranking a contiguous quarter of the tile, so equal keys keep
+//      construction order; the tile is reordered in LDS and written out in
+//      digit runs;
 //   4. segment heads, one thread per segment summing its coefficients in
 //      sorted (= construction) order, tolerance test with hypot;
 //   5. survivor compaction by scan and a final scatter into a new operator.
@@ -134,53 +136,116 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const u64* __restric
   hist[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x] = counts[threadIdx.x];
 }
 
-__device__ inline u64 match_digit(u32 d) {
-  // Lanes of this wave holding the same 9-bit value d.
-  u64 peers = ~0ULL;
-#pragma unroll
-  for (int b = 0; b < 9; ++b) {
-    const bool bit = (d >> b) & 1U;
-    const u64 m = __ballot(bit);
-    peers &= bit ? m : ~m;
-  }
-  return peers;
-}
-
+// Stable scatter of one 8-bit digit pass over a tile of kTile records.
+// Wave w owns the tile's records [w * kTile / 4, (w + 1) * kTile / 4) in order
+// (rows of 64), so the tile order is (wave, row, lane) and a wave ranks its
+// records without block barriers: per row, the lanes holding the same digit
+// are found with ballots and the wave's running count per digit lives in its
+// own LDS row. The block then forms, per digit, the waves' offsets and the
+// digit's start in the sorted tile, places every record at its sorted position
+// in LDS, and writes the sorted tile out: consecutive threads write
+// consecutive positions of a digit's run (coalesced), where scattering each
+// record from its original slot would touch one cache line per record.
+template <int KW>
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const u64* __restrict__ keys_in, const u32* __restrict__ idx_in, u64* __restrict__ keys_out,
-    u32* __restrict__ idx_out, std::size_t T, int KW, int word, int shift, std::size_t nblocks,
+    u32* __restrict__ idx_out, std::size_t T, int word, int shift, std::size_t nblocks,
     const u32* __restrict__ offsets) {
-  __shared__ u32 base[256];
-  __shared__ u32 counts[kBlock / 64][256];
-  const unsigned lane = threadIdx.x & 63;
-  const unsigned wave = threadIdx.x >> 6;
-  base[threadIdx.x] = offsets[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x];
-  for (int w = 0; w < kBlock / 64; ++w) counts[w][threadIdx.x] = 0;
+  constexpr int kWaves = kBlock / 64;
+  constexpr int kRows = kTile / kBlock;
+  static_assert(kBlock == 256, "one thread per digit");
+  __shared__ u32 counts[kWaves][256];  // running counts per wave, then the waves' offsets
+  __shared__ u32 tstart[256];          // digit's first position in the sorted tile
+  __shared__ u32 gbase[256];           // digit's first output position for this tile
+  __shared__ u32 wsum[kWaves];
+  __shared__ u64 skeys[kTile * KW];
+  __shared__ u32 sidx[kTile];
+  const u32 tid = threadIdx.x, lane = tid & 63U, wave = tid >> 6;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) counts[w][tid] = 0;
+  gbase[tid] = offsets[static_cast<std::size_t>(tid) * nblocks + blockIdx.x];
   __syncthreads();
-  const u64 lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
   const std::size_t tile = static_cast<std::size_t>(blockIdx.x) * kTile;
-  for (int r = 0; r < kItems; ++r) {
-    const std::size_t i = tile + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
-    const bool valid = i < T;
-    const u32 d = valid ? (static_cast<u32>(keys_in[i * KW + word] >> shift) & 0xFFU) : 256U;
-    const u64 peers = match_digit(d);
-    const u32 rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) counts[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      u32 pos = base[d] + rank;
-      for (unsigned w = 0; w < wave; ++w) pos += counts[w][d];
-      for (int k = 0; k < KW; ++k) keys_out[static_cast<std::size_t>(pos) * KW + k] = keys_in[i * KW + k];
-      idx_out[pos] = idx_in[i];
+  const u32 n_tile = static_cast<u32>(std::min<std::size_t>(kTile, T - tile));
+  const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  u64 key[kRows][KW];
+  u32 idx[kRows], dig[kRows], rank[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const u32 j = wave * (kTile / kWaves) + static_cast<u32>(r) * 64U + lane;
+    const std::size_t i = tile + j;
+    if (j < n_tile) {
+#pragma unroll
+      for (int k = 0; k < KW; ++k) key[r][k] = keys_in[i * KW + k];
+      idx[r] = idx_in[i];
+    } else {
+#pragma unroll
+      for (int k = 0; k < KW; ++k) key[r][k] = 0;
+      idx[r] = 0;
     }
-    __syncthreads();
-    u32 add = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-      add += counts[w][threadIdx.x];
-      counts[w][threadIdx.x] = 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const u32 j = wave * (kTile / kWaves) + static_cast<u32>(r) * 64U + lane;
+    const bool valid = j < n_tile;
+    // (word is 0 or KW - 1; a select keeps the key array in registers.)
+    const u64 kword = word == 0 ? key[r][0] : key[r][KW - 1];
+    const u32 d = static_cast<u32>(kword >> shift) & 0xFFU;
+    u64 peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1U;
+      const u64 m = __ballot(bit);
+      peers &= bit ? m : ~m;
     }
-    base[threadIdx.x] += add;
-    __syncthreads();
+    dig[r] = d;
+    const u32 before = valid ? counts[wave][d] : 0U;
+    rank[r] = before + static_cast<u32>(__popcll(peers & lt));
+    if (valid && (peers & lt) == 0) counts[wave][d] = before + static_cast<u32>(__popcll(peers));
+  }
+  __syncthreads();
+  // Thread tid handles digit tid: the waves' offsets, then the tile's
+  // exclusive scan over digits.
+  u32 total = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const u32 c = counts[w][tid];
+    counts[w][tid] = total;
+    total += c;
+  }
+  u32 incl = total;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const u32 t = static_cast<u32>(__shfl_up(static_cast<int>(incl), off, 64));
+    if (lane >= static_cast<u32>(off)) incl += t;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  u32 before_wave = 0;
+  for (u32 w = 0; w < wave; ++w) before_wave += wsum[w];
+  tstart[tid] = before_wave + incl - total;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const u32 j = wave * (kTile / kWaves) + static_cast<u32>(r) * 64U + lane;
+    if (j < n_tile) {
+      const u32 pos = tstart[dig[r]] + counts[wave][dig[r]] + rank[r];
+#pragma unroll
+      for (int k = 0; k < KW; ++k) skeys[pos * KW + k] = key[r][k];
+      sidx[pos] = idx[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kRows; ++r) {
+    const u32 j = static_cast<u32>(r) * kBlock + tid;
+    if (j < n_tile) {
+      const u32 d = static_cast<u32>(skeys[j * KW + word] >> shift) & 0xFFU;
+      const std::size_t g = static_cast<std::size_t>(gbase[d]) + (j - tstart[d]);
+#pragma unroll
+      for (int k = 0; k < KW; ++k) keys_out[g * KW + k] = skeys[j * KW + k];
+      idx_out[g] = sidx[j];
+    }
   }
 }
 
@@ -471,8 +536,15 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
     check_launch("radix_hist");
     exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
     ProfileScope prof("radix_scatter", stream);
-    radix_scatter_kernel<<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
-        keys_a, idx_a, keys_b, idx_b, n, KW, p.word, p.shift, nblocks, offs);
+    if (KW == 1) {
+      radix_scatter_kernel<1><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs);
+    } else if (KW == 2) {
+      radix_scatter_kernel<2><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs);
+    } else {
+      fail_runtime("radix sort records hold one or two key words");
+    }
     check_launch("radix_scatter");
     std::swap(keys_a, keys_b);
     std::swap(idx_a, idx_b);
