@@ -21,7 +21,8 @@
 //      digit runs;
 //   4. segment heads, one thread per segment summing its coefficients in
 //      sorted (= construction) order, tolerance test with hypot;
-//   5. survivor compaction by scan and a final scatter into a new operator.
+//   5. survivor compaction by scan and a final gather into a new operator
+//      (masks of up to 4 words first interleaved into one record per term).
 //
 // The accumulation seed follows the reference exactly: its hash path starts
 // from the first coefficient and its sort path from 0.0, which differ only for
@@ -427,6 +428,54 @@ __global__ __launch_bounds__(256) void compact_gather_kernel(
   }
 }
 
+// Masks of every term interleaved into one record of RW words (x words, z
+// words, zero padding; RW a power of two), so that the compaction's random
+// gather touches one memory sector per term instead of one per mask array.
+__global__ __launch_bounds__(256) void mask_records_kernel(const u64* __restrict__ x,
+                                                           const u64* __restrict__ z,
+                                                           std::size_t T, std::size_t W,
+                                                           std::size_t RW, u64* __restrict__ rec) {
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    u64* r = rec + t * RW;
+    for (std::size_t w = 0; w < W; ++w) {
+      r[w] = x[t * W + w];
+      r[W + w] = z[t * W + w];
+    }
+    for (std::size_t w = 2 * W; w < RW; ++w) r[w] = 0;
+  }
+}
+
+constexpr std::size_t record_words(std::size_t W) { return W == 1 ? 2 : W == 2 ? 4 : 8; }
+
+template <int W>
+__global__ __launch_bounds__(256) void compact_records_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    const u32* __restrict__ out_pos, const u32* __restrict__ idx, const u64* __restrict__ rec,
+    const double2* __restrict__ sums, u64* __restrict__ ox, u64* __restrict__ oz,
+    double2* __restrict__ oc) {
+  constexpr int RW = static_cast<int>(record_words(W));
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (!keep[s]) continue;
+    const std::size_t o = out_pos[s];
+    const std::size_t t = idx[seg_start[s]];
+    u64 r[RW];
+#pragma unroll
+    for (int k = 0; k < RW; k += 2) {
+      const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(rec + t * RW + k);
+      r[k] = v.x;
+      r[k + 1] = v.y;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ox[o * W + w] = r[w];
+      oz[o * W + w] = r[W + w];
+    }
+    oc[o] = sums[s];
+  }
+}
+
 unsigned stride_grid(std::size_t work) {
   const std::size_t g = (work + 255) / 256;
   return static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g, 65536)));
@@ -702,6 +751,23 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   if (U && packed) {
     compact_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, keys_a, 1, W,
                                                        sums, out->x, out->z, out->coeffs);
+    check_launch("compact");
+  } else if (U && W <= 4) {
+    // Interleaved mask records (16, 32 or 64 bytes per term), then the gather.
+    const std::size_t RW = record_words(W);
+    u64* rec = scratch.take<u64>(checked_mul(T, RW, "mask records"));
+    mask_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, RW, rec);
+    check_launch("mask_records");
+    auto gather = [&](auto kernel) {
+      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, idx_a, rec, sums,
+                                                 out->x, out->z, out->coeffs);
+    };
+    switch (W) {
+      case 1: gather(compact_records_kernel<1>); break;
+      case 2: gather(compact_records_kernel<2>); break;
+      case 3: gather(compact_records_kernel<3>); break;
+      default: gather(compact_records_kernel<4>); break;
+    }
     check_launch("compact");
   } else if (U) {
     compact_gather_kernel<<<stride_grid(S), 256, 0, stream>>>(
