@@ -139,6 +139,19 @@ int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint6
 int wg_expectation(const wg_pauli_sum* op, const void* psi, int dtype, size_t length,
                    int psi_on_device, void* stream, double* out2);
 
+/* expectation_z_counts (reference include/wolfgang/pauli_sum.hpp:98,
+ * src/expectation.cpp:157-247; binding bindings/python/pauli_sum_py.cpp:1614-1631):
+ * E = sum_t c_t * sum_r counts[r] (-1)^{|z_t & b_r|} / sum_r counts[r] over
+ * computational-basis count rows, for diagonal terms only. Row r's bitstring
+ * is chars[offsets[r] .. offsets[r+1]) ('0'/'1', right-most character =
+ * qubit 0; offsets has num_rows + 1 entries), counts[r] its count; all three
+ * arrays are host memory. Fails with WG_ERR_INVALID and the reference's
+ * messages, in its order: a term with an X or Y factor, a negative or
+ * non-finite count, a non-positive total, a bitstring of the wrong length or
+ * with another character. out2 receives (Re E, Im E). */
+int wg_expectation_z_counts(const wg_pauli_sum* op, const char* chars, const size_t* offsets,
+                            const double* counts, size_t num_rows, void* stream, double* out2);
+
 /* Forward plus adjoint in one call, all pointers device-resident complex128 on
  * op's device (no reference counterpart: the new adjoint). coeffs (num_terms
  * complex128, may be NULL) replaces the operator's own coefficients for this

@@ -45,6 +45,16 @@ def _prep(n: int, x, z, c):
     return x, z, c
 
 
+def _z_count_args(bitstrings, counts):
+    """Concatenated UTF-8 bitstrings with row offsets, and the counts."""
+    enc = [b.encode("utf-8") for b in bitstrings]
+    offsets = np.zeros(len(enc) + 1, dtype=np.uintp)
+    if enc:
+        offsets[1:] = np.cumsum([len(b) for b in enc])
+    counts = np.ascontiguousarray(counts, dtype=np.float64)
+    return b"".join(enc), offsets, counts
+
+
 class _Lib:
     def __init__(self, path: pathlib.Path, prefix: str) -> None:
         if not path.exists():
@@ -176,6 +186,17 @@ class Oracle(_Lib):
         self.check(f(n, c.shape[0], _arr(x), _arr(z), _arr(c), _arr(psi), psi.shape[0], threads, out))
         return complex(out[0], out[1])
 
+    def expectation_z_counts(self, n, x, z, c, bitstrings, counts):
+        """E over computational-basis counts (diagonal terms only)."""
+        x, z, c = _prep(n, x, z, c)
+        chars, offsets, counts = _z_count_args(bitstrings, counts)
+        out = (C.c_double * 2)()
+        f = self.fn("expectation_z_counts")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, C.c_char_p, _vp, _vp, _sz, C.POINTER(C.c_double)]
+        self.check(f(n, c.shape[0], _arr(x), _arr(z), _arr(c), chars, _arr(offsets), _arr(counts),
+                     counts.shape[0], out))
+        return complex(out[0], out[1])
+
     def apply(self, n, x, z, c, psi, threads=1):
         """sigma = H psi."""
         x, z, c = _prep(n, x, z, c)
@@ -223,6 +244,17 @@ class Reference(_Lib):
             out.append((gx[k:k + m, :W], gz[k:k + m, :W], gc[k:k + m]))
             k += m
         return out
+
+    def expectation_z_counts(self, n, x, z, c, bitstrings, counts):
+        """E over computational-basis counts (diagonal terms only)."""
+        x, z, c = _prep(n, x, z, c)
+        chars, offsets, counts = _z_count_args(bitstrings, counts)
+        out = (C.c_double * 2)()
+        f = self.fn("expectation_z_counts")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, C.c_char_p, _vp, _vp, _sz, C.POINTER(C.c_double)]
+        self.check(f(n, c.shape[0], _arr(x), _arr(z), _arr(c), chars, _arr(offsets), _arr(counts),
+                     counts.shape[0], out))
+        return complex(out[0], out[1])
 
 
 _oracle: Oracle | None = None

@@ -123,6 +123,23 @@ int wgref_expectation(std::size_t n, std::size_t T, const std::uint64_t* x,
   });
 }
 
+int wgref_expectation_z_counts(std::size_t n, std::size_t T, const std::uint64_t* x,
+                               const std::uint64_t* z, const double* c, const char* chars,
+                               const std::size_t* offsets, const double* counts, std::size_t R,
+                               double* out2) {
+  return guarded([&] {
+    const auto op = build(n, T, x, z, c);
+    std::vector<std::string> bitstrings;
+    std::vector<double> values(counts, counts + R);
+    for (std::size_t r = 0; r < R; ++r) {
+      bitstrings.emplace_back(chars + offsets[r], offsets[r + 1] - offsets[r]);
+    }
+    const auto e = op.expectation_z_counts(bitstrings, values);
+    out2[0] = e.real();
+    out2[1] = e.imag();
+  });
+}
+
 int wgref_expectation_c64(std::size_t n, std::size_t T, const std::uint64_t* x,
                           const std::uint64_t* z, const double* c, const float* psi,
                           std::size_t len, double* out2) {

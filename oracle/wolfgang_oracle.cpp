@@ -635,6 +635,57 @@ int orc_expectation_c64(std::size_t n, std::size_t T, const u64* x, const u64* z
   });
 }
 
+// expectation_z_counts (reference src/expectation.cpp:157-247): diagonal
+// terms only; counts validated (non-negative, finite, positive total) before
+// the bitstrings are packed (character i is qubit n - 1 - i); then, term by
+// term, the signed counts summed in row order, divided by the total and
+// scaled by the coefficient, accumulated in term order. Row r's bitstring is
+// chars[offsets[r] .. offsets[r + 1]).
+int orc_expectation_z_counts(std::size_t n, std::size_t T, const u64* x, const u64* z,
+                             const double* c, const char* chars, const std::size_t* offsets,
+                             const double* counts, std::size_t R, double* out2) {
+  return guarded([&] {
+    const std::size_t W = n == 0 ? 0 : (n + 63) / 64;
+    for (std::size_t i = 0; i < T * W; ++i) {
+      if (x[i] != 0) throw std::invalid_argument("expectation_z_counts requires diagonal Pauli terms");
+    }
+    double total = 0.0;
+    for (std::size_t r = 0; r < R; ++r) {
+      if (counts[r] < 0.0) throw std::invalid_argument("Z-count values must be non-negative");
+      if (!std::isfinite(counts[r])) throw std::invalid_argument("Z-count values must be finite");
+      total += counts[r];
+    }
+    if (total <= 0.0) throw std::invalid_argument("Z-count total count must be positive");
+    std::vector<u64> packed(R * W, 0);
+    for (std::size_t r = 0; r < R; ++r) {
+      const std::size_t len = offsets[r + 1] - offsets[r];
+      if (len != n) throw std::invalid_argument("Z-count bitstring length must equal num_qubits");
+      for (std::size_t i = 0; i < len; ++i) {
+        const char bit = chars[offsets[r] + i];
+        if (bit != '0' && bit != '1') {
+          throw std::invalid_argument("Z-count bitstrings may contain only 0 or 1");
+        }
+        if (bit == '1') {
+          const std::size_t q = n - 1 - i;
+          packed[r * W + q / 64] |= u64{1} << (q % 64);
+        }
+      }
+    }
+    cplx result{0.0, 0.0};
+    for (std::size_t t = 0; t < T; ++t) {
+      double weighted = 0.0;
+      for (std::size_t r = 0; r < R; ++r) {
+        bool odd = false;
+        for (std::size_t w = 0; w < W; ++w) odd ^= (std::popcount(z[t * W + w] & packed[r * W + w]) & 1) != 0;
+        weighted += counts[r] * (odd ? -1.0 : 1.0);
+      }
+      result += cplx(c[2 * t], c[2 * t + 1]) * (weighted / total);
+    }
+    out2[0] = result.real();
+    out2[1] = result.imag();
+  });
+}
+
 // Per-term <psi|P_k|psi> (real) with compensated summation, terms spread over
 // threads. The reference computes the same quantity as `term_expectation`
 // (src/expectation.cpp:117-153); the adjoint needs it per term (dE/dc_k).

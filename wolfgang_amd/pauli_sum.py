@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import numbers
+from collections.abc import Mapping
 from typing import Any, Iterable
 
 import numpy as np
@@ -365,6 +366,14 @@ class PauliSum:
     def expectation_statevector(self, psi: Any) -> complex:
         return self.to_device().expectation_statevector(psi)
 
+    def expectation_z_counts(self, counts: Any) -> complex:
+        """Expectation over computational-basis counts (reference
+        bindings/python/pauli_sum_py.cpp:1614-1631): ``counts`` maps dense
+        bitstrings (right-most bit = qubit 0) to finite non-negative counts;
+        only diagonal terms are supported."""
+        bitstrings, values = _parse_z_counts(counts)
+        return self.to_device().expectation_z_counts(bitstrings, values)
+
     @staticmethod
     def _checked_matmul_size_for_testing(
         lhs_terms: int, rhs_terms: int, max_intermediate_terms: int
@@ -604,6 +613,27 @@ class DevicePauliSum:
         _native.call("wg_expectation", self._handle(), ptr, dtype, length, on_device, stream_ptr, out)
         return complex(out[0], out[1])
 
+    def expectation_z_counts(self, counts: Any, values: Any = None, stream: Any = None) -> complex:
+        """Expectation over computational-basis counts, on the device. ``counts``
+        is a mapping of dense bitstrings to counts, or (with ``values``) a
+        sequence of bitstrings and their counts."""
+        if values is None:
+            bitstrings, values = _parse_z_counts(counts)
+        else:
+            bitstrings = list(counts)
+            values = np.ascontiguousarray(values, dtype=np.float64)
+            if values.ndim != 1 or values.shape[0] != len(bitstrings):
+                raise ValueError("Z-count bitstring and count arrays must have the same length")
+        encoded = [b.encode("utf-8") for b in bitstrings]
+        offsets = np.zeros(len(encoded) + 1, dtype=np.uint64)
+        if encoded:
+            offsets[1:] = np.cumsum([len(b) for b in encoded], dtype=np.uint64)
+        chars = b"".join(encoded)
+        out = (C.c_double * 2)()
+        _native.call("wg_expectation_z_counts", self._handle(), chars, offsets.ctypes.data,
+                     values.ctypes.data, len(encoded), _stream_arg(stream), out)
+        return complex(out[0], out[1])
+
     def expectation_plan_info(self) -> dict[str, Any]:
         """Summary of the cached expectation plan (built on first use)."""
         keys, groups, sweeps, tiled = C.c_size_t(), C.c_size_t(), C.c_size_t(), C.c_int()
@@ -626,6 +656,28 @@ class DevicePauliSum:
 
     def __repr__(self) -> str:
         return f"DevicePauliSum(num_qubits={self._n}, num_terms={self._t}, device={self._dev}, backend='hip')"
+
+
+def _parse_z_counts(counts: Any) -> tuple[list[str], np.ndarray]:
+    """Bitstrings and counts of a counts mapping, with the reference's checks
+    and messages (bindings/python/pauli_sum_py.cpp:261-300)."""
+    if not isinstance(counts, Mapping):
+        raise ValueError("counts must be a mapping from dense bitstrings to numeric counts")
+    bitstrings: list[str] = []
+    values: list[float] = []
+    for item in counts.items():
+        bitstring, count = item
+        if not isinstance(bitstring, str):
+            raise ValueError("Z-count bitstrings must be strings")
+        if isinstance(count, (str, bytes, bytearray)):
+            raise ValueError("Z-count values must be numeric")
+        try:
+            value = float(count)
+        except (TypeError, ValueError):
+            raise ValueError("Z-count values must be numeric") from None
+        bitstrings.append(bitstring)
+        values.append(value)
+    return bitstrings, np.asarray(values, dtype=np.float64)
 
 
 def _size_t() -> C.c_size_t:
