@@ -161,8 +161,8 @@ def main() -> int:
     args = parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU only (requested --gpus {args.gpus}); the sharded design is "
-            "described in DESIGN.md and no multi-GPU figure is produced",
+            f"bench.py measures one GPU only (requested --gpus {args.gpus}); no multi-GPU "
+            "figure is produced",
             file=sys.stderr,
         )
         return 2

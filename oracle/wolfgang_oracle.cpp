@@ -14,8 +14,8 @@
 //   x[t*W + w], z[t*W + w] : uint64 words, qubit q is bit q%64 of word q/64
 //   c[2*t], c[2*t+1]       : complex128 coefficient (re, im)
 //
-// Parity status of each function (what pins it) is listed in DESIGN.md and in
-// the docstrings below.
+// What pins each function (golden vectors, closed forms, the compiled
+// reference) is stated in the comments below and exercised by tests/.
 
 #include <algorithm>
 #include <atomic>
