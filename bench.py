@@ -17,6 +17,10 @@ cpu_baseline = the reference's scalar CPU expectation (oracle/_ref, or the
              oracle's restatement where the reference build is absent) timed on
              a bounded sample of the same workload on the host cores.
 
+The other workloads (--workload commutation | matmul | products | batch |
+adjoint | grouping | copy) print their own lines; commutation, matmul,
+grouping and adjoint carry a cpu_baseline of the same kind on a bounded sample.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 
@@ -141,6 +145,132 @@ def cpu_baseline(op, psi_dev, sample_terms: int, gpu_term_values=None) -> dict:
     }
 
 
+def _oracle_module():
+    root = os.path.dirname(os.path.abspath(__file__))
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import oracle as oracle_mod
+
+    return oracle_mod
+
+
+def cpu_baseline_commutation() -> dict:
+    """The reference's commutes_with (oracle/_ref, its own CPU dispatch) on the
+    first 1,024 terms of the benchmark's 128-qubit, 65,536-term operator against
+    all 65,536, single-threaded; beside it the oracle's AVX-512 single-thread
+    and row-threaded restatements of the reference's optimized CPU kernels."""
+    oracle_mod = _oracle_module()
+    from wolfgang_amd.workloads import random_pauli_masks
+
+    n, T, rows = 128, 65_536, 1_024
+    op = random_pauli_masks(n, T, 0.25, 2001)
+    x, z = op.x_words(), op.z_words()
+    ref = oracle_mod.reference()
+    lib = ref if ref is not None else oracle_mod.oracle()
+    t0 = time.perf_counter()
+    lib.commutes_with(n, (x[:rows], z[:rows]), (x, z), max_entries=rows * T)
+    dt = time.perf_counter() - t0
+    orc = oracle_mod.oracle()
+    variants = {}
+    if orc.has_avx512():
+        t0 = time.perf_counter()
+        orc.commutes_with(n, (x[:rows], z[:rows]), (x, z), max_entries=rows * T, variant=1)
+        variants["avx512_single_thread_pairs_per_s"] = rows * T / (time.perf_counter() - t0)
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    orc.commutes_with(n, (x[: 4 * rows], z[: 4 * rows]), (x, z), max_entries=4 * rows * T,
+                      variant=2, threads=threads)
+    variants[f"row_threaded_{threads}_threads_pairs_per_s"] = 4 * rows * T / (time.perf_counter() - t0)
+    return {
+        "value": rows * T / dt,
+        "unit": "pairs/s",
+        "cores": 1,
+        "kind": "reference" if ref is not None else "port",
+        "sample": f"{rows} x {T} pairs (first {rows} terms against all), {dt:.2f} s; "
+        f"full 65,536^2 extrapolated {T * T / (rows * T / dt):.1f} s",
+        "variants_port": variants,
+    }
+
+
+def cpu_baseline_matmul() -> dict:
+    """The reference's matmul with simplify (oracle/_ref, scalar) on the first
+    2,048 terms of the benchmark's lhs operator against all 8,192 rhs terms at
+    128 qubits."""
+    oracle_mod = _oracle_module()
+    from wolfgang_amd.workloads import random_pauli_masks
+
+    n, T, rows = 128, 8_192, 2_048
+    a = random_pauli_masks(n, T, 0.25, 2002)
+    b = random_pauli_masks(n, T, 0.25, 2003)
+    ref = oracle_mod.reference()
+    lib = ref if ref is not None else oracle_mod.oracle()
+    lhs = (a.x_words()[:rows], a.z_words()[:rows], a.coeffs()[:rows])
+    rhs = (b.x_words(), b.z_words(), b.coeffs())
+    t0 = time.perf_counter()
+    lib.matmul(n, lhs, rhs, simplify=True, max_terms=rows * T)
+    dt = time.perf_counter() - t0
+    return {
+        "value": rows * T / dt,
+        "unit": "products/s",
+        "cores": 1,
+        "kind": "reference" if ref is not None else "port",
+        "sample": f"{rows} x {T} products with simplify, scalar single-thread, {dt:.2f} s",
+    }
+
+
+def cpu_baseline_grouping() -> dict:
+    """The reference's group_commuting (oracle/_ref, scalar) on the benchmark's
+    full 16,384-term, 128-qubit operator, both modes."""
+    oracle_mod = _oracle_module()
+    from wolfgang_amd.workloads import random_pauli_masks
+
+    n, T = 128, 16_384
+    op = random_pauli_masks(n, T, 0.25, 2004)
+    ref = oracle_mod.reference()
+    if ref is None:
+        return None
+    out = {}
+    for mode in ("full", "qwc"):
+        t0 = time.perf_counter()
+        groups = ref.group_commuting(n, op.x_words(), op.z_words(), op.coeffs(), mode)
+        out[mode] = {"ms": (time.perf_counter() - t0) * 1000, "groups": len(groups)}
+    return {
+        "value": out["full"]["ms"],
+        "unit": "ms",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"the full operator ({T} terms), scalar single-thread",
+        "results": out,
+    }
+
+
+def cpu_baseline_adjoint() -> dict:
+    """sigma = H psi for 8 terms of the benchmark's random 20,000-term operator
+    over the full 24-qubit state, by the oracle's restatement (the reference has
+    no adjoint), single-threaded."""
+    oracle_mod = _oracle_module()
+    from wolfgang_amd.workloads import random_pauli_masks, random_statevector
+
+    n, T, k = 24, 20_000, 8
+    op = random_pauli_masks(n, T, 0.25, 4001)
+    psi = random_statevector(n, 4002)
+    x, z, c = op.x_words()[:k], op.z_words()[:k], op.coeffs()[:k]
+    t0 = time.perf_counter()
+    oracle_mod.oracle().apply(n, x, z, c, psi, threads=1)
+    dt = time.perf_counter() - t0
+    return {
+        "value": k * psi.shape[0] / dt,
+        "unit": "term·amp pairs/s (sigma)",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{k} terms x 2^{n} amplitudes, sigma only, {dt:.2f} s",
+    }
+
+
+CPU_BASELINES = {"commutation": cpu_baseline_commutation, "matmul": cpu_baseline_matmul,
+                 "grouping": cpu_baseline_grouping, "adjoint": cpu_baseline_adjoint}
+
+
 def copy_calibration(args) -> int:
     """A plain 4 GiB device-to-device copy (torch's copy kernel): the reference
     point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -171,7 +301,11 @@ def main() -> int:
     if args.workload != "expectation":
         from wolfgang_amd import bench_workloads
 
-        return bench_workloads.run(args)
+        line = bench_workloads.compute(args)
+        if args.workload in CPU_BASELINES:
+            line["cpu_baseline"] = None if args.no_cpu_baseline else CPU_BASELINES[args.workload]()
+        print(json.dumps(line), flush=True)
+        return 0
 
     import torch
 

@@ -288,11 +288,16 @@ def adjoint(args) -> dict:
     }
 
 
-def run(args) -> int:
+def compute(args) -> dict:
+    """The JSON line of the selected workload."""
     fn = {"commutation": commutation, "matmul": matmul, "batch": batch, "adjoint": adjoint,
           "products": products, "grouping": grouping}[args.workload]
     line = fn(args)
     line.update({"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
                  "data": "synthetic"})
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def run(args) -> int:
+    print(json.dumps(compute(args)), flush=True)
     return 0
