@@ -979,22 +979,29 @@ __device__ __forceinline__ void owner_dispatch(int kind, std::integer_sequence<i
    ...);
 }
 
-// Grid: bundles * tile_blocks blocks; the blocks of one tile block are
-// consecutive multiples of 8 apart (one XCD, close in time), as in
-// expect_tiled_kernel. 128 VGPRs per lane: two blocks per CU.
+// One launch for the bundles of every pass: the pass of a block comes from
+// the table (its blocks are a contiguous range starting at a multiple of 8);
+// within a pass, bundles * tile_blocks blocks, and the blocks of one tile
+// block are consecutive multiples of 8 apart (one XCD, close in time). The
+// passes are independent (each writes its own partial rows), so one pass's
+// tail overlaps the next pass's start. 128 VGPRs per lane: two blocks per CU.
 __global__ __launch_bounds__(kOwnerBlock) __attribute__((amdgpu_waves_per_eu(4)))
-void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __restrict__ pd,
-                         const OwnerBundle* __restrict__ bundles, u32 groups, u32 tile_blocks,
-                         u32 partial_base, double2* __restrict__ partials) {
+void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __restrict__ passes,
+                         const OwnerBundle* __restrict__ bundles,
+                         const OwnerLaunch* __restrict__ table, const u32* __restrict__ entry,
+                         double2* __restrict__ partials) {
   __shared__ double2 tile[kTilePoints];
   __shared__ double red[kOwnerWaves][kOwnerMaxAcc];
   __shared__ u64 step_mask[32];
-  const u32 b = blockIdx.x;
+  const OwnerLaunch& L = table[entry[blockIdx.x >> 3]];
+  const u32 groups = L.groups, tile_blocks = L.tile_blocks, partial_base = L.partial_base;
+  const PassDesc* pd = passes + L.pass;
+  const u32 b = blockIdx.x - L.block_begin;
   const u32 xcd = b & 7U;
   const u32 slot_id = b >> 3;
   const u32 ig = slot_id % groups;
   const u32 tb = (slot_id / groups) * 8U + xcd;
-  const OwnerBundle& B = bundles[ig];
+  const OwnerBundle& B = bundles[L.owner_first + ig];
   const int kind = static_cast<int>(__builtin_amdgcn_readfirstlane(B.kind));
   owner_dispatch(kind, std::make_integer_sequence<int, kOwnerKinds>{}, psi, pd, B, tb, tile_blocks,
                  partial_base, tile, red, step_mask, partials);
@@ -2259,6 +2266,8 @@ ExpectationPlan::~ExpectationPlan() {
   device_free(d_flat_key);
   device_free(d_key_terms);
   device_free(d_bundles);
+  device_free(d_owner_table);
+  device_free(d_owner_entry);
   device_free(d_sigma_keys);
   if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
 }
@@ -2500,6 +2509,24 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   }
   upload(plan->d_key_terms, key_terms);
   upload(plan->d_bundles, plan->bundles);
+  {
+    // One owner launch for all passes: each pass's blocks are contiguous.
+    std::vector<u32> entry;
+    u64 blocks = 0;
+    for (std::size_t p = 0; p < plan->passes.size(); ++p) {
+      const PassLaunch& pl = plan->passes[p];
+      if (!pl.owner_count) continue;
+      const u64 n_blocks = static_cast<u64>(pl.owner_count) * pl.tile_blocks;
+      if (blocks + n_blocks > 0x7fffffffULL) fail_runtime("owner sweep launch exceeds the grid limit");
+      plan->owner_table.push_back({static_cast<u32>(p), pl.owner_first, pl.owner_count, pl.tile_blocks,
+                                   pl.partial_base, static_cast<u32>(blocks)});
+      entry.insert(entry.end(), n_blocks / 8, static_cast<u32>(plan->owner_table.size() - 1));
+      blocks += n_blocks;
+    }
+    plan->owner_blocks = blocks;
+    upload(plan->d_owner_table, plan->owner_table);
+    upload(plan->d_owner_entry, entry);
+  }
   upload(plan->d_sigma_keys, sigma_keys);
   sync(stream);
   return plan;
@@ -2541,14 +2568,12 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
               partials);
           check_launch("expect_tiled");
         }
-        if (p.owner_count) {
-          const unsigned grid = p.owner_count * p.tile_blocks;
-          ProfileScope prof("expect_owner", stream);
-          expect_owner_kernel<<<grid, kOwnerBlock, 0, stream>>>(
-              psi, plan.d_passes + (&p - plan.passes.data()), plan.d_bundles + p.owner_first,
-              p.owner_count, p.tile_blocks, p.partial_base, partials);
-          check_launch("expect_owner");
-        }
+      }
+      if (plan.owner_blocks) {
+        ProfileScope prof("expect_owner", stream);
+        expect_owner_kernel<<<static_cast<unsigned>(plan.owner_blocks), kOwnerBlock, 0, stream>>>(
+            psi, plan.d_passes, plan.d_bundles, plan.d_owner_table, plan.d_owner_entry, partials);
+        check_launch("expect_owner");
       }
     } else {
       const u64 len = u64{1} << plan.n;

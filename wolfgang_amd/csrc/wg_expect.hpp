@@ -111,6 +111,18 @@ struct SigmaKey {
   double2 cd;  // the same with conj(c_key) (sigma^dagger), per call
 };
 
+// The owner sweeps of all passes run in one launch: each pass with bundles
+// takes a contiguous range of blocks (a multiple of 8, so the XCD of a block
+// is the same for the pass-relative block index).
+struct OwnerLaunch {
+  u32 pass;          // pass index (descriptor)
+  u32 owner_first;   // first bundle of the pass (plan-wide index)
+  u32 groups;        // bundles of the pass
+  u32 tile_blocks;
+  u32 partial_base;
+  u32 block_begin;   // first block of the pass in the launch
+};
+
 // Items of one sweep variant inside a pass: one launch.
 struct SubLaunch {
   int variant;
@@ -146,6 +158,10 @@ struct ExpectationPlan {
   std::vector<PassLaunch> passes;    // tiled path
   std::vector<OwnerBundle> bundles;  // owner sweep bundles of all passes
   OwnerBundle* d_bundles = nullptr;
+  std::vector<OwnerLaunch> owner_table;  // passes with bundles, in launch order
+  OwnerLaunch* d_owner_table = nullptr;
+  u32* d_owner_entry = nullptr;          // [owner_blocks / 8] table entry of each group of 8 blocks
+  u64 owner_blocks = 0;                  // blocks of the single owner launch
   u32 sigma_log_blocks = 0;          // tiled sigma grid: 2^sigma_log_blocks blocks (<= CUs)
   std::size_t sigma_keys = 0;        // tiled sigma key records, all passes
   SigmaKey* d_sigma_keys = nullptr;
