@@ -686,7 +686,8 @@ template <int M, int NZ, bool kOdd>
 __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi,
                                                    const PassDesc* __restrict__ pd,
                                                    const OwnerBundle& B, u32 tb, u32 tile_blocks,
-                                                   u32 partial_base, double2* __restrict__ tile,
+                                                   u32 partial_base, double2* __restrict__ lo,
+                                                   double2* __restrict__ hi,
                                                    double (*red)[kOwnerMaxAcc],
                                                    u64* __restrict__ step_mask,
                                                    double2* __restrict__ partials) {
@@ -742,14 +743,14 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
   WG_OWNER_FETCH(rep);
   for (u64 step = 0; step < run; ++step) {
     __syncthreads();
-    tile[tid + 0 * kOwnerBlock] = st0;
-    tile[tid + 1 * kOwnerBlock] = st1;
-    tile[tid + 2 * kOwnerBlock] = st2;
-    tile[tid + 3 * kOwnerBlock] = st3;
-    tile[tid + 4 * kOwnerBlock] = st4;
-    tile[tid + 5 * kOwnerBlock] = st5;
-    tile[tid + 6 * kOwnerBlock] = st6;
-    tile[tid + 7 * kOwnerBlock] = st7;
+    lo[tid + 0 * kOwnerBlock] = st0;
+    lo[tid + 1 * kOwnerBlock] = st1;
+    lo[tid + 2 * kOwnerBlock] = st2;
+    lo[tid + 3 * kOwnerBlock] = st3;
+    hi[tid + 0 * kOwnerBlock] = st4;
+    hi[tid + 1 * kOwnerBlock] = st5;
+    hi[tid + 2 * kOwnerBlock] = st6;
+    hi[tid + 3 * kOwnerBlock] = st7;
     const double2 own0 = st0, own1 = st1, own2 = st2, own3 = st3;
     const double2 own4 = st4, own5 = st5, own6 = st6, own7 = st7;
     __syncthreads();
@@ -764,10 +765,12 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
     for (int m = 0; m < M; ++m) {
       const bool h = (hmask >> m) & 1U;
       const double2 a[4] = {h ? own4 : own0, h ? own5 : own1, h ? own6 : own2, h ? own7 : own3};
-      const u32 base = (tid16 ^ addr0[m]) | (h ? (4U << 13) : 0U);
+      // The partners share the representatives' half (x is inside the lane bits).
+      const double2* half = h ? hi : lo;
+      const u32 base = tid16 ^ addr0[m];
       double2 q[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base + (static_cast<u32>(j) << 13));
+      for (int j = 0; j < 4; ++j) q[j] = lds_at(half, base + (static_cast<u32>(j) << 13));
       double v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -821,19 +824,24 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
 // only that half of the tile (32 KiB), double-buffered: while tile s is
 // evaluated, global_load_lds fills the other half-buffer with tile s + 1 and
 // the own half of tile s + 1 is loaded into registers. One barrier per tile.
+// The two half-buffers are distinct LDS arrays and the tile loop is unrolled
+// by two, so each step's buffers are known at compile time: the compiler then
+// sees that the reads of tile s cannot alias the loads of tile s + 1 and does
+// not wait for those loads before the reads (with one array and a computed
+// buffer it inserts that wait, and loads would not overlap evaluation).
 typedef __attribute__((address_space(3))) void lds_void;
 
 template <int M, int NZ, bool kOdd>
 __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi,
                                                  const PassDesc* __restrict__ pd,
                                                  const OwnerBundle& B, u32 tb, u32 tile_blocks,
-                                                 u32 partial_base, double2* __restrict__ lds,
+                                                 u32 partial_base, double2* __restrict__ buf0,
+                                                 double2* __restrict__ buf1,
                                                  double (*red)[kOwnerMaxAcc],
                                                  u64* __restrict__ step_mask,
                                                  double2* __restrict__ partials) {
   static_assert(M * NZ <= kOwnerMaxAcc && M <= kOwnerMaxSlots && NZ <= kOwnerMaxKeys,
                 "owner bundle shape");
-  constexpr u32 kHalf = kTilePoints / 2;
   const u32 tid = threadIdx.x;
   const u32 wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const u32 lane = tid & 63U;
@@ -864,22 +872,22 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
   if (tid < 32U) step_mask[tid] = tile_rep(static_cast<u64>(tile_blocks) << tid, free_mask);
   u64 rep = tile_rep(tb, free_mask);
   double2 nx0, nx1, nx2, nx3;
-  auto issue = [&](u64 r, u32 buf) {
+  auto issue = [&](u64 r, double2* buf) {
     const u64 r0 = r ^ sb_w, r1 = r0 ^ sb_e0, r2 = r0 ^ sb_e1, r3 = r1 ^ sb_e1;
     nx0 = psi[r0 | lane];
     nx1 = psi[r1 | lane];
     nx2 = psi[r2 | lane];
     nx3 = psi[r3 | lane];
-    double2* half = lds + buf * kHalf + 64U * wave;
+    double2* half = buf + 64U * wave;
     __builtin_amdgcn_global_load_lds(psi + ((r0 ^ sb_e2) | lane), (lds_void*)(half + 0 * kOwnerBlock), 16, 0, 0);
     __builtin_amdgcn_global_load_lds(psi + ((r1 ^ sb_e2) | lane), (lds_void*)(half + 1 * kOwnerBlock), 16, 0, 0);
     __builtin_amdgcn_global_load_lds(psi + ((r2 ^ sb_e2) | lane), (lds_void*)(half + 2 * kOwnerBlock), 16, 0, 0);
     __builtin_amdgcn_global_load_lds(psi + ((r3 ^ sb_e2) | lane), (lds_void*)(half + 3 * kOwnerBlock), 16, 0, 0);
   };
-  issue(rep, 0);
-  for (u64 step = 0; step < run; ++step) {
-    // This wave's loads of tile `step` have landed; the barrier makes every
-    // wave's half-buffer writes visible and retires tile step - 1's reads.
+  // One step: this wave's loads of tile `step` have landed; the barrier makes
+  // every wave's half-buffer writes visible and retires tile step - 1's reads
+  // of `next`, which then receives tile step + 1 while `cur` is evaluated.
+  auto step_once = [&](u64 step, const double2* cur, double2* next) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const double2 own0 = nx0, own1 = nx1, own2 = nx2, own3 = nx3;
@@ -887,16 +895,15 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
       const u64 msk = step_mask[__builtin_ctzll(step + 1)];
       rep ^= (static_cast<u64>(__builtin_amdgcn_readfirstlane(static_cast<u32>(msk >> 32))) << 32) |
              __builtin_amdgcn_readfirstlane(static_cast<u32>(msk));
-      issue(rep, static_cast<u32>(step + 1) & 1U);
+      issue(rep, next);
     }
-    const double2* tile = lds + (static_cast<u32>(step) & 1U) * kHalf;
     const u32 t32 = tb | (static_cast<u32>(step ^ (step >> 1)) << log_tb);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       const u32 base = tid16 ^ haddr[m];
       double2 q[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] = lds_at(tile, base ^ (static_cast<u32>(j) << 13));
+      for (int j = 0; j < 4; ++j) q[j] = lds_at(cur, base ^ (static_cast<u32>(j) << 13));
       const double2 a[4] = {own0, own1, own2, own3};
       double v[4];
 #pragma unroll
@@ -916,6 +923,12 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
         acc[m][k] = sacc;
       }
     }
+  };
+  issue(rep, buf0);
+  for (u64 step = 0; step < run; step += 2) {
+    step_once(step, buf0, buf1);
+    if (step + 1 == run) break;
+    step_once(step + 1, buf1, buf0);
   }
   // Per-thread sign, then a fixed-order block sum per slot key.
 #pragma unroll
@@ -948,7 +961,8 @@ template <int K>
 __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __restrict__ psi,
                                                    const PassDesc* __restrict__ pd,
                                                    const OwnerBundle& B, u32 tb, u32 tile_blocks,
-                                                   u32 partial_base, double2* __restrict__ tile,
+                                                   u32 partial_base, double2* __restrict__ lo,
+                                                   double2* __restrict__ hi,
                                                    double (*red)[kOwnerMaxAcc],
                                                    u64* __restrict__ step_mask,
                                                    double2* __restrict__ partials) {
@@ -957,10 +971,10 @@ __device__ __forceinline__ void owner_dispatch_one(int kind, const double2* __re
     constexpr int shape = K >> 2;
     if constexpr ((K >> 1) & 1) {
       owner_lane_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], (K & 1) != 0>(
-          psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask, partials);
+          psi, pd, B, tb, tile_blocks, partial_base, lo, hi, red, step_mask, partials);
     } else {
       owner_half_sweep<kOwnerShapeSlots[shape], kOwnerShapeKeys[shape], (K & 1) != 0>(
-          psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask, partials);
+          psi, pd, B, tb, tile_blocks, partial_base, lo, hi, red, step_mask, partials);
     }
   }
 }
@@ -970,11 +984,12 @@ __device__ __forceinline__ void owner_dispatch(int kind, std::integer_sequence<i
                                                const double2* __restrict__ psi,
                                                const PassDesc* __restrict__ pd,
                                                const OwnerBundle& B, u32 tb, u32 tile_blocks,
-                                               u32 partial_base, double2* __restrict__ tile,
+                                               u32 partial_base, double2* __restrict__ lo,
+                                               double2* __restrict__ hi,
                                                double (*red)[kOwnerMaxAcc],
                                                u64* __restrict__ step_mask,
                                                double2* __restrict__ partials) {
-  (owner_dispatch_one<K>(kind, psi, pd, B, tb, tile_blocks, partial_base, tile, red, step_mask,
+  (owner_dispatch_one<K>(kind, psi, pd, B, tb, tile_blocks, partial_base, lo, hi, red, step_mask,
                          partials),
    ...);
 }
@@ -990,7 +1005,10 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
                          const OwnerBundle* __restrict__ bundles,
                          const OwnerLaunch* __restrict__ table, const u32* __restrict__ entry,
                          double2* __restrict__ partials) {
-  __shared__ double2 tile[kTilePoints];
+  // Lane-pivot bundles stage points 0..3 of each thread in `lo` and 4..7 in
+  // `hi`; register-pivot bundles use them as the two partner half-buffers.
+  __shared__ double2 lo[kTilePoints / 2];
+  __shared__ double2 hi[kTilePoints / 2];
   __shared__ double red[kOwnerWaves][kOwnerMaxAcc];
   __shared__ u64 step_mask[32];
   const OwnerLaunch& L = table[entry[blockIdx.x >> 3]];
@@ -1004,7 +1022,7 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
   const OwnerBundle& B = bundles[L.owner_first + ig];
   const int kind = static_cast<int>(__builtin_amdgcn_readfirstlane(B.kind));
   owner_dispatch(kind, std::make_integer_sequence<int, kOwnerKinds>{}, psi, pd, B, tb, tile_blocks,
-                 partial_base, tile, red, step_mask, partials);
+                 partial_base, lo, hi, red, step_mask, partials);
 }
 
 // ------------------------------------------------------------ tiled sigma
