@@ -135,13 +135,22 @@ class Scratch {
   }
   void* take_bytes(std::size_t bytes);
 
+  // mark() / release(m): buffers taken after the mark are given back, so a
+  // loop of calls reuses the same memory. The work that used them must have
+  // completed (the caller has synchronized the stream).
+  std::size_t mark() const { return used_; }
+  void release(std::size_t m);
+
   struct Arena;
 
  private:
   Arena* arena_;
   std::unique_lock<std::mutex> lock_;
   std::size_t used_ = 0;
+  std::size_t high_ = 0;    // high-water mark of used_
+  bool overflowed_ = false;  // some take did not fit the arena
   std::vector<void*> overflow_;
+  std::vector<std::size_t> overflow_at_;  // used_ before each overflow buffer
 };
 
 // ------------------------------------------------------------------ handles

@@ -309,8 +309,12 @@ int wg_product_batch_run(wg_product_batch* b, int simplify_output, void* stream_
         const u64* pz = b->d_z + word_off;
         const double2* pc = b->d_c + c.product_offset[k];
         if (simplify_output) {
+          // simplify_device synchronizes before it returns: its scratch is
+          // free again for the next pair.
+          const std::size_t mark = scratch.mark();
           results[i].reset(simplify_device(px, pz, pc, L->num_qubits, P, b->device, 1.0e-12, 0.0,
                                            b->hash_seed[i], scratch, stream));
+          scratch.release(mark);
         } else {
           std::unique_ptr<wg_pauli_sum> r(new_pauli_sum(L->num_qubits, P, b->device));
           if (W) {

@@ -69,7 +69,7 @@ Scratch::Arena* arena_for(int device) {
 Scratch::Scratch(int device) : arena_(arena_for(device)), lock_(arena_->mutex) {}
 
 Scratch::~Scratch() {
-  if (!overflow_.empty()) {
+  if (overflowed_) {
     // Work using the overflow buffers has completed (entry points synchronize
     // before returning); grow the arena to this call's high-water mark.
     for (void* p : overflow_) device_free(p);
@@ -77,7 +77,8 @@ Scratch::~Scratch() {
     arena_->base = nullptr;
     arena_->capacity = 0;
     void* p = nullptr;
-    const std::size_t want = used_ + used_ / 4;
+    const std::size_t top = std::max(high_, used_);
+    const std::size_t want = top + top / 4;
     if (hipMalloc(&p, want) == hipSuccess) {
       arena_->base = p;
       arena_->capacity = want;
@@ -98,7 +99,19 @@ void* Scratch::take_bytes(std::size_t bytes) {
   void* p = nullptr;
   check_hip(hipMalloc(&p, rounded), "allocate scratch buffer");
   overflow_.push_back(p);
+  overflow_at_.push_back(offset);
+  overflowed_ = true;
   return p;
+}
+
+void Scratch::release(std::size_t m) {
+  while (!overflow_.empty() && overflow_at_.back() >= m) {
+    device_free(overflow_.back());
+    overflow_.pop_back();
+    overflow_at_.pop_back();
+  }
+  high_ = std::max(high_, used_);
+  used_ = std::min(used_, m);
 }
 
 wg_pauli_sum* new_pauli_sum(std::size_t n, std::size_t terms, int device) {
@@ -176,23 +189,56 @@ __global__ __launch_bounds__(kScanBlock) void scan_apply_kernel(const u32* in, s
   // Thread t owns the contiguous run [base + t*kScanItems, +kScanItems).
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kScanTile +
                            static_cast<std::size_t>(threadIdx.x) * kScanItems;
+  // Full runs of 16-byte aligned buffers move as 16-byte vectors (base is a
+  // multiple of kScanItems).
+  const bool full = base + kScanItems <= n &&
+                    ((reinterpret_cast<std::uintptr_t>(in) | reinterpret_cast<std::uintptr_t>(out)) &
+                     15U) == 0;
   u32 vals[kScanItems];
   u32 s = 0;
+  if (full) {
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const std::size_t idx = base + i;
-    vals[i] = idx < n ? in[idx] : 0;
-    s += vals[i];
+    for (int i = 0; i < kScanItems; i += 4) {
+      const uint4 v = *reinterpret_cast<const uint4*>(in + base + i);
+      vals[i] = v.x;
+      vals[i + 1] = v.y;
+      vals[i + 2] = v.z;
+      vals[i + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const std::size_t idx = base + i;
+      vals[i] = idx < n ? in[idx] : 0;
+    }
   }
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) s += vals[i];
   u32 total;
   u32 run = block_exclusive_scan(s, lds, &total);
   const u32 off = block_offsets ? block_offsets[blockIdx.x] : 0;
   run += off;
+  if (full) {
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const std::size_t idx = base + i;
-    if (idx < n) out[idx] = run;
-    run += vals[i];
+    for (int i = 0; i < kScanItems; i += 4) {
+      uint4 v;
+      v.x = run;
+      run += vals[i];
+      v.y = run;
+      run += vals[i + 1];
+      v.z = run;
+      run += vals[i + 2];
+      v.w = run;
+      run += vals[i + 3];
+      *reinterpret_cast<uint4*>(out + base + i) = v;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const std::size_t idx = base + i;
+      if (idx < n) out[idx] = run;
+      run += vals[i];
+    }
   }
   if (total_out != nullptr && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     *total_out = off + total;

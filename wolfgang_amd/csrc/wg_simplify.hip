@@ -448,6 +448,73 @@ __global__ __launch_bounds__(256) void mask_records_kernel(const u64* __restrict
 
 constexpr std::size_t record_words(std::size_t W) { return W == 1 ? 2 : W == 2 ? 4 : 8; }
 
+// One pass over the terms of a multi-word operator (2 <= W <= 4, whose keys
+// never pack into one word): OR / AND of every key word ([x0, z0, x1, z1,
+// ...], combined with atomics), the block's max |c|, the word-0 sort records
+// and the interleaved mask records of the final gather. Replaces separate
+// passes that each read the masks again.
+template <int W>
+__global__ __launch_bounds__(256) void prologue_kernel(
+    const u64* __restrict__ x, const u64* __restrict__ z, const double2* __restrict__ c,
+    std::size_t T, unsigned long long* or_out, unsigned long long* and_out,
+    double* __restrict__ block_max, u64* __restrict__ keys, u32* __restrict__ idx,
+    u64* __restrict__ rec) {
+  static_assert(W >= 2 && W <= 4, "prologue for multi-word keys");
+  constexpr int RW = static_cast<int>(record_words(W));
+  __shared__ double lds[4];
+  u64 o[2 * W], a[2 * W];
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    o[k] = 0;
+    a[k] = ~0ULL;
+  }
+  double m = 0.0;
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    u64 r[RW];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      r[w] = x[t * W + w];
+      r[W + w] = z[t * W + w];
+    }
+#pragma unroll
+    for (int w = 2 * W; w < RW; ++w) r[w] = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      o[2 * w] |= r[w];
+      a[2 * w] &= r[w];
+      o[2 * w + 1] |= r[W + w];
+      a[2 * w + 1] &= r[W + w];
+    }
+    const double2 ct = c[t];
+    m = fmax(m, hypot(ct.x, ct.y));
+    keys[t] = r[0];
+    idx[t] = static_cast<u32>(t);
+#pragma unroll
+    for (int k = 0; k < RW; k += 2) {
+      *reinterpret_cast<ulonglong2*>(rec + t * RW + k) = make_ulonglong2(r[k], r[k + 1]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    u64 ok = o[k], ak = a[k];
+    for (int d = 32; d > 0; d >>= 1) {
+      ok |= __shfl_xor(ok, d, 64);
+      ak &= __shfl_xor(ak, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicOr(&or_out[k], static_cast<unsigned long long>(ok));
+      atomicAnd(&and_out[k], static_cast<unsigned long long>(ak));
+    }
+  }
+  for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    block_max[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  }
+}
+
 template <int W>
 __global__ __launch_bounds__(256) void compact_records_kernel(
     const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
@@ -609,15 +676,56 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   if (T == 0) return new_pauli_sum(n, 0, device);
   if (T >= 0xFFFFFFFFULL) fail_invalid("simplify supports at most 2^32 - 2 terms per call");
 
-  // Threshold atol + rtol * max|c| (reference src/simplify.cpp:156-160).
+  // Threshold atol + rtol * max|c| (reference src/simplify.cpp:156-160), key
+  // statistics and, for multi-word keys, the sort and mask records in the
+  // same pass.
+  const bool fused = W >= 2 && W <= 4;
+  u32* flags = scratch.take<u32>(T);
+  u64* keys_a = scratch.take<u64>(T);
+  u32* idx_a = scratch.take<u32>(T);
+  u64* keys_b = scratch.take<u64>(T);
+  u32* idx_b = scratch.take<u32>(T);
+  u64* rec = fused ? scratch.take<u64>(checked_mul(T, record_words(W), "mask records")) : nullptr;
   const unsigned mgrid = std::min<unsigned>(stride_grid(T), 1024);
   double* dmax = scratch.take<double>(mgrid);
-  max_abs_kernel<<<mgrid, 256, 0, stream>>>(c, T, dmax);
-  check_launch("max_abs");
+  std::vector<unsigned long long> ors(2 * W, 0ULL), ands(2 * W, ~0ULL);
+  unsigned long long* dstat = W ? scratch.take<unsigned long long>(4 * W) : nullptr;
+  if (W) {
+    check_hip(hipMemcpyAsync(dstat, ors.data(), 2 * W * sizeof(u64), hipMemcpyHostToDevice, stream),
+              "copy host buffer to device");
+    check_hip(hipMemcpyAsync(dstat + 2 * W, ands.data(), 2 * W * sizeof(u64),
+                             hipMemcpyHostToDevice, stream),
+              "copy host buffer to device");
+  }
+  if (fused) {
+    auto launch = [&](auto kernel) {
+      kernel<<<mgrid, 256, 0, stream>>>(x, z, c, T, dstat, dstat + 2 * W, dmax, keys_a, idx_a, rec);
+    };
+    switch (W) {
+      case 2: launch(prologue_kernel<2>); break;
+      case 3: launch(prologue_kernel<3>); break;
+      default: launch(prologue_kernel<4>); break;
+    }
+    check_launch("prologue");
+  } else {
+    max_abs_kernel<<<mgrid, 256, 0, stream>>>(c, T, dmax);
+    check_launch("max_abs");
+    if (W) {
+      key_stats_kernel<<<mgrid, 256, 0, stream>>>(x, z, T, W, dstat, dstat + 2 * W);
+      check_launch("key_stats");
+    }
+  }
   std::vector<double> hmax(mgrid);
   check_hip(hipMemcpyAsync(hmax.data(), dmax, mgrid * sizeof(double), hipMemcpyDeviceToHost,
                            stream),
             "copy device buffer to host");
+  if (W) {
+    check_hip(hipMemcpyAsync(ors.data(), dstat, 2 * W * sizeof(u64), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    check_hip(hipMemcpyAsync(ands.data(), dstat + 2 * W, 2 * W * sizeof(u64),
+                             hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+  }
   sync(stream);
   double max_abs = 0.0;
   for (double v : hmax) max_abs = std::max(max_abs, v);
@@ -630,33 +738,9 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     hash_seed = seed_mode != 0;
   }
 
-  // Key statistics and the digit plan.
-  std::vector<unsigned long long> ors(2 * W, 0ULL), ands(2 * W, ~0ULL);
-  if (W) {
-    unsigned long long* dstat = scratch.take<unsigned long long>(4 * W);
-    check_hip(hipMemcpyAsync(dstat, ors.data(), 2 * W * sizeof(u64), hipMemcpyHostToDevice, stream),
-              "copy host buffer to device");
-    check_hip(hipMemcpyAsync(dstat + 2 * W, ands.data(), 2 * W * sizeof(u64),
-                             hipMemcpyHostToDevice, stream),
-              "copy host buffer to device");
-    key_stats_kernel<<<std::min<unsigned>(stride_grid(T), 1024), 256, 0, stream>>>(
-        x, z, T, W, dstat, dstat + 2 * W);
-    check_launch("key_stats");
-    check_hip(hipMemcpyAsync(ors.data(), dstat, 2 * W * sizeof(u64), hipMemcpyDeviceToHost, stream),
-              "copy device buffer to host");
-    check_hip(hipMemcpyAsync(ands.data(), dstat + 2 * W, 2 * W * sizeof(u64),
-                             hipMemcpyDeviceToHost, stream),
-              "copy device buffer to host");
-    sync(stream);
-  }
   // Segments of equal keys in sorted order, as positions into idx; the
   // word-0 keys (one-word keys: the whole key) in sorted order.
   const bool packed = W == 0 || (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0);
-  u32* flags = scratch.take<u32>(T);
-  u64* keys_a = scratch.take<u64>(T);
-  u32* idx_a = scratch.take<u32>(T);
-  u64* keys_b = scratch.take<u64>(T);
-  u32* idx_b = scratch.take<u32>(T);
   if (packed) {
     // One-word keys: x << 32 | z (or 0 for zero-qubit operators).
     const u64 kor = W ? (ors[0] << 32) | ors[1] : 0;
@@ -668,8 +752,10 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     check_launch("run_heads");
   } else {
     // Word 0 (x0) for every item, then the tied items word by word.
-    word_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, keys_a, idx_a);
-    check_launch("word_records");
+    if (!fused) {
+      word_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, keys_a, idx_a);
+      check_launch("word_records");
+    }
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
     u64* k0 = keys_a;
@@ -754,10 +840,12 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     check_launch("compact");
   } else if (U && W <= 4) {
     // Interleaved mask records (16, 32 or 64 bytes per term), then the gather.
-    const std::size_t RW = record_words(W);
-    u64* rec = scratch.take<u64>(checked_mul(T, RW, "mask records"));
-    mask_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, RW, rec);
-    check_launch("mask_records");
+    if (!fused) {
+      const std::size_t RW = record_words(W);
+      rec = scratch.take<u64>(checked_mul(T, RW, "mask records"));
+      mask_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, RW, rec);
+      check_launch("mask_records");
+    }
     auto gather = [&](auto kernel) {
       kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, idx_a, rec, sums,
                                                  out->x, out->z, out->coeffs);
