@@ -122,19 +122,34 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const u64* __restric
                                                             std::size_t T, int KW, int word,
                                                             int shift, std::size_t nblocks,
                                                             u32* __restrict__ hist) {
-  __shared__ u32 counts[256];
-  counts[threadIdx.x] = 0;
+  // One histogram per wave (fewer colliding LDS atomics), summed at the end.
+  __shared__ u32 counts[kBlock / 64][256];
+  static_assert(kBlock == 256 && kItems % 2 == 0, "histogram layout");
+  const u32 wave = threadIdx.x >> 6;
+  for (int w = 0; w < kBlock / 64; ++w) counts[w][threadIdx.x] = 0;
   __syncthreads();
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kTile;
-  for (int r = 0; r < kItems; ++r) {
-    const std::size_t i = base + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
-    if (i < T) {
-      const u32 d = static_cast<u32>(keys[i * KW + word] >> shift) & 0xFFU;
-      atomicAdd(&counts[d], 1U);
+  if (KW == 1 && base + kTile <= T && (reinterpret_cast<std::uintptr_t>(keys) & 15U) == 0) {
+    // Full tile of one-word keys: two keys per 16-byte load.
+    const ulonglong2* k2 = reinterpret_cast<const ulonglong2*>(keys + base);
+    for (int r = 0; r < kItems / 2; ++r) {
+      const ulonglong2 v = k2[r * kBlock + threadIdx.x];
+      atomicAdd(&counts[wave][static_cast<u32>(v.x >> shift) & 0xFFU], 1U);
+      atomicAdd(&counts[wave][static_cast<u32>(v.y >> shift) & 0xFFU], 1U);
+    }
+  } else {
+    for (int r = 0; r < kItems; ++r) {
+      const std::size_t i = base + static_cast<std::size_t>(r) * kBlock + threadIdx.x;
+      if (i < T) {
+        const u32 d = static_cast<u32>(keys[i * KW + word] >> shift) & 0xFFU;
+        atomicAdd(&counts[wave][d], 1U);
+      }
     }
   }
   __syncthreads();
-  hist[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x] = counts[threadIdx.x];
+  u32 c = 0;
+  for (int w = 0; w < kBlock / 64; ++w) c += counts[w][threadIdx.x];
+  hist[static_cast<std::size_t>(threadIdx.x) * nblocks + blockIdx.x] = c;
 }
 
 // Stable scatter of one 8-bit digit pass over a tile of kTile records.
