@@ -9,21 +9,33 @@ coefficients N(0,1) exp(-|q-p|/4)). One step = one full
 ``DevicePauliSum.expectation_statevector(psi)`` over resident inputs.
 
 value      = num_terms * 2^n / seconds per step  (term*amplitude pairs/s)
-roofline   = algorithmic bytes of the dominant kernel per launch over its
-             average launch time (HIP events on its launch stream), against
-             8 TB/s HBM. The dominant kernel is the owner sweep: one launch per
-             evaluation covering every off-diagonal x-group, (G_x - 1) * 2^n *
-             16 B (G_x = distinct nonzero x-masks + 1 if any term is diagonal;
-             the diagonal group has its own Walsh-Hadamard sweep).
+roofline   = the dominant kernel (the owner sweep: one launch per evaluation
+             covering every off-diagonal x-group of every tiled pass) against
+             8 TB/s HBM: achieved = its HBM bytes per launch (FETCH_SIZE +
+             WRITE_SIZE from the committed rocprofv3 summary of this workload,
+             scaled by a plain copy's counters) over its average launch time
+             (HIP events on its launch stream, measured in this run), frac =
+             achieved / peak. Beside it: the definition-equivalent rate (one
+             read of psi per x-group, (G_x - 1) * 2^n * 16 B per launch), the
+             floor of this algorithm (one read of psi per tiled pass) and the
+             measured traffic over that floor.
 cpu_baseline = the reference's scalar CPU expectation (oracle/_ref, or the
              oracle's restatement where the reference build is absent) timed on
-             a bounded sample of the same workload on the host cores.
+             16 terms of the same Hamiltonian (4 of each kind) over the full
+             state, one call per term after one warm-up call, on one host core:
+             median / min / max of the per-call rates, with the host's CPU
+             model, sockets, cores and flags.
+batched    = the 256-instance forward + adjoint batch (n in 12..22,
+             100..50,000 random terms; sigma and sigma^dagger) in one graph
+             replay against the plain per-instance path on the same buffers:
+             the speedup and the largest energy difference (secondary fields;
+             --no-batch skips them).
 
 The other workloads (--workload commutation | matmul | products | batch |
 adjoint | grouping | copy) print their own lines; commutation, matmul,
 grouping and adjoint carry a cpu_baseline of the same kind on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batch]
 """
 
 from __future__ import annotations
@@ -50,7 +62,9 @@ def parse_args() -> argparse.Namespace:
     p.add_argument("--op-seed", type=int, default=3001)
     p.add_argument("--psi-seed", type=int, default=3002)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample-terms", type=int, default=8)
+    p.add_argument("--cpu-sample-terms", type=int, default=16)
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the batched forward+adjoint speedup fields of the default line")
     p.add_argument(
         "--workload",
         default="expectation",
@@ -68,18 +82,27 @@ def parse_args() -> argparse.Namespace:
 TRAFFIC_PROFILE = "profiles/expect_sweep_n30_chem100k.md"
 
 
-def measured_traffic(args):
-    """HBM bytes per launch of the dominant sweep kernel (FETCH_SIZE +
-    WRITE_SIZE, scaled by a plain copy's counters) from the committed rocprofv3
-    summary of the default workload; None for any other configuration."""
+def profile_figures(args) -> dict:
+    """key=value figures (HBM bytes per launch of the dominant sweep kernel,
+    its counter-derived VALU-busy and wait fractions) from the committed
+    rocprofv3 summary of the default workload; empty for any other
+    configuration."""
     default = (args.num_qubits, args.num_terms, args.op_seed, args.psi_seed) == (30, 100_000, 3001, 3002)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    out = {}
     if not default or not os.path.exists(path):
-        return None
+        return out
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch="):
-            return int(line.split("=", 1)[1])
-    return None
+        line = line.strip()
+        if "=" in line and " " not in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            for conv in (int, float, str):
+                try:
+                    out[k] = conv(v)
+                    break
+                except ValueError:
+                    continue
+    return out
 
 
 def distinct_x_groups(op) -> int:
@@ -91,12 +114,46 @@ def distinct_x_groups(op) -> int:
     return nonzero + (1 if (xs == 0).any() else 0)
 
 
+def host_cpu_info() -> dict:
+    """The host's CPU as the baseline record states it: logical CPUs (and the
+    ones this process may run on), model, sockets, cores per socket, threads
+    per core and the popcount / vector flags that matter for the path."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        fields = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
+        info["model"] = fields.get("Model name")
+        for key, name in (("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+                          ("Thread(s) per core", "threads_per_core")):
+            if fields.get(key, "").isdigit():
+                info[name] = int(fields[key])
+        flags = set(fields.get("Flags", "").split())
+        info["flags"] = sorted(flags & {"popcnt", "avx2", "avx512f", "avx512bw", "avx512vl",
+                                        "avx512_vpopcntdq", "fma"})
+    except (OSError, ValueError):
+        pass
+    return info
+
+
 def cpu_baseline(op, psi_dev, sample_terms: int, gpu_term_values=None) -> dict:
-    """Time the reference's scalar expectation on `sample_terms` terms of the
-    same Hamiltonian (the first terms of each kind, in term order) over the full
-    statevector copied to host memory; rate in term*amplitude pairs/s. When the
-    GPU's per-term values of the full operator are given, the same sample's
-    energy from them is compared with the reference's result."""
+    """Time the reference's scalar expectation (benchmarks/bench_cpu_dispatch.py
+    protocol: warm-up, then repeats; median / min / max) on `sample_terms`
+    terms of the same Hamiltonian, a quarter of each kind, over the full
+    statevector copied to host memory: one call per term, one warm-up call
+    before them; rate in term*amplitude pairs/s per call. When the GPU's
+    per-term values of the full operator are given, the sample's energy from
+    them is compared with the reference's result and with the oracle's
+    compensated sums on the host's cores."""
     root = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, root)
     import oracle as oracle_mod
@@ -105,26 +162,37 @@ def cpu_baseline(op, psi_dev, sample_terms: int, gpu_term_values=None) -> dict:
     lib = ref if ref is not None else oracle_mod.oracle()
     kind = "reference" if ref is not None else "port"
     x, z, c = op.x_words(), op.z_words(), op.coeffs()
-    # Pick terms round-robin over (diagonal?, weight class) so hopping and
-    # diagonal terms are both represented.
-    diag = np.nonzero(x[:, 0] == 0)[0]
-    off = np.nonzero(x[:, 0] != 0)[0]
-    pick = list(off[: sample_terms // 2]) + list(diag[: sample_terms - sample_terms // 2])
-    pick = np.asarray(pick, dtype=np.int64)
+    n = op.num_qubits
+    xs, zs = x[:, 0], z[:, 0]
+    popx, popz = np.bitwise_count(xs), np.bitwise_count(zs)
+    off = np.nonzero(popx == 2)[0]
+    # XX hopping: no z bit on p or q (z & x = 0); YY hopping: z covers p and q.
+    xx = off[(xs[off] & zs[off]) == 0]
+    yy = off[(xs[off] & zs[off]) != 0]
+    diag = np.nonzero(popx == 0)[0]
+    z1 = diag[popz[diag] == 1]
+    z2 = diag[popz[diag] == 2]
+    q = max(1, sample_terms // 4)
+    pick = np.concatenate([xx[:q], yy[:q], z1[:q], z2[:sample_terms - 3 * q]]).astype(np.int64)
     psi = psi_dev.cpu().numpy()
-    t0 = time.perf_counter()
-    ref_e = lib.expectation(op.num_qubits, x[pick], z[pick], c[pick], psi)
-    dt = time.perf_counter() - t0
-    pairs = len(pick) * psi.shape[0]
+    # Warm-up call (pages psi in and warms the caches), then one timed call per term.
+    lib.expectation(n, x[pick[-1:]], z[pick[-1:]], c[pick[-1:]], psi)
+    rates, per_term = [], []
+    t_all = time.perf_counter()
+    for t in pick:
+        t0 = time.perf_counter()
+        e = lib.expectation(n, x[t:t + 1], z[t:t + 1], c[t:t + 1], psi)
+        dt = time.perf_counter() - t0
+        rates.append(psi.shape[0] / dt)
+        per_term.append(e)
+    total_s = time.perf_counter() - t_all
+    ref_e = complex(sum(per_term))
     parity = None
     if gpu_term_values is not None:
         gpu_e = complex(np.sum(c[pick] * gpu_term_values[pick]))
         scale = float(np.sum(np.abs(c[pick])))
-        # The oracle's compensated (double-double) sums on the same terms, on
-        # the host's cores: the reference's plain sequential sums carry their
-        # own rounding, so both are compared with it.
         comp = oracle_mod.oracle().expectation_compensated(
-            op.num_qubits, x[pick], z[pick], c[pick], psi, threads=min(16, os.cpu_count() or 1))
+            n, x[pick], z[pick], c[pick], psi, threads=min(16, os.cpu_count() or 1))
         parity = {
             "reference_energy": [ref_e.real, ref_e.imag],
             "gpu_energy_same_terms": [gpu_e.real, gpu_e.imag],
@@ -135,15 +203,21 @@ def cpu_baseline(op, psi_dev, sample_terms: int, gpu_term_values=None) -> dict:
             "gpu_rel_error_vs_sum_abs_c": abs(gpu_e - comp) / scale if scale else None,
             "note": "GPU per-term values of the full operator (tiled sweep) on the sampled terms",
         }
+    rates = np.asarray(rates)
     return {
         "parity": parity,
-        "value": pairs / dt,
+        "value": float(np.median(rates)),
         "unit": "term·amp pairs/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{len(pick)} terms ({len(pick) - sum(1 for p in pick if x[p, 0] == 0)} hopping, "
-        f"{sum(1 for p in pick if x[p, 0] == 0)} diagonal) x 2^{op.num_qubits} amplitudes, "
-        f"scalar single-thread, {dt:.2f} s",
+        "min": float(rates.min()),
+        "max": float(rates.max()),
+        "repeats": int(rates.size),
+        "warmup": 1,
+        "host": host_cpu_info(),
+        "sample": f"{len(pick)} terms ({len(xx[:q])} XX, {len(yy[:q])} YY hopping, {len(z1[:q])} Z_p, "
+        f"{len(pick) - 3 * q} Z_pZ_q) x 2^{n} amplitudes, one call per term after one warm-up "
+        f"call, scalar single-thread, {total_s:.1f} s; value = median per-call rate",
     }
 
 
@@ -361,7 +435,18 @@ def main() -> int:
     kernel_s_per_step = kernel_ms / 1000.0 / args.steps
     dom_bytes_per_launch = groups_in_dominant * (1 << n) * 16
     dom_s_per_launch = dom_ms / 1000.0 / dom_launches if dom_launches else float("nan")
-    achieved = dom_bytes_per_launch / dom_s_per_launch / 1e9
+    definition_gbs = dom_bytes_per_launch / dom_s_per_launch / 1e9
+    # The floor of the tiled algorithm: one read of psi per pass.
+    passes = info["sweeps"] if info["tiled"] else None
+    pass_floor = passes * (1 << n) * 16 if passes else None
+    prof = profile_figures(args)
+    traffic = prof.get("traffic_bytes_per_launch")
+    if traffic:
+        achieved = traffic / dom_s_per_launch / 1e9
+        achieved_source = f"measured HBM bytes per launch ({TRAFFIC_PROFILE}) / launch time in this run"
+    else:
+        achieved = (pass_floor or dom_bytes_per_launch) / dom_s_per_launch / 1e9
+        achieved_source = "one read of psi per pass (no counter summary for this configuration)"
 
     line = {
         "metric": METRIC,
@@ -393,16 +478,24 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args),
-            "traffic_source": TRAFFIC_PROFILE if measured_traffic(args) else None,
+            "traffic": traffic,
+            "achieved_source": achieved_source,
+            "traffic_source": TRAFFIC_PROFILE if traffic else None,
             "kernel": dominant,
             "kernel_avg_ms": dom_s_per_launch * 1000.0,
             "launches_per_step": dom_launches / args.steps if args.steps else 0,
-            "algorithmic_bytes_per_launch": dom_bytes_per_launch,
+            "passes_per_launch": passes,
+            "pass_floor_bytes_per_launch": pass_floor,
+            "traffic_over_pass_floor": traffic / pass_floor if traffic and pass_floor else None,
+            "valu_busy_frac": prof.get("valu_busy_frac"),
+            "wave_wait_frac": prof.get("wave_wait_frac"),
+            "binding": prof.get("binding"),
+            "definition_bytes_per_launch": dom_bytes_per_launch,
+            "definition_equivalent_GBs": definition_gbs,
             "x_groups_per_launch": groups_in_dominant,
             "all_sweep_kernels_ms_per_step": kernel_s_per_step * 1000.0,
             "kernel_ms_per_step_each": {k: ms / args.steps for k, (ms, _) in per_kernel.items()},
-            "evaluation_achieved_GBs": bytes_per_step / kernel_s_per_step / 1e9,
+            "evaluation_definition_equivalent_GBs": bytes_per_step / kernel_s_per_step / 1e9,
         },
     }
     if not args.no_cpu_baseline:
@@ -413,6 +506,12 @@ def main() -> int:
         line["cpu_baseline"] = cpu_baseline(op, psi, args.cpu_sample_terms, tv.cpu().numpy())
     else:
         line["cpu_baseline"] = None
+    if not args.no_batch:
+        del psi, dop
+        torch.cuda.empty_cache()
+        from wolfgang_amd import bench_workloads
+
+        line["batched"] = bench_workloads.batch_speedup(batched_steps=3, plain_steps=2)
     print(json.dumps(line), flush=True)
     return 0
 

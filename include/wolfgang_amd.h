@@ -101,7 +101,8 @@ int wg_commutes_with_host(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, uint
                           size_t out_len, size_t max_entries, void* stream);
 
 /* DeviceCommutationMatrix::empty (reference
- * include/wolfgang/device_commutation_matrix.hpp:206-214). The matrix is stored
+ * include/wolfgang/device_commutation_matrix.hpp:27-35;
+ * src/hip/device_commutation_matrix.hip.cpp:177-212). The matrix is stored
  * as packed bits: row i occupies row_words = ceil(cols/64) uint64 words, bit j%64
  * of word j/64 is entry (i, j); bits past cols are zero. */
 int wg_comm_matrix_create(size_t rows, size_t cols, int device, wg_comm_matrix** out);
@@ -124,8 +125,8 @@ int wg_comm_matrix_to_host_packed(const wg_comm_matrix* m, uint64_t* out, size_t
                                   void* stream);
 
 /* count_commuting / count_commuting_rows / count_commuting_cols and
- * conflict_degrees (reference include/wolfgang/device_commutation_matrix.hpp:216-218;
- * bindings/python/pauli_sum_py.cpp:598-661). axis -1: total into out[0];
+ * conflict_degrees (reference include/wolfgang/device_commutation_matrix.hpp:37-39;
+ * src/hip/device_commutation_matrix.hip.cpp:225-321; bindings/python/pauli_sum_py.cpp:598-661). axis -1: total into out[0];
  * axis 1: per-row counts (out_len = rows); axis 0: per-column counts
  * (out_len = cols). conflicts != 0 counts anticommuting entries instead. */
 int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint64_t* out,
@@ -241,6 +242,15 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
  * *num_passes gets the count. */
 int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                          int* rows, size_t max_passes, size_t* num_passes);
+
+/* Host-only: where the tiled plan of a W = 1 operator evaluates each distinct
+ * key (first-appearance order; test hook, no reference counterpart):
+ * key_pass[k] = its pass (-1: the diagonal Walsh-Hadamard sweep), key_kind[k]
+ * = the kind word of the first owner bundle holding it (bit 0 odd |x & z|,
+ * bit 1 lane pivot, bits 2.. shape; -1: none). Both arrays hold num_terms
+ * entries; *num_keys gets the key count. */
+int wg_debug_key_sweeps(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        int* key_pass, int* key_kind, size_t* num_keys);
 
 /* Kernel timing with HIP events (no reference counterpart; measurement only).
  * While enabled, the library brackets each launch of its hot kernels

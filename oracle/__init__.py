@@ -139,6 +139,12 @@ class Oracle(_Lib):
         self.check(f(n, L, _arr(lx), _arr(lz), R, _arr(rx), _arr(rz), max_entries, variant, threads, _arr(out)))
         return out.reshape(L, R)
 
+    def set_threads(self, threads: int) -> None:
+        """Worker threads for simplify's sort and matmul's products (the
+        results are the same for any count)."""
+        self.fn("set_threads").argtypes = [C.c_int]
+        self.fn("set_threads")(int(threads))
+
     def has_avx512(self) -> bool:
         return bool(self.fn("has_avx512_popcnt")())
 
@@ -206,6 +212,83 @@ class Oracle(_Lib):
         f.argtypes = [_sz, _sz, _vp, _vp, _vp, _vp, _sz, C.c_int, _vp]
         self.check(f(n, c.shape[0], _arr(x), _arr(z), _arr(c), _arr(psi), psi.shape[0], threads, _arr(out)))
         return out
+
+
+    def commutes_packed(self, n, lhs_xz, rhs_xz, threads=1):
+        """Packed commutation bits (rows x ceil(R/64) uint64 words) from the
+        row-threaded kernel."""
+        lx, lz = (np.ascontiguousarray(a, dtype=np.uint64) for a in lhs_xz)
+        rx, rz = (np.ascontiguousarray(a, dtype=np.uint64) for a in rhs_xz)
+        L, R = lx.shape[0], rx.shape[0]
+        out = np.zeros((L, (R + 63) // 64), dtype=np.uint64)
+        f = self.fn("commutes_packed")
+        f.argtypes = [_sz, _sz, _vp, _vp, _sz, _vp, _vp, C.c_int, _vp]
+        self.check(f(n, L, _arr(lx), _arr(lz), R, _arr(rx), _arr(rz), threads, _arr(out)))
+        return out
+
+    def apply_rows(self, n, x, z, c, psi, rows, threads=1):
+        """sigma = H psi at the given output rows."""
+        x, z, c = _prep(n, x, z, c)
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        out = np.zeros(rows.shape[0], dtype=np.complex128)
+        f = self.fn("apply_rows")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, _vp, _sz, _vp, _sz, C.c_int, _vp]
+        self.check(f(n, c.shape[0], _arr(x), _arr(z), _arr(c), _arr(psi), psi.shape[0], _arr(rows),
+                     rows.shape[0], threads, _arr(out)))
+        return out
+
+    @staticmethod
+    def _word0(n, x, z):
+        """Word 0 of one-word masks as flat arrays (zeros at n = 0)."""
+        x = np.asarray(x, dtype=np.uint64)
+        z = np.asarray(z, dtype=np.uint64)
+        T = x.shape[0]
+        if n == 0:
+            return np.zeros(T, dtype=np.uint64), np.zeros(T, dtype=np.uint64)
+        return (np.ascontiguousarray(x.reshape(T, -1)[:, 0]), np.ascontiguousarray(z.reshape(T, -1)[:, 0]))
+
+    def distinct_keys(self, n, x, z):
+        """(number of distinct keys, key id of each term) in first-appearance order."""
+        x, z = self._word0(n, x, z)
+        T = x.shape[0]
+        key = np.zeros(max(T, 1), dtype=np.uint32)
+        k = C.c_size_t()
+        f = self.fn("distinct_keys")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, C.POINTER(C.c_size_t)]
+        self.check(f(n, T, _arr(x), _arr(z), _arr(key), C.byref(k)))
+        return k.value, key[:T]
+
+    def key_values_grouped(self, n, x, z, psi, threads=1, progress=None, pieces=1):
+        """Per-key <P_k> (keys in first-appearance order) by the x-grouped,
+        chunk-paired double-double evaluation; `progress(done, total)` is
+        called after each of `pieces` chunk ranges."""
+        x, z = self._word0(n, x, z)
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        K, _ = self.distinct_keys(n, x, z)
+        hi = np.zeros(max(K, 1), dtype=np.float64)
+        lo = np.zeros(max(K, 1), dtype=np.float64)
+        chunks = psi.shape[0] >> min(n, 16)
+        f = self.fn("key_values_accumulate")
+        f.argtypes = [_sz, _sz, _vp, _vp, _vp, _sz, _sz, _sz, C.c_int, _vp, _vp]
+        bounds = np.linspace(0, chunks, max(1, min(pieces, chunks)) + 1).astype(np.int64)
+        for i in range(len(bounds) - 1):
+            self.check(f(n, x.shape[0], _arr(x), _arr(z), _arr(psi), psi.shape[0], int(bounds[i]),
+                         int(bounds[i + 1]), threads, _arr(hi), _arr(lo)))
+            if progress is not None:
+                progress(int(bounds[i + 1]), chunks)
+        return (hi + lo)[:K]
+
+    def expectation_grouped(self, n, x, z, c, psi, threads=1, progress=None, pieces=1):
+        """E = sum_t c_t <P_t> from key_values_grouped, the products summed
+        exactly rounded (math.fsum)."""
+        import math
+
+        x, z, c = _prep(n, x, z, c)
+        _, key = self.distinct_keys(n, x, z)
+        vals = self.key_values_grouped(n, x, z, psi, threads, progress, pieces)
+        tv = vals[key] if c.shape[0] else np.zeros(0)
+        return complex(math.fsum((c.real * tv).tolist()), math.fsum((c.imag * tv).tolist()))
 
 
 class Reference(_Lib):

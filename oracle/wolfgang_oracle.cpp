@@ -146,6 +146,172 @@ bool simplify_uses_hash_path(const Op& op) {
   return sampled_duplicate_rate(op, std::min<std::size_t>(op.T, 4096)) >= 0.25;
 }
 
+template <typename Fn>
+void parallel_for(std::size_t count, int threads, Fn fn) {
+  if (threads <= 1 || count <= 1) {
+    for (std::size_t i = 0; i < count; ++i) fn(i);
+    return;
+  }
+  std::atomic<std::size_t> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    pool.emplace_back([&]() {
+      for (std::size_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) fn(i);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// Worker threads for the oracle's sorts and products (orc_set_threads;
+// default 1). Any thread count gives the same result: the canonical order is
+// a total order (key, then construction index).
+int g_threads = 1;
+
+template <int W>
+struct KeyRec {
+  u64 k[2 * W];  // x0, z0, x1, z1, ...
+  u64 idx;
+  bool operator<(const KeyRec& o) const {
+    for (int i = 0; i < 2 * W; ++i) {
+      if (k[i] != o.k[i]) return k[i] < o.k[i];
+    }
+    return idx < o.idx;
+  }
+};
+
+// Sorts [0, n) of v with `threads` chunk sorts and pairwise merges.
+template <typename V>
+void parallel_sort(std::vector<V>& v, int threads) {
+  const std::size_t n = v.size();
+  const std::size_t parts = std::max<std::size_t>(1, std::min<std::size_t>(threads, n / 4096 + 1));
+  std::vector<std::size_t> bounds(parts + 1);
+  for (std::size_t i = 0; i <= parts; ++i) bounds[i] = n * i / parts;
+  {
+    std::vector<std::thread> pool;
+    for (std::size_t i = 0; i < parts; ++i) {
+      pool.emplace_back([&, i] { std::sort(v.begin() + bounds[i], v.begin() + bounds[i + 1]); });
+    }
+    for (auto& t : pool) t.join();
+  }
+  std::vector<V> tmp(n);
+  for (std::size_t width = 1; width < parts; width *= 2) {
+    std::vector<std::thread> pool;
+    for (std::size_t i = 0; i < parts; i += 2 * width) {
+      const std::size_t lo = bounds[i], mid = bounds[std::min(parts, i + width)],
+                        hi = bounds[std::min(parts, i + 2 * width)];
+      pool.emplace_back([&, lo, mid, hi] {
+        std::merge(v.begin() + lo, v.begin() + mid, v.begin() + mid, v.begin() + hi, tmp.begin() + lo);
+      });
+    }
+    for (auto& t : pool) t.join();
+    v.swap(tmp);
+  }
+}
+
+// Term indices in canonical key order, equal keys in construction order (a
+// stable sort by key), and run_start[i] = 1 where order[i] starts a new key.
+struct Canonical {
+  std::vector<std::size_t> order;
+  std::vector<char> run_start;
+};
+
+template <int W>
+Canonical canonical_order_w(const Op& op) {
+  const std::size_t T = op.T;
+  std::vector<KeyRec<W>> recs(T);
+  const std::size_t blocks = (T + 65535) / 65536;
+  parallel_for(blocks, g_threads, [&](std::size_t b) {
+    for (std::size_t t = b * 65536; t < std::min(T, (b + 1) * 65536); ++t) {
+      for (int w = 0; w < W; ++w) {
+        recs[t].k[2 * w] = op.x[t * W + w];
+        recs[t].k[2 * w + 1] = op.z[t * W + w];
+      }
+      recs[t].idx = t;
+    }
+  });
+  parallel_sort(recs, g_threads);
+  Canonical out;
+  out.order.resize(T);
+  out.run_start.resize(T);
+  parallel_for(blocks, g_threads, [&](std::size_t b) {
+    for (std::size_t i = b * 65536; i < std::min(T, (b + 1) * 65536); ++i) {
+      out.order[i] = recs[i].idx;
+      bool start = i == 0;
+      for (int k = 0; k < 2 * W && !start; ++k) start = recs[i].k[k] != recs[i - 1].k[k];
+      out.run_start[i] = start ? 1 : 0;
+    }
+  });
+  return out;
+}
+
+Canonical canonical_order(const Op& op) {
+  switch (op.W) {
+    case 1: return canonical_order_w<1>(op);
+    case 2: return canonical_order_w<2>(op);
+    case 3: return canonical_order_w<3>(op);
+    case 4: return canonical_order_w<4>(op);
+    default: break;
+  }
+  Canonical out;
+  out.order.resize(op.T);
+  std::iota(out.order.begin(), out.order.end(), std::size_t{0});
+  std::stable_sort(out.order.begin(), out.order.end(), [&](std::size_t a, std::size_t b) {
+    return key_cmp(op, a, b) < 0;
+  });
+  out.run_start.resize(op.T);
+  for (std::size_t i = 0; i < op.T; ++i) {
+    out.run_start[i] = (i == 0 || key_cmp(op, out.order[i - 1], out.order[i]) != 0) ? 1 : 0;
+  }
+  return out;
+}
+
+// Sums each run of equal keys of `src` (canonical order `can`) in
+// construction order, the accumulator seeded with the run's first
+// coefficient (seed_first) or with 0.0, and keeps the sums with
+// |sum| > threshold. Runs are split over threads at run starts and the pieces
+// concatenated in order, so the output does not depend on the thread count.
+Op reduce_runs(const Op& src, const Canonical& can, bool seed_first, double threshold) {
+  const std::size_t T = src.T, W = src.W;
+  const std::size_t parts = std::max<std::size_t>(1, std::min<std::size_t>(g_threads * 4, T / 65536 + 1));
+  std::vector<std::size_t> cut(parts + 1);
+  for (std::size_t i = 0; i <= parts; ++i) {
+    std::size_t c = T * i / parts;
+    while (c < T && c > 0 && !can.run_start[c]) ++c;
+    cut[i] = c;
+  }
+  cut[parts] = T;
+  std::vector<Op> piece(parts);
+  parallel_for(parts, g_threads, [&](std::size_t part) {
+    Op& out = piece[part];
+    std::size_t pos = cut[part];
+    while (pos < cut[part + 1]) {
+      std::size_t end = pos + 1;
+      while (end < T && !can.run_start[end]) ++end;
+      cplx acc = seed_first ? src.c[can.order[pos]] : cplx(0.0, 0.0);
+      for (std::size_t i = seed_first ? pos + 1 : pos; i < end; ++i) acc += src.c[can.order[i]];
+      if (std::abs(acc) > threshold) {
+        const std::size_t t = can.order[pos];
+        for (std::size_t w = 0; w < W; ++w) {
+          out.x.push_back(src.x[t * W + w]);
+          out.z.push_back(src.z[t * W + w]);
+        }
+        out.c.push_back(acc);
+      }
+      pos = end;
+    }
+  });
+  Op out;
+  out.n = src.n;
+  out.W = W;
+  for (const Op& pc : piece) {
+    out.x.insert(out.x.end(), pc.x.begin(), pc.x.end());
+    out.z.insert(out.z.end(), pc.z.begin(), pc.z.end());
+    out.c.insert(out.c.end(), pc.c.begin(), pc.c.end());
+  }
+  out.T = out.c.size();
+  return out;
+}
+
 Op simplify_impl(const Op& op, double atol, double rtol) {
   if (atol < 0.0 || rtol < 0.0 || !std::isfinite(atol) || !std::isfinite(rtol)) {
     throw std::invalid_argument("simplify tolerances must be non-negative finite values");
@@ -161,33 +327,11 @@ Op simplify_impl(const Op& op, double atol, double rtol) {
 
   // Distinct keys in canonical order, each with the construction-ordered list
   // of its terms.
-  std::vector<std::size_t> order(op.T);
-  std::iota(order.begin(), order.end(), std::size_t{0});
-  std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
-    return key_cmp(op, a, b) < 0;
-  });
-
+  const Canonical can = canonical_order(op);
+  // Hash path: the accumulator is seeded with the first coefficient
+  // (try_emplace) and later ones are added. Sort path: seeded with 0.0.
   const bool hash_path = simplify_uses_hash_path(op);
-  std::size_t pos = 0;
-  while (pos < order.size()) {
-    std::size_t end = pos + 1;
-    while (end < order.size() && key_cmp(op, order[pos], order[end]) == 0) ++end;
-    // Hash path: the accumulator is seeded with the first coefficient
-    // (try_emplace) and later ones are added. Sort path: seeded with 0.0.
-    cplx acc = hash_path ? op.c[order[pos]] : cplx(0.0, 0.0);
-    for (std::size_t i = hash_path ? pos + 1 : pos; i < end; ++i) acc += op.c[order[i]];
-    if (std::abs(acc) > threshold) {
-      const std::size_t t = order[pos];
-      for (std::size_t w = 0; w < op.W; ++w) {
-        out.x.push_back(op.x[t * op.W + w]);
-        out.z.push_back(op.z[t * op.W + w]);
-      }
-      out.c.push_back(acc);
-    }
-    pos = end;
-  }
-  out.T = out.c.size();
-  return out;
+  return reduce_runs(op, can, hash_path, threshold);
 }
 
 // ---------------------------------------------------------------- matmul
@@ -233,8 +377,8 @@ Op matmul_impl(const Op& lhs, const Op& rhs, bool simplify_output, std::size_t m
   prod.x.resize(P * W);
   prod.z.resize(P * W);
   prod.c.resize(P);
-  std::size_t p = 0;
-  for (std::size_t i = 0; i < lhs.T; ++i) {
+  parallel_for(lhs.T, g_threads, [&](std::size_t i) {
+    std::size_t p = i * rhs.T;
     for (std::size_t j = 0; j < rhs.T; ++j, ++p) {
       // k = |out_x & out_z| - |lx & lz| - |rx & rz| + 2 |lx & rz|
       // (reference src/multiply.cpp:279-283, :329-334, :368-382).
@@ -250,7 +394,7 @@ Op matmul_impl(const Op& lhs, const Op& rhs, bool simplify_output, std::size_t m
       }
       prod.c[p] = apply_phase(lhs.c[i] * rhs.c[j], k);
     }
-  }
+  });
   if (!simplify_output) return prod;
 
   // Reference src/multiply.cpp:107-261: for W in {1,2} with a duplicate-heavy
@@ -262,31 +406,7 @@ Op matmul_impl(const Op& lhs, const Op& rhs, bool simplify_output, std::size_t m
                          (operand_duplicate_heavy(lhs) || operand_duplicate_heavy(rhs));
   if (!hash_path) return simplify_impl(prod, 1.0e-12, 0.0);
 
-  std::vector<std::size_t> order(P);
-  std::iota(order.begin(), order.end(), std::size_t{0});
-  std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
-    return key_cmp(prod, a, b) < 0;
-  });
-  Op out;
-  out.n = prod.n;
-  out.W = W;
-  std::size_t pos = 0;
-  while (pos < P) {
-    std::size_t end = pos + 1;
-    while (end < P && key_cmp(prod, order[pos], order[end]) == 0) ++end;
-    cplx acc = prod.c[order[pos]];
-    for (std::size_t i = pos + 1; i < end; ++i) acc += prod.c[order[i]];
-    if (std::abs(acc) > 1.0e-12) {
-      for (std::size_t w = 0; w < W; ++w) {
-        out.x.push_back(prod.x[order[pos] * W + w]);
-        out.z.push_back(prod.z[order[pos] * W + w]);
-      }
-      out.c.push_back(acc);
-    }
-    pos = end;
-  }
-  out.T = out.c.size();
-  return out;
+  return reduce_runs(prod, canonical_order(prod), true, 1.0e-12);
 }
 
 // ---------------------------------------------------------------- commutation
@@ -490,21 +610,6 @@ double term_value_compensated(u64 xm, u64 zm, const cplx* psi, std::size_t N) {
   return acc.value();
 }
 
-template <typename Fn>
-void parallel_for(std::size_t count, int threads, Fn fn) {
-  if (threads <= 1 || count <= 1) {
-    for (std::size_t i = 0; i < count; ++i) fn(i);
-    return;
-  }
-  std::atomic<std::size_t> next{0};
-  std::vector<std::thread> pool;
-  for (int t = 0; t < threads; ++t) {
-    pool.emplace_back([&]() {
-      for (std::size_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) fn(i);
-    });
-  }
-  for (auto& th : pool) th.join();
-}
 
 template <typename Fn>
 int guarded(Fn fn) {
@@ -607,6 +712,10 @@ int orc_commutes_with(std::size_t n, std::size_t TL, const u64* lx, const u64* l
     }
   });
 }
+
+// Worker threads for simplify's sort and matmul's products (results do not
+// depend on it).
+void orc_set_threads(int threads) { g_threads = threads < 1 ? 1 : threads; }
 
 int orc_has_avx512_popcnt(void) { return cpu_has_avx512_popcnt() ? 1 : 0; }
 
@@ -861,3 +970,285 @@ int orc_group_commuting(std::size_t n, std::size_t T, const u64* x, const u64* z
 }
 
 }  // extern "C"
+
+namespace {
+
+// One x-group's pairs inside a chunk pair (see orc_key_values_accumulate):
+// representatives only (pivot bit clear), signs applied by flipping the sign
+// bit (no data-dependent branches), flushed every 256 representatives.
+template <typename Flush>
+__attribute__((target("popcnt"))) void sweep_group(const cplx* own, const cplx* other, u64 xl,
+                                                   u64 pivot, u64 mask, const u64* zl,
+                                                   const int* odd, std::size_t nk, double* blk,
+                                                   Flush& flush) {
+  const u64 low = pivot <= mask ? pivot - 1 : mask;
+  const u64 reps = pivot <= mask ? (mask + 1) >> 1 : mask + 1;
+  for (u64 r0 = 0; r0 < reps; r0 += 256) {
+    const u64 r1 = std::min<u64>(reps, r0 + 256);
+    for (u64 r = r0; r < r1; ++r) {
+      const u64 u = pivot <= mask ? (((r & ~low) << 1) | (r & low)) : r;
+      const cplx av = own[u], tv = other[u ^ xl];
+      const double re = tv.real() * av.real() + tv.imag() * av.imag();
+      const double im = tv.real() * av.imag() - tv.imag() * av.real();
+      for (std::size_t j = 0; j < nk; ++j) {
+        double v = odd[j] ? im : re;
+        u64 bits;
+        std::memcpy(&bits, &v, 8);
+        bits ^= static_cast<u64>(__builtin_popcountll(zl[j] & u) & 1) << 63;
+        std::memcpy(&v, &bits, 8);
+        blk[j] += v;
+      }
+    }
+    flush();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- checks at full size
+// Packed commutation bits (row-major, row_words = ceil(TR/64), bit j%64 of
+// word j/64 = entry (i, j)) from the row-threaded kernel above, so that a
+// 65,536^2 matrix is checked in 512 MiB of host memory instead of 4 GiB.
+int orc_commutes_packed(std::size_t n, std::size_t TL, const u64* lx, const u64* lz,
+                        std::size_t TR, const u64* rx, const u64* rz, int threads, u64* out) {
+  return guarded([&] {
+    std::vector<double> zl(2 * TL, 0.0), zr(2 * TR, 0.0);
+    const Op l = make_op(n, TL, lx, lz, zl.data());
+    const Op r = make_op(n, TR, rx, rz, zr.data());
+    const std::size_t row_words = (TR + 63) / 64;
+    const std::size_t chunk = 16;
+    const std::size_t blocks = (TL + chunk - 1) / chunk;
+    const bool simd = (l.W == 1 || l.W == 2) && cpu_has_avx512_popcnt();
+    parallel_for(blocks, threads, [&](std::size_t b) {
+      const std::size_t lo = b * chunk, hi = std::min(TL, lo + chunk);
+      Op sub;
+      sub.n = l.n;
+      sub.W = l.W;
+      sub.T = hi - lo;
+      sub.x.assign(l.x.begin() + lo * l.W, l.x.begin() + hi * l.W);
+      sub.z.assign(l.z.begin() + lo * l.W, l.z.begin() + hi * l.W);
+      std::vector<std::uint8_t> rows(sub.T * TR);
+      if (simd) {
+        commute_rows_avx512(sub, r, rows.data(), 0, sub.T);
+      } else {
+        commute_rows_scalar(sub, r, rows.data(), 0, sub.T);
+      }
+      for (std::size_t i = 0; i < sub.T; ++i) {
+        u64* dst = out + (lo + i) * row_words;
+        std::fill(dst, dst + row_words, 0ULL);
+        for (std::size_t j = 0; j < TR; ++j) dst[j / 64] |= static_cast<u64>(rows[i * TR + j]) << (j % 64);
+      }
+    });
+  });
+}
+
+// sigma = H psi on a subset of output rows only (as orc_apply, same term order
+// per row): out[i] = sigma[rows[i]].
+int orc_apply_rows(std::size_t n, std::size_t T, const u64* x, const u64* z, const double* c,
+                   const double* psi, std::size_t len, const u64* rows, std::size_t nrows,
+                   int threads, double* out) {
+  return guarded([&] {
+    if (n > 63) throw std::invalid_argument("expectation_statevector requires num_qubits <= 63");
+    if (len != (std::size_t{1} << n)) {
+      throw std::invalid_argument("expectation_statevector requires len(psi) == 2 ** num_qubits");
+    }
+    const std::size_t W = n == 0 ? 0 : (n + 63) / 64;
+    const cplx* p = reinterpret_cast<const cplx*>(psi);
+    cplx* s = reinterpret_cast<cplx*>(out);
+    parallel_for(nrows, threads, [&](std::size_t i) {
+      const std::size_t out_b = rows[i];
+      if (out_b >= len) throw std::invalid_argument("row index out of range");
+      cplx acc{0.0, 0.0};
+      for (std::size_t t = 0; t < T; ++t) {
+        const u64 xm = W ? x[t * W] : 0, zm = W ? z[t * W] : 0;
+        const std::size_t b = out_b ^ static_cast<std::size_t>(xm);
+        const bool odd = (std::popcount(zm & static_cast<u64>(b)) & 1U) != 0;
+        cplx v = apply_phase(p[b], std::popcount(xm & zm));
+        if (odd) v = -v;
+        acc += cplx(c[2 * t], c[2 * t + 1]) * v;
+      }
+      s[i] = acc;
+    });
+  });
+}
+
+// Distinct (x, z) keys of a one-word operator in first-appearance order:
+// key_of_term[t] = key id; returns the key count in *num_keys.
+int orc_distinct_keys(std::size_t n, std::size_t T, const u64* x, const u64* z,
+                      unsigned* key_of_term, std::size_t* num_keys) {
+  return guarded([&] {
+    if (n > 64) throw std::invalid_argument("distinct keys are formed for one-word operators");
+    const bool w = n != 0;
+    struct PairHash {
+      std::size_t operator()(const std::pair<u64, u64>& k) const noexcept {
+        return k.first * 0x9E3779B97F4A7C15ULL ^ (k.second + 0x632BE59BD9B4E019ULL);
+      }
+    };
+    std::unordered_map<std::pair<u64, u64>, unsigned, PairHash> idx;
+    for (std::size_t t = 0; t < T; ++t) {
+      auto [it, ins] = idx.try_emplace({w ? x[t] : 0, w ? z[t] : 0}, static_cast<unsigned>(idx.size()));
+      key_of_term[t] = it->second;
+    }
+    *num_keys = idx.size();
+  });
+}
+
+// Per-key values <P_k> = <psi|P_k|psi> of a one-word operator, accumulated
+// over the chunks [chunk_begin, chunk_end) of 2^C amplitudes each
+// (C = min(n, 16)) into caller-held double-double sums (key_hi, key_lo; keys
+// as numbered by orc_distinct_keys), so a caller can report progress between
+// calls. The evaluation is organised differently from the term-by-term
+// oracle above, so the two check each other:
+//  * diagonal keys (x = 0): per chunk beta, p(u) = |psi[beta 2^C + u]|^2 and its
+//    Walsh-Hadamard transform P(v) (pairwise butterflies); key z gains
+//    (-1)^{|(z >> C) & beta|} P(z mod 2^C);
+//  * off-diagonal keys: grouped by x, and the groups by h = x >> C; per chunk
+//    beta and h, one sweep of the pairs (b, b ^ x) whose representative b (pivot
+//    bit of x clear) lies in beta, reading the partner chunk beta ^ h, serves every
+//    key of every group with that h; per key the signed products
+//    (-1)^{|z & b|} conj(psi[b ^ x]) psi[b] are summed in blocks of 256 in double
+//    and the block sums enter the double-double accumulator
+//    (value = 2 Re(i^{|x & z|} S), reference src/expectation.cpp:137-152).
+__attribute__((target("popcnt"))) int orc_key_values_accumulate(
+    std::size_t n, std::size_t T, const u64* x, const u64* z, const double* psi, std::size_t len,
+    std::size_t chunk_begin, std::size_t chunk_end, int threads, double* key_hi, double* key_lo) {
+  return guarded([&] {
+    if (n > 40) throw std::invalid_argument("full-size key values support num_qubits <= 40");
+    if (len != (std::size_t{1} << n)) {
+      throw std::invalid_argument("expectation_statevector requires len(psi) == 2 ** num_qubits");
+    }
+    const bool w = n != 0;
+    const int C = static_cast<int>(std::min<std::size_t>(n, 16));
+    const u64 mask = (u64{1} << C) - 1;
+    const std::size_t chunks = len >> C;
+    if (chunk_end > chunks || chunk_begin > chunk_end) throw std::invalid_argument("chunk range out of bounds");
+    const cplx* p = reinterpret_cast<const cplx*>(psi);
+    // Keys in first-appearance order (as orc_distinct_keys numbers them).
+    struct PairHash {
+      std::size_t operator()(const std::pair<u64, u64>& k) const noexcept {
+        return k.first * 0x9E3779B97F4A7C15ULL ^ (k.second + 0x632BE59BD9B4E019ULL);
+      }
+    };
+    std::unordered_map<std::pair<u64, u64>, unsigned, PairHash> idx;
+    std::vector<u64> kx, kz;
+    for (std::size_t t = 0; t < T; ++t) {
+      const u64 xm = w ? x[t] : 0, zm = w ? z[t] : 0;
+      if (idx.try_emplace({xm, zm}, static_cast<unsigned>(kx.size())).second) {
+        kx.push_back(xm);
+        kz.push_back(zm);
+      }
+    }
+    const std::size_t K = kx.size();
+    std::vector<unsigned> diag;
+    std::unordered_map<u64, std::vector<unsigned>> by_x;
+    std::vector<u64> xs;
+    for (unsigned k = 0; k < K; ++k) {
+      if (kx[k] == 0) {
+        diag.push_back(k);
+      } else {
+        auto [it, ins] = by_x.try_emplace(kx[k]);
+        if (ins) xs.push_back(kx[k]);
+        it->second.push_back(k);
+      }
+    }
+    // Off-diagonal x-groups bucketed by their high part h, with the per-key
+    // constants of the sweep precomputed.
+    struct Group {
+      u64 xl, pivot;
+      std::vector<unsigned> keys;
+      std::vector<u64> zl, zh;
+      std::vector<int> odd;
+      std::vector<double> f;  // 2 Re(i^y S) = f * S_re (y even) or f * S_im (y odd)
+    };
+    std::unordered_map<u64, std::size_t> bucket_of;
+    std::vector<u64> hs;
+    std::vector<std::vector<Group>> buckets;
+    for (u64 xm : xs) {
+      auto [it, ins] = bucket_of.try_emplace(xm >> C, buckets.size());
+      if (ins) {
+        hs.push_back(xm >> C);
+        buckets.emplace_back();
+      }
+      Group g;
+      g.xl = xm & mask;
+      g.pivot = xm & (0 - xm);
+      for (unsigned k : by_x[xm]) {
+        const int y = std::popcount(xm & kz[k]) & 3;
+        g.keys.push_back(k);
+        g.zl.push_back(kz[k] & mask);
+        g.zh.push_back(kz[k] >> C);
+        g.odd.push_back(y & 1);
+        g.f.push_back((y == 0 || y == 3) ? 2.0 : -2.0);
+      }
+      buckets[it->second].push_back(std::move(g));
+    }
+    const int nt = std::max(1, threads);
+    std::vector<std::vector<DD>> acc(nt, std::vector<DD>(K));
+    std::atomic<std::size_t> next{chunk_begin};
+    auto worker = [&](int tid) {
+      std::vector<DD>& a = acc[tid];
+      std::vector<double> pw(std::size_t{1} << C);
+      std::vector<double> blk;
+      for (std::size_t beta = next.fetch_add(1); beta < chunk_end; beta = next.fetch_add(1)) {
+        const cplx* own = p + (beta << C);
+        if (!diag.empty()) {
+          for (std::size_t u = 0; u <= mask; ++u) {
+            pw[u] = own[u].real() * own[u].real() + own[u].imag() * own[u].imag();
+          }
+          for (std::size_t h = 1; h <= mask; h <<= 1) {
+            for (std::size_t i = 0; i <= mask; i += 2 * h) {
+              for (std::size_t j = i; j < i + h; ++j) {
+                const double a0 = pw[j], a1 = pw[j + h];
+                pw[j] = a0 + a1;
+                pw[j + h] = a0 - a1;
+              }
+            }
+          }
+          for (unsigned k : diag) {
+            const double v = pw[kz[k] & mask];
+            a[k].add((std::popcount((kz[k] >> C) & beta) & 1) ? -v : v);
+          }
+        }
+        for (std::size_t hb = 0; hb < hs.size(); ++hb) {
+          const cplx* other = p + ((beta ^ hs[hb]) << C);
+          for (const Group& g : buckets[hb]) {
+            const std::size_t nk = g.keys.size();
+            // The pivot bit lies in the chunk index: this chunk holds every
+            // representative of the group's pairs or none.
+            if (g.pivot > mask && ((static_cast<u64>(beta) << C) & g.pivot)) continue;
+            blk.assign(nk, 0.0);
+            auto flush = [&]() {
+              for (std::size_t j = 0; j < nk; ++j) {
+                const double cs = (std::popcount(g.zh[j] & beta) & 1) ? -g.f[j] : g.f[j];
+                a[g.keys[j]].add(cs * blk[j]);
+                blk[j] = 0.0;
+              }
+            };
+            sweep_group(own, other, g.xl, g.pivot, mask, g.zl.data(), g.odd.data(), nk, blk.data(),
+                        flush);
+          }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+    for (std::size_t k = 0; k < K; ++k) {
+      DD tot;
+      tot.hi = key_hi[k];
+      tot.lo = key_lo[k];
+      for (int t = 0; t < nt; ++t) {
+        tot.add(acc[t][k].hi);
+        tot.add(acc[t][k].lo);
+      }
+      key_hi[k] = tot.hi;
+      key_lo[k] = tot.lo;
+    }
+  });
+}
+
+}  // extern "C"
+

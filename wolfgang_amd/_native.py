@@ -66,6 +66,7 @@ SIGNATURES = {
         C.c_int, [_vp, C.POINTER(_size), C.POINTER(_size), C.POINTER(_size), C.POINTER(C.c_int), _vp]
     ),
     "wg_debug_plan_passes": (C.c_int, [_size, _size, _vp, _vp, _vp, _size, C.POINTER(_size)]),
+    "wg_debug_key_sweeps": (C.c_int, [_size, _size, _vp, _vp, _vp, _vp, C.POINTER(_size)]),
     "wg_debug_tiled_plan": (
         C.c_int, [_size, _size, _vp, _vp, _vp, _vp, C.POINTER(_size), C.POINTER(_size)]
     ),

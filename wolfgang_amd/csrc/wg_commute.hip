@@ -133,58 +133,101 @@ __global__ __launch_bounds__(kBlock) void commute_packed_kernel(
   }
 }
 
-// Dense uint8 view: out[i*R + j] = bit (i, j).
+// Dense uint8 view: out[i*cols + j] = bit (i, j). A block covers kUnpackBytes
+// consecutive columns of one row (one 64-bit division per block, none per
+// byte); each thread expands 16 bits of one packed word into 16 bytes.
+constexpr unsigned kUnpackBytes = 256 * 16;
+__device__ inline u32 spread4(u32 v) { return ((v & 0xFU) * 0x00204081U) & 0x01010101U; }
+
 __global__ __launch_bounds__(256) void unpack_u8_kernel(const u64* __restrict__ bits,
-                                                        std::size_t rows, std::size_t cols,
-                                                        std::size_t row_words,
+                                                        std::size_t cols, std::size_t row_words,
+                                                        std::size_t chunks_per_row,
                                                         std::uint8_t* __restrict__ out) {
-  const std::size_t e = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
-  const std::size_t total = rows * cols;
-  for (std::size_t i = e; i < total; i += stride) {
-    const std::size_t r = i / cols, c = i % cols;
-    out[i] = static_cast<std::uint8_t>((bits[r * row_words + c / 64] >> (c % 64)) & 1U);
+  const std::size_t r = blockIdx.x / chunks_per_row;
+  const std::size_t c0 =
+      (blockIdx.x - r * chunks_per_row) * kUnpackBytes + static_cast<std::size_t>(threadIdx.x) * 16;
+  if (c0 >= cols) return;
+  // c0 is a multiple of 16, so its 16 columns share one packed word.
+  const u32 v = static_cast<u32>(bits[r * row_words + c0 / 64] >> (c0 % 64)) & 0xFFFFU;
+  std::uint8_t* dst = out + r * cols + c0;
+  if (c0 + 16 <= cols && (reinterpret_cast<std::uintptr_t>(dst) & 15U) == 0) {
+    *reinterpret_cast<uint4*>(dst) =
+        make_uint4(spread4(v), spread4(v >> 4), spread4(v >> 8), spread4(v >> 12));
+  } else {
+    const std::size_t n = cols - c0 < 16 ? cols - c0 : 16;
+    for (std::size_t j = 0; j < n; ++j) dst[j] = static_cast<std::uint8_t>((v >> j) & 1U);
   }
 }
 
-// One wave per row: popcount of the row's words; optional per-row output and
-// a total accumulated with integer atomics (order-independent, so exact).
+// Row counts: one wave per row (grid-stride over rows), popcount of the row's
+// words; optional per-row output. The total is reduced per block and added
+// with one integer atomic per block (exact in any order).
+constexpr unsigned kCountBlocks = 2048;
 __global__ __launch_bounds__(256) void count_rows_kernel(const u64* __restrict__ bits,
                                                          std::size_t rows, std::size_t row_words,
                                                          unsigned long long* row_counts,
                                                          unsigned long long* total) {
-  const std::size_t wave_global =
-      (static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
-  const unsigned lane = threadIdx.x & 63;
-  if (wave_global >= rows) return;
-  unsigned long long c = 0;
-  for (std::size_t w = lane; w < row_words; w += 64) c += __popcll(bits[wave_global * row_words + w]);
-  for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (lane == 0) {
-    if (row_counts) row_counts[wave_global] = c;
-    if (total) atomicAdd(total, c);
+  __shared__ unsigned long long wave_sum[4];
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long mine = 0;
+  for (std::size_t row = static_cast<std::size_t>(blockIdx.x) * 4 + wave; row < rows;
+       row += static_cast<std::size_t>(gridDim.x) * 4) {
+    unsigned long long c = 0;
+    for (std::size_t w = lane; w < row_words; w += 64) c += __popcll(bits[row * row_words + w]);
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0 && row_counts) row_counts[row] = c;
+    mine += c;
+  }
+  if (!total) return;
+  if (lane == 0) wave_sum[wave] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long s = wave_sum[0] + wave_sum[1] + wave_sum[2] + wave_sum[3];
+    if (s) atomicAdd(total, s);
   }
 }
 
-// Column counts: thread per column over a chunk of rows; the 64 lanes of a wave
-// share one packed word per row (broadcast load).
-constexpr std::size_t kColRowChunk = 2048;
-__global__ __launch_bounds__(256) void count_cols_kernel(const u64* __restrict__ bits,
-                                                         std::size_t rows, std::size_t cols,
-                                                         std::size_t row_words,
-                                                         std::size_t col_blocks,
-                                                         unsigned long long* col_counts) {
-  const std::size_t cb = blockIdx.x % col_blocks;
-  const std::size_t chunk = blockIdx.x / col_blocks;
-  const std::size_t col = cb * blockDim.x + threadIdx.x;
-  if (col >= cols) return;
+// Column counts: thread per packed word column over a chunk of rows, with the
+// 64 per-bit counters of its word in registers (one coalesced 8-byte load per
+// row and thread); one atomic per nonzero counter at the end.
+constexpr std::size_t kColRowChunk = 1024;
+__global__ __launch_bounds__(64) void count_cols_kernel(const u64* __restrict__ bits,
+                                                        std::size_t rows, std::size_t cols,
+                                                        std::size_t row_words,
+                                                        std::size_t word_groups,
+                                                        unsigned long long* col_counts) {
+  const std::size_t wg = blockIdx.x % word_groups;
+  const std::size_t chunk = blockIdx.x / word_groups;
+  const std::size_t word = wg * 64 + threadIdx.x;
+  if (word >= row_words) return;
   const std::size_t r0 = chunk * kColRowChunk;
   const std::size_t r1 = (r0 + kColRowChunk) < rows ? (r0 + kColRowChunk) : rows;
-  const std::size_t word = col / 64;
-  const unsigned bit = col % 64;
-  unsigned long long c = 0;
-  for (std::size_t r = r0; r < r1; ++r) c += (bits[r * row_words + word] >> bit) & 1U;
-  if (c) atomicAdd(&col_counts[col], c);
+  u32 cnt[64];
+#pragma unroll
+  for (int b = 0; b < 64; ++b) cnt[b] = 0;
+  for (std::size_t r = r0; r < r1; ++r) {
+    const u64 v = bits[r * row_words + word];
+    const u32 lo = static_cast<u32>(v), hi = static_cast<u32>(v >> 32);
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      cnt[b] += (lo >> b) & 1U;
+      cnt[32 + b] += (hi >> b) & 1U;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 64; ++b) {
+    const std::size_t col = word * 64 + b;
+    if (cnt[b] && col < cols) atomicAdd(&col_counts[col], static_cast<unsigned long long>(cnt[b]));
+  }
+}
+
+void launch_unpack(const u64* bits, std::size_t rows, std::size_t cols, std::size_t row_words,
+                   std::uint8_t* dense, hipStream_t stream) {
+  const std::size_t chunks = (cols + kUnpackBytes - 1) / kUnpackBytes;
+  ProfileScope prof("unpack_u8", stream);
+  unpack_u8_kernel<<<grid_for(checked_mul(rows, chunks, "unpack grid"), 1), 256, 0, stream>>>(
+      bits, cols, row_words, chunks, dense);
+  check_launch("unpack_u8");
 }
 
 void check_entries(std::size_t L, std::size_t R, std::size_t max_entries) {
@@ -266,8 +309,7 @@ int wg_commutes_with_host(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, uint
     u64* bits = scratch.take<u64>(checked_mul(L, row_words, "commutation matrix"));
     std::uint8_t* dense = scratch.take<std::uint8_t>(L * R);
     launch_commute(lhs, rhs, bits, row_words, stream);
-    unpack_u8_kernel<<<grid_for(L * R, 256 * 8), 256, 0, stream>>>(bits, L, R, row_words, dense);
-    check_launch("unpack_u8");
+    launch_unpack(bits, L, R, row_words, dense, stream);
     check_hip(hipMemcpyAsync(out, dense, L * R, hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
     sync(stream);
@@ -353,9 +395,7 @@ int wg_comm_matrix_to_host_u8(const wg_comm_matrix* m, uint8_t* out, size_t out_
     hipStream_t stream = as_stream(stream_ptr);
     Scratch scratch(m->device);
     std::uint8_t* dense = scratch.take<std::uint8_t>(entries);
-    unpack_u8_kernel<<<grid_for(entries, 256 * 8), 256, 0, stream>>>(m->bits, m->rows, m->cols,
-                                                                    m->row_words, dense);
-    check_launch("unpack_u8");
+    launch_unpack(m->bits, m->rows, m->cols, m->row_words, dense, stream);
     check_hip(hipMemcpyAsync(out, dense, entries, hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
     sync(stream);
@@ -392,14 +432,19 @@ int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint6
     unsigned long long* dev = scratch.take<unsigned long long>(want);
     check_hip(hipMemsetAsync(dev, 0, want * sizeof(unsigned long long), stream), "clear counts");
     if (axis == 0) {
-      const std::size_t col_blocks = (m->cols + 255) / 256;
+      const std::size_t groups = (m->row_words + 63) / 64;
       const std::size_t chunks = (m->rows + kColRowChunk - 1) / kColRowChunk;
-      count_cols_kernel<<<grid_for(col_blocks * chunks, 1), 256, 0, stream>>>(
-          m->bits, m->rows, m->cols, m->row_words, col_blocks, dev);
+      ProfileScope prof("count_cols", stream);
+      count_cols_kernel<<<grid_for(groups * chunks, 1), 64, 0, stream>>>(
+          m->bits, m->rows, m->cols, m->row_words, groups, dev);
       check_launch("count_cols");
     } else {
-      count_rows_kernel<<<grid_for(m->rows * 64, 256), 256, 0, stream>>>(
-          m->bits, m->rows, m->row_words, axis == 1 ? dev : nullptr, axis == -1 ? dev : nullptr);
+      const unsigned grid =
+          static_cast<unsigned>(std::min<std::size_t>((m->rows + 3) / 4, kCountBlocks));
+      ProfileScope prof("count_rows", stream);
+      count_rows_kernel<<<grid, 256, 0, stream>>>(m->bits, m->rows, m->row_words,
+                                                  axis == 1 ? dev : nullptr,
+                                                  axis == -1 ? dev : nullptr);
       check_launch("count_rows");
     }
     check_hip(hipMemcpyAsync(out, dev, want * sizeof(u64), hipMemcpyDeviceToHost, stream),

@@ -2821,29 +2821,54 @@ int wg_expectation_plan_info(const wg_pauli_sum* op_c, size_t* distinct_keys, si
 // up to max_passes rows of {k, items, item_groups, tile_blocks, diagonal
 // items, off-diagonal items, owner bundles, owner slots} and the pass count.
 constexpr int kDebugPassColumns = 8;
+}  // extern "C"
+
+namespace wg {
+namespace {
+
+// Work items of a W = 1 operator given as host masks, as the device plan
+// forms them: distinct keys numbered in first-appearance order, grouped by x
+// in first-appearance order of x, kZ keys per item. *num_keys gets the count.
+std::vector<ExpItem> debug_items(std::size_t num_terms, const uint64_t* x, const uint64_t* z,
+                                 std::size_t* num_keys) {
+  std::unordered_map<u64, std::vector<std::pair<u64, u32>>> zs_of;
+  std::vector<u64> order_x;
+  std::unordered_map<u64, std::unordered_map<u64, u32>> seen;
+  u32 keys = 0;
+  for (std::size_t t = 0; t < num_terms; ++t) {
+    if (seen[x[t]].emplace(z[t], keys).second) {
+      if (zs_of.find(x[t]) == zs_of.end()) order_x.push_back(x[t]);
+      zs_of[x[t]].push_back({z[t], keys++});
+    }
+  }
+  std::vector<ExpItem> items;
+  for (u64 gx : order_x) {
+    const auto& zs = zs_of[gx];
+    for (std::size_t s = 0; s < zs.size(); s += kZ) {
+      ExpItem it{};
+      it.x = gx;
+      it.nz = static_cast<int>(std::min<std::size_t>(kZ, zs.size() - s));
+      for (int j = 0; j < it.nz; ++j) {
+        it.z[j] = zs[s + j].first;
+        it.key[j] = zs[s + j].second;
+        it.y[j] = __builtin_popcountll(gx & zs[s + j].first) & 3;
+      }
+      items.push_back(it);
+    }
+  }
+  if (num_keys) *num_keys = keys;
+  return items;
+}
+
+}  // namespace
+}  // namespace wg
+
+extern "C" {
+
 int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                          int* rows, size_t max_passes, size_t* num_passes) {
   return guard([&] {
-    std::unordered_map<u64, std::vector<u64>> zs_of;
-    std::vector<u64> order_x;
-    std::unordered_map<u64, std::unordered_map<u64, int>> seen;
-    for (std::size_t t = 0; t < num_terms; ++t) {
-      if (seen[x[t]].emplace(z[t], 1).second) {
-        if (zs_of.find(x[t]) == zs_of.end()) order_x.push_back(x[t]);
-        zs_of[x[t]].push_back(z[t]);
-      }
-    }
-    std::vector<ExpItem> items;
-    for (u64 gx : order_x) {
-      const auto& zs = zs_of[gx];
-      for (std::size_t s = 0; s < zs.size(); s += kZ) {
-        ExpItem it{};
-        it.x = gx;
-        it.nz = static_cast<int>(std::min<std::size_t>(kZ, zs.size() - s));
-        for (int j = 0; j < it.nz; ++j) it.z[j] = zs[s + j];
-        items.push_back(it);
-      }
-    }
+    std::vector<ExpItem> items = debug_items(num_terms, x, z, nullptr);
     std::vector<PassLaunch> passes;
     std::size_t diag_keys = 0;
     for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
@@ -2869,6 +2894,53 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
       row[5] = static_cast<int>(passes[p].num_items) - diag;
       row[6] = static_cast<int>(passes[p].owner_count);
       row[7] = slots;
+    }
+  });
+}
+
+// Host-only: where each distinct key (first-appearance order) of a W = 1
+// operator is evaluated by the tiled plan: key_pass[k] = the pass holding it
+// (-1: the diagonal Walsh-Hadamard sweep), key_kind[k] = the kind word of the
+// first owner bundle holding it (bit 0 odd |x & z|, bit 1 lane pivot, bits
+// 2.. shape; -1 when no owner bundle holds it). Both arrays hold num_terms
+// entries; *num_keys gets the key count.
+int wg_debug_key_sweeps(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        int* key_pass, int* key_kind, size_t* num_keys) {
+  return guard([&] {
+    std::size_t K = 0;
+    std::vector<ExpItem> items = debug_items(num_terms, x, z, &K);
+    *num_keys = K;
+    for (std::size_t k = 0; k < K; ++k) {
+      key_pass[k] = -1;
+      key_kind[k] = -1;
+    }
+    std::vector<PassLaunch> passes;
+    std::size_t diag_keys = 0;
+    for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
+    u32 diag_front = 0;
+    plan_tiled_passes(num_qubits, items, passes,
+                      diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ, &diag_front,
+                      SIZE_MAX);
+    std::vector<OwnerBundle> bundles;
+    if (kOwnerSweep) plan_owner_bundles(items, passes, bundles);
+    for (std::size_t p = 0; p < passes.size(); ++p) {
+      const PassLaunch& pl = passes[p];
+      for (u32 i = 0; i < pl.num_items; ++i) {
+        const ExpItem& it = items[pl.first_item + i];
+        for (int j = 0; j < it.nz; ++j) key_pass[it.key[j]] = static_cast<int>(p);
+      }
+      for (u32 b = pl.owner_first; b < pl.owner_first + pl.owner_count; ++b) {
+        for (int sl = 0; sl < kOwnerMaxSlots; ++sl) {
+          const OwnerSlot& slot = bundles[b].slot[sl];
+          bool used = false;
+          for (int kk = 0; kk < kOwnerMaxKeys; ++kk) used |= (slot.key[kk].info & (1U << 15)) != 0;
+          if (!used || slot.item >= pl.num_items) continue;
+          const ExpItem& it = items[pl.first_item + slot.item];
+          for (int j = 0; j < it.nz; ++j) {
+            if (key_kind[it.key[j]] < 0) key_kind[it.key[j]] = static_cast<int>(bundles[b].kind);
+          }
+        }
+      }
     }
   });
 }
