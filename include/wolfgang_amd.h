@@ -140,6 +140,21 @@ int wg_comm_matrix_count(const wg_comm_matrix* m, int axis, int conflicts, uint6
 int wg_expectation(const wg_pauli_sum* op, const void* psi, int dtype, size_t length,
                    int psi_on_device, void* stream, double* out2);
 
+/* Allocation-free variant of wg_expectation. No reference counterpart: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). psi is a
+ * device complex128 statevector on op's device (length 2^n); out2_device is a
+ * 16-byte aligned DEVICE location receiving (Re E, Im E). The operator's
+ * expectation plan must already be built (wg_expectation_plan_info or any
+ * expectation call builds it; WG_ERR_INVALID otherwise). Partial sums use a
+ * per-device static scratch of 256 MiB (held until the process ends); a plan
+ * needing more is rejected with WG_ERR_INVALID. It calls no hipMalloc/hipFree,
+ * enqueues on `stream` and returns without synchronizing. Calls that overlap
+ * on one device share that device's scratch and race. Results are
+ * bit-identical to wg_expectation's. */
+int wg_expectation_noalloc(const wg_pauli_sum* op, const double* psi, size_t length,
+                           double* out2_device, void* stream);
+
 /* expectation_z_counts (reference include/wolfgang/pauli_sum.hpp:98,
  * src/expectation.cpp:157-247; binding bindings/python/pauli_sum_py.cpp:1614-1631):
  * E = sum_t c_t * sum_r counts[r] (-1)^{|z_t & b_r|} / sum_r counts[r] over

@@ -61,6 +61,7 @@ SIGNATURES = {
     "wg_comm_matrix_count": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _size, _vp]),
     "wg_expectation": (C.c_int, [_vp, _vp, C.c_int, _size, C.c_int, _vp, _dblp]),
     "wg_expectation_adjoint": (C.c_int, [_vp, _vp, _size, _vp, _vp, _vp, _vp, _vp, _dblp]),
+    "wg_expectation_noalloc": (C.c_int, [_vp, _vp, _size, _vp, _vp]),
     "wg_expectation_z_counts": (C.c_int, [_vp, C.c_char_p, _vp, _vp, _size, _vp, _dblp]),
     "wg_expectation_plan_info": (
         C.c_int, [_vp, C.POINTER(_size), C.POINTER(_size), C.POINTER(_size), C.POINTER(C.c_int), _vp]

@@ -2565,9 +2565,30 @@ namespace {
 void run_forward(const double2* coeffs, const ExpectationPlan& plan, const double2* psi,
                  double2* term_values, double2* out_dev, Scratch& scratch, hipStream_t stream) {
   double* key_vals = scratch.take<double>(std::max<std::size_t>(plan.num_keys, 1));
+  double2* partials =
+      plan.num_items
+          ? scratch.take<double2>(checked_mul(plan.partial_slots, kZ, "expectation partials"))
+          : nullptr;
+  forward_into(coeffs, plan, psi, term_values, out_dev, key_vals, partials, stream);
+}
+
+std::size_t validate_psi(const wg_pauli_sum* op, const void* psi, std::size_t length);
+
+}  // namespace
+
+std::size_t forward_scratch_bytes(const ExpectationPlan& plan) {
+  const std::size_t keys = (std::max<std::size_t>(plan.num_keys, 1) * sizeof(double) + 255) & ~std::size_t{255};
+  return keys + (plan.num_items ? checked_mul(checked_mul(plan.partial_slots, kZ, "expectation partials"),
+                                              sizeof(double2), "expectation partials")
+                                : 0);
+}
+
+// Every partial row and key value the reduce and finalize kernels read is
+// written earlier in the same sequence, so the buffers need no clearing.
+void forward_into(const double2* coeffs, const ExpectationPlan& plan, const double2* psi,
+                  double2* term_values, double2* out_dev, double* key_vals, double2* partials,
+                  hipStream_t stream) {
   if (plan.num_items) {
-    double2* partials =
-        scratch.take<double2>(checked_mul(plan.partial_slots, kZ, "expectation partials"));
     if (plan.tiled) {
       for (u32 k0 = 0; k0 < plan.diag_keys; k0 += kWhtMaxKeys) {
         ProfileScope prof("expect_diag_wht", stream);
@@ -2608,6 +2629,8 @@ void run_forward(const double2* coeffs, const ExpectationPlan& plan, const doubl
                                                  term_values, out_dev);
   check_launch("expect_finalize");
 }
+
+namespace {
 
 std::size_t validate_psi(const wg_pauli_sum* op, const void* psi, std::size_t length) {
   if (op->num_qubits > 63) {

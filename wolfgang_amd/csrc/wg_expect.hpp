@@ -197,6 +197,18 @@ struct ExpectationPlan {
 
 std::shared_ptr<ExpectationPlan> get_plan(wg_pauli_sum* op, hipStream_t stream);
 
+// Device bytes the forward sequence of `plan` needs beside its inputs and
+// outputs: per-key values (256-byte aligned) followed by the partial sums.
+std::size_t forward_scratch_bytes(const ExpectationPlan& plan);
+
+// Enqueues the forward sequence (sweeps, ordered reduction, finalize) of a
+// plan on `stream`: E into the device location out_dev, per-term values when
+// term_values is non-null; key_vals and partials are the caller's buffers laid
+// out as forward_scratch_bytes describes. No allocation, no synchronization.
+void forward_into(const double2* coeffs, const ExpectationPlan& plan, const double2* psi,
+                  double2* term_values, double2* out_dev, double* key_vals, double2* partials,
+                  hipStream_t stream);
+
 // Forward + adjoint of one operator with at least one term (the work of
 // wg_expectation_adjoint), enqueued on `stream`: E goes to the device location
 // `dout`, per-term values and sigma / sigma^dagger where their pointers are
