@@ -176,7 +176,11 @@ int wg_expectation_z_counts(const wg_pauli_sum* op, const char* chars, const siz
  *   term_values     <- <psi|P_k|psi> per term (num_terms complex128; real for
  *                      Hermitian P_k), i.e. dE/dc_k          (may be NULL)
  *   sigma           <- H psi (length complex128), i.e. dE/dpsi*  (may be NULL)
- *   sigma_dagger    <- H^dagger psi (coefficients conjugated)    (may be NULL) */
+ *   sigma_dagger    <- H^dagger psi (coefficients conjugated)    (may be NULL)
+ * out2 may be NULL when E is not wanted. For states swept in tiles (18 qubits
+ * and up) E and the per-term values come from the forward sweeps and sigma /
+ * sigma^dagger from one sigma sweep over the same passes; with out2 and
+ * term_values both NULL only the sigma sweep runs. */
 int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t length,
                            const double* coeffs, double* term_values, double* sigma,
                            double* sigma_dagger, void* stream, double* out2);
@@ -266,6 +270,16 @@ int wg_debug_plan_passes(size_t num_qubits, size_t num_terms, const uint64_t* x,
  * entries; *num_keys gets the key count. */
 int wg_debug_key_sweeps(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
                         int* key_pass, int* key_kind, size_t* num_keys);
+
+/* Host-only replay of the tiled sigma sweep of a W = 1 operator with
+ * 18 <= n <= 24 (test hook; no reference counterpart): builds the plan's items,
+ * passes and sigma key records and evaluates them on the CPU over psi
+ * (complex128, 2^n), writing sigma = H psi and H^dagger psi (2^n complex128
+ * each). coeffs holds 2*num_terms doubles. *num_records gets the number of key
+ * records (padding included). */
+int wg_debug_sigma_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* coeffs, const double* psi, double* sigma, double* sigma_dag,
+                        size_t* num_records);
 
 /* Kernel timing with HIP events (no reference counterpart; measurement only).
  * While enabled, the library brackets each launch of its hot kernels

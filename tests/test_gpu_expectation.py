@@ -193,3 +193,48 @@ def test_tiled_sweep_matches_oracle(gpu, orc, gen, n, terms):
     want = orc.expectation_compensated(*packed(op), psi, threads=8)
     assert abs(got - want) <= RTOL * scale(op, psi)
     assert dop.expectation_statevector(psi) == got
+
+
+@pytest.mark.parametrize("variant", ["sigma_values", "pair_no_energy", "sigma_no_energy", "long_pass"])
+def test_tiled_adjoint_variants(gpu, orc, variant):
+    """The tiled adjoint in each of its forms: sigma with E and the per-term
+    values (forward sweeps, then the sigma sweep), sigma and sigma^dagger with
+    no energy wanted (the sigma sweep alone), sigma alone with no energy, and a
+    pass of 300 keys on one x-mask. Repeated calls give identical bits."""
+    import torch
+
+    from test_sigma_plan import one_x_many_z
+
+    n = 18
+    op = one_x_many_z(n, 300, 84) if variant == "long_pass" else random_pauli_masks(n, 500, 0.3, 91)
+    psi = random_statevector(n, 92)
+    dop = op.to_device()
+    assert dop.expectation_plan_info()["tiled"]
+    t = torch.from_numpy(psi).to("cuda:0")
+    sigma, sigma_dag = torch.empty_like(t), torch.empty_like(t)
+    tv = torch.empty(op.num_terms, dtype=torch.complex128, device="cuda:0")
+    pair = variant in ("pair_no_energy", "long_pass")
+    values = variant in ("sigma_values", "long_pass")
+
+    def call():
+        return dop.expectation_adjoint(t.data_ptr(), t.numel(), tv.data_ptr() if values else None,
+                                       sigma.data_ptr(), sigma_dag.data_ptr() if pair else None,
+                                       want_energy=values)
+
+    e = call()
+    s = scale(op, psi)
+    np.testing.assert_allclose(sigma.cpu().numpy(), orc.apply(*packed(op), psi, threads=8), rtol=0, atol=RTOL * s)
+    if pair:
+        want_dag = orc.apply(n, op.x_words(), op.z_words(), op.coeffs().conj(), psi, threads=8)
+        np.testing.assert_allclose(sigma_dag.cpu().numpy(), want_dag, rtol=0, atol=RTOL * s)
+    if values:
+        want_tv = orc.term_values(n, op.x_words(), op.z_words(), psi, threads=8)
+        np.testing.assert_allclose(tv.cpu().numpy().real, want_tv, rtol=0, atol=RTOL)
+        assert abs(e - orc.expectation_compensated(*packed(op), psi, threads=8)) <= RTOL * s
+    else:
+        assert e is None
+    first = (sigma.clone(), tv.clone())
+    e2 = call()
+    assert torch.equal(sigma, first[0]) and e2 == e
+    if values:
+        assert torch.equal(tv, first[1])

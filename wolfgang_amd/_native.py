@@ -68,6 +68,7 @@ SIGNATURES = {
     ),
     "wg_debug_plan_passes": (C.c_int, [_size, _size, _vp, _vp, _vp, _size, C.POINTER(_size)]),
     "wg_debug_key_sweeps": (C.c_int, [_size, _size, _vp, _vp, _vp, _vp, C.POINTER(_size)]),
+    "wg_debug_sigma_plan": (C.c_int, [_size, _size, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_size)]),
     "wg_debug_tiled_plan": (
         C.c_int, [_size, _size, _vp, _vp, _vp, _vp, C.POINTER(_size), C.POINTER(_size)]
     ),

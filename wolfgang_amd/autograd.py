@@ -5,13 +5,18 @@ and is differentiable in ``psi`` (complex128, on the operator's device) and in
 ``coeffs`` (complex128 or float64, one per term of ``op``; when omitted the
 operator's own coefficients are used and only ``psi`` is differentiated).
 
-Backward runs the library's adjoint entry point once: it returns the per-term
-values <P_k> (dE/dc_k) and sigma = H psi (dE/dpsi*), plus H^dagger psi when the
-coefficients are complex. With PyTorch's convention for complex gradients
-(grad = dL/dRe + i dL/dIm for a real loss L) and grad_output g for E:
+With PyTorch's convention for complex gradients (grad = dL/dRe + i dL/dIm for a
+real loss L) and grad_output g for E:
 
-    grad_psi = conj(g) * (H psi) + g * (H^dagger psi)
-    grad_c   = g * conj(<P>)          (real part for real coefficients)
+    grad_psi = conj(g) * (H psi) + g * (H^dagger psi) = H_d psi,
+               H_d = sum_k d_k P_k with real d_k = 2 Re(conj(g) c_k)
+    grad_c   = g * <P_k>               (real part for real coefficients)
+
+so the backward is one application of H_d: one sigma sweep with real
+coefficients (no H^dagger psi, no per-term values), and the per-term values
+<P_k> are produced by the forward sweep and saved when the coefficients need a
+gradient. Forward plus backward therefore costs one forward evaluation and one
+sigma sweep.
 
 PyTorch is used only here, at the Python boundary; the arithmetic runs in the
 HIP kernels behind the C ABI.
@@ -19,6 +24,7 @@ HIP kernels behind the C ABI.
 
 from __future__ import annotations
 
+import weakref
 from typing import Any
 
 import torch
@@ -42,6 +48,19 @@ def _check_psi(op: DevicePauliSum, psi: torch.Tensor) -> torch.Tensor:
     return psi.contiguous()
 
 
+_OWN_COEFFS: "weakref.WeakKeyDictionary[DevicePauliSum, torch.Tensor]" = weakref.WeakKeyDictionary()
+
+
+def _own_coeffs(op: DevicePauliSum, device: torch.device) -> torch.Tensor:
+    """The operator's own coefficients as a device tensor (copied once per
+    operator and kept while the operator lives)."""
+    cached = _OWN_COEFFS.get(op)
+    if cached is None or cached.device != device:
+        cached = torch.from_numpy(op.to_host().coeffs()).to(device)
+        _OWN_COEFFS[op] = cached
+    return cached
+
+
 class PauliExpectation(torch.autograd.Function):
     @staticmethod
     def forward(ctx: Any, psi: torch.Tensor, coeffs: torch.Tensor | None, op: DevicePauliSum) -> torch.Tensor:
@@ -51,47 +70,33 @@ class PauliExpectation(torch.autograd.Function):
             if coeffs.shape != (op.num_terms,):
                 raise ValueError("coeffs must have one entry per term")
             cplx = coeffs.detach().to(device=psi.device, dtype=torch.complex128).contiguous()
+        want_tv = coeffs is not None and coeffs.requires_grad
+        tv = torch.empty(op.num_terms, dtype=torch.complex128, device=psi.device) if want_tv else None
         e = op.expectation_adjoint(
-            psi.data_ptr(), psi.numel(), None, None, None, stream=_stream(psi.device),
-            coeffs_ptr=cplx.data_ptr() if cplx is not None else None,
+            psi.data_ptr(), psi.numel(), tv.data_ptr() if tv is not None else None, None, None,
+            stream=_stream(psi.device), coeffs_ptr=cplx.data_ptr() if cplx is not None else None,
         )
         ctx.op = op
         ctx.coeffs_given = coeffs is not None
         ctx.real_coeffs = coeffs is not None and not coeffs.is_complex()
-        ctx.save_for_backward(psi, cplx if cplx is not None else torch.empty(0, device=psi.device))
+        empty = torch.empty(0, device=psi.device)
+        ctx.save_for_backward(psi, cplx if cplx is not None else empty, tv if tv is not None else empty)
         return torch.tensor(complex(e), dtype=torch.complex128, device=psi.device)
 
     @staticmethod
     def backward(ctx: Any, grad: torch.Tensor):
-        psi, cplx = ctx.saved_tensors
+        psi, cplx, tv = ctx.saved_tensors
         op: DevicePauliSum = ctx.op
         need_psi = ctx.needs_input_grad[0]
         need_c = ctx.coeffs_given and ctx.needs_input_grad[1]
-        if not (need_psi or need_c):
-            return None, None, None
-        sigma = torch.empty_like(psi) if need_psi else None
-        # H^dagger psi differs from H psi only for complex coefficients.
-        if ctx.coeffs_given:
-            hermitian = ctx.real_coeffs
-        else:
-            hermitian = False
-        sigma_dag = torch.empty_like(psi) if (need_psi and not hermitian) else None
-        tv = torch.empty(op.num_terms, dtype=torch.complex128, device=psi.device) if need_c else None
-        op.expectation_adjoint(
-            psi.data_ptr(), psi.numel(),
-            tv.data_ptr() if tv is not None else None,
-            sigma.data_ptr() if sigma is not None else None,
-            sigma_dag.data_ptr() if sigma_dag is not None else None,
-            stream=_stream(psi.device),
-            coeffs_ptr=cplx.data_ptr() if ctx.coeffs_given else None,
-        )
         g = grad.to(torch.complex128)
         grad_psi = None
         if need_psi:
-            if sigma_dag is None:
-                grad_psi = (g.conj() + g) * sigma
-            else:
-                grad_psi = g.conj() * sigma + g * sigma_dag
+            c = cplx if ctx.coeffs_given else _own_coeffs(op, psi.device)
+            d = (2.0 * (g.conj() * c).real).to(torch.complex128).contiguous()
+            grad_psi = torch.empty_like(psi)
+            op.expectation_adjoint(psi.data_ptr(), psi.numel(), None, grad_psi.data_ptr(), None,
+                                   stream=_stream(psi.device), coeffs_ptr=d.data_ptr(), want_energy=False)
         grad_c = None
         if need_c:
             grad_c = g * tv.conj()

@@ -271,7 +271,20 @@ def adjoint(args) -> dict:
             d.expectation_adjoint(psi.data_ptr(), psi.numel(), tv.data_ptr(), sigma.data_ptr(), None,
                                   stream=stream)
 
+        from .autograd import expectation_value
+
+        c_t = torch.tensor(op.coeffs(), device="cuda:0")
+
+        def fwd_bwd():
+            # One training step through the autograd function: E, then dE
+            # with respect to both psi and the coefficients.
+            p = psi.detach().requires_grad_(True)
+            cc = c_t.detach().requires_grad_(True)
+            e = expectation_value(d, p, cc)
+            (e.real + 0.3 * e.imag).backward()
+
         t_fwd = _timed(fwd, args.steps, args.warmup)
+        t_train = _timed(fwd_bwd, args.steps, args.warmup)
         _native.profile_enable(True)
         _native.profile_reset()
         t_adj = _timed(fwd_adj, args.steps, args.warmup)
@@ -286,6 +299,7 @@ def adjoint(args) -> dict:
             "distinct_x_groups": gx,
             "forward_ms": t_fwd * 1000,
             "forward_plus_adjoint_ms": t_adj * 1000,
+            "autograd_forward_backward_ms": t_train * 1000,
             "sigma_kernels_ms": sig_ms,
             "forward_term_amp_pairs_per_s": T * amp / t_fwd,
             "forward_achieved_GBs_alg": fwd_bytes / t_fwd / 1e9,

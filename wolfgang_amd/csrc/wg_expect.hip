@@ -1472,7 +1472,7 @@ __global__ __launch_bounds__(1024) void expect_finalize_kernel(const double2* __
   }
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = make_double2(r, i);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && out) {
     double sr = 0.0, si = 0.0;
     for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) {
       sr += lds[w].x;
@@ -2266,6 +2266,40 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
 
 // ------------------------------------------------------------ plan
 
+namespace {
+
+// Sigma key records, pass by pass (pass 0 from item 0: it also carries the
+// diagonal items), each pass's list padded to a whole number of chunks with
+// zero records. Sets each pass's [sigma_begin, sigma_end).
+void build_sigma_records(const std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
+                         std::vector<SigmaKey>& sigma_keys) {
+  for (std::size_t p = 0; p < passes.size(); ++p) {
+    PassLaunch& pl = passes[p];
+    const u32 first = p == 0 ? 0 : pl.first_item;
+    pl.sigma_begin = static_cast<u32>(sigma_keys.size());
+    for (u32 i = first; i < pl.first_item + pl.num_items; ++i) {
+      const ExpItem& it = items[i];
+      for (int j = 0; j < it.nz; ++j) {
+        SigmaKey k{};
+        k.z = it.z[j];
+        k.zl = static_cast<u32>(it.zl[j]);
+        k.xl = static_cast<u32>(it.xl) | (j + 1 == it.nz ? kSigmaLast : 0U);
+        for (u32 r = 1; r < 4; ++r) {
+          k.sh[r - 1] = 0x3FF00000U | (host_parity(k.zl & (r << 10)) ? 0x80000000U : 0U);
+        }
+        k.key = it.key[j];
+        sigma_keys.push_back(k);
+      }
+    }
+    SigmaKey pad{};
+    pad.key = kSigmaPadKey;
+    while ((sigma_keys.size() - pl.sigma_begin) % kSigmaChunk != 0) sigma_keys.push_back(pad);
+    pl.sigma_end = static_cast<u32>(sigma_keys.size());
+  }
+}
+
+}  // namespace
+
 ExpectationPlan::~ExpectationPlan() {
   int prev = -1;
   const hipError_t r = hipGetDevice(&prev);
@@ -2451,31 +2485,7 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
     }
     if (slot > 0xFFFFFFFFULL) fail_runtime("tiled expectation plan has too many partial rows");
     plan->partial_slots = slot;
-    // Sigma key records, pass by pass (pass 0 from item 0: it also carries
-    // the diagonal items), then the zero record the sweep reads ahead.
-    for (std::size_t p = 0; p < plan->passes.size(); ++p) {
-      PassLaunch& pl = plan->passes[p];
-      const u32 first = p == 0 ? 0 : pl.first_item;
-      pl.sigma_begin = static_cast<u32>(sigma_keys.size());
-      for (u32 i = first; i < pl.first_item + pl.num_items; ++i) {
-        const ExpItem& it = items[i];
-        for (int j = 0; j < it.nz; ++j) {
-          SigmaKey k{};
-          k.z = it.z[j];
-          k.zl = static_cast<u32>(it.zl[j]);
-          k.xl = static_cast<u32>(it.xl) | (j + 1 == it.nz ? kSigmaLast : 0U);
-          for (u32 i = 1; i < 4; ++i) {
-            k.sh[i - 1] = 0x3FF00000U | (host_parity(k.zl & (i << 10)) ? 0x80000000U : 0U);
-          }
-          k.key = it.key[j];
-          sigma_keys.push_back(k);
-        }
-      }
-      SigmaKey pad{};
-      pad.key = kSigmaPadKey;
-      while ((sigma_keys.size() - pl.sigma_begin) % kSigmaChunk != 0) sigma_keys.push_back(pad);
-      pl.sigma_end = static_cast<u32>(sigma_keys.size());
-    }
+    build_sigma_records(items, plan->passes, sigma_keys);
     plan->sigma_keys = sigma_keys.size();
     int cus = 0;
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, op->device),
@@ -2711,9 +2721,12 @@ void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const
   const double2* dpsi = reinterpret_cast<const double2*>(psi);
   const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
   if ((sigma || sigma_dagger) && plan->tiled) {
-    // Forward sweeps for E and the per-key values, then sigma over the
-    // same passes (one read of psi and one sigma update per pass).
-    run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+    // Forward sweeps for E and the per-key values (only when either is
+    // wanted), then sigma over the same passes (one read of psi and one sigma
+    // update per pass).
+    if (dout || term_values) {
+      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+    }
     const double2* key_c = plan->d_key_c;
     if (coeffs) {
       double2* kc = scratch.take<double2>(plan->num_keys);
@@ -2806,8 +2819,10 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
     const std::size_t len = validate_psi(op, psi, length);
     DeviceScope scope(op->device);
     hipStream_t stream = as_stream(stream_ptr);
-    out2[0] = 0.0;
-    out2[1] = 0.0;
+    if (out2) {
+      out2[0] = 0.0;
+      out2[1] = 0.0;
+    }
     if (op->num_terms == 0) {
       if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
       if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
@@ -2815,14 +2830,18 @@ int wg_expectation_adjoint(const wg_pauli_sum* op_c, const double* psi, size_t l
       return;
     }
     Scratch scratch(op->device);
-    double2* dout = scratch.take<double2>(1);
+    double2* dout = out2 ? scratch.take<double2>(1) : nullptr;
     adjoint_enqueue(op, psi, len, coeffs, term_values, sigma, sigma_dagger, stream, scratch, dout);
-    double2 h;
-    check_hip(hipMemcpyAsync(&h, dout, sizeof(double2), hipMemcpyDeviceToHost, stream),
-              "copy device buffer to host");
-    sync(stream);
-    out2[0] = h.x;
-    out2[1] = h.y;
+    if (out2) {
+      double2 h;
+      check_hip(hipMemcpyAsync(&h, dout, sizeof(double2), hipMemcpyDeviceToHost, stream),
+                "copy device buffer to host");
+      sync(stream);
+      out2[0] = h.x;
+      out2[1] = h.y;
+    } else {
+      sync(stream);
+    }
   });
 }
 
@@ -2961,6 +2980,117 @@ int wg_debug_key_sweeps(size_t num_qubits, size_t num_terms, const uint64_t* x, 
           const ExpItem& it = items[pl.first_item + slot.item];
           for (int j = 0; j < it.nz; ++j) {
             if (key_kind[it.key[j]] < 0) key_kind[it.key[j]] = static_cast<int>(bundles[b].kind);
+          }
+        }
+      }
+    }
+  });
+}
+
+// Host-only replay of the tiled sigma sweep's records (test hook): builds the
+// items, passes and sigma records of a W = 1 operator as the device plan does
+// and evaluates them point by point on the CPU with the kernel's sign
+// decomposition (uniform, lane and register parts; the constant sign
+// (-1)^{parity(zl & xl)} folded into the per-call coefficients), accumulating
+// sigma = H psi and H^dagger psi.
+int wg_debug_sigma_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
+                        const double* coeffs, const double* psi, double* sigma, double* sigma_dag,
+                        size_t* num_records) {
+  return guard([&] {
+    const std::size_t n = num_qubits;
+    if (n < kTiledMinQubits || n > 24) fail_invalid("debug sigma plan supports 18 <= n <= 24");
+    std::size_t K = 0;
+    std::vector<ExpItem> items = debug_items(num_terms, x, z, &K);
+    // Key coefficients summed in term order (first-appearance key ids).
+    std::vector<double2> kc(K, make_double2(0.0, 0.0));
+    std::vector<u64> kxm(K), kzm(K);
+    std::vector<char> seen(K, 0);
+    {
+      std::unordered_map<u64, std::unordered_map<u64, u32>> idx;
+      u32 next = 0;
+      for (std::size_t t = 0; t < num_terms; ++t) {
+        auto [itr, ins] = idx[x[t]].emplace(z[t], next);
+        if (ins) ++next;
+        const u32 k = itr->second;
+        const double2 c = make_double2(coeffs[2 * t], coeffs[2 * t + 1]);
+        if (!seen[k]) {
+          kc[k] = c;
+          seen[k] = 1;
+          kxm[k] = x[t];
+          kzm[k] = z[t];
+        } else {
+          kc[k].x += c.x;
+          kc[k].y += c.y;
+        }
+      }
+    }
+    std::vector<PassLaunch> passes;
+    std::size_t diag_keys = 0;
+    for (const ExpItem& it : items) diag_keys += it.x == 0 ? static_cast<std::size_t>(it.nz) : 0;
+    u32 diag_front = 0;
+    plan_tiled_passes(n, items, passes, diag_keys > static_cast<std::size_t>(kDiagWhtMinItems) * kZ,
+                      &diag_front, SIZE_MAX);
+    std::vector<SigmaKey> recs;
+    build_sigma_records(items, passes, recs);
+    *num_records = recs.size();
+    auto times_iy = [](double2 v, int y) {
+      switch (y & 3) {
+        case 0: return v;
+        case 1: return make_double2(-v.y, v.x);
+        case 2: return make_double2(-v.x, -v.y);
+        default: return make_double2(v.y, -v.x);
+      }
+    };
+    for (SigmaKey& r : recs) {
+      if (r.key == kSigmaPadKey) continue;
+      const int y = __builtin_popcountll(kxm[r.key] & kzm[r.key]) & 3;
+      const bool neg = host_parity(r.zl & r.xl & ~kSigmaLast) != 0;
+      const double2 a = times_iy(kc[r.key], y), b = times_iy(make_double2(kc[r.key].x, -kc[r.key].y), y);
+      r.c = neg ? make_double2(-a.x, -a.y) : a;
+      r.cd = neg ? make_double2(-b.x, -b.y) : b;
+    }
+    const std::size_t len = std::size_t{1} << n;
+    const auto* amp = reinterpret_cast<const double2*>(psi);
+    auto* out = reinterpret_cast<double2*>(sigma);
+    auto* out_d = reinterpret_cast<double2*>(sigma_dag);
+    std::fill(out, out + len, make_double2(0.0, 0.0));
+    std::fill(out_d, out_d + len, make_double2(0.0, 0.0));
+    std::vector<double2> tile(kTilePoints);
+    std::vector<u64> gidx(kTilePoints);
+    for (const PassLaunch& pl : passes) {
+      const PassDesc& d = pl.desc;
+      if (d.k != kTileMaxLog) fail_runtime("sigma replay expects full tiles");
+      const u64 ntiles = u64{1} << (n - d.k);
+      for (u64 t = 0; t < ntiles; ++t) {
+        u64 rep = 0;
+        for (int j = 0; j < d.nfree; ++j) rep |= ((t >> j) & 1ULL) << d.free_pos[j];
+        for (u32 u = 0; u < kTilePoints; ++u) {
+          gidx[u] = (rep ^ d.seg_off[u >> d.l]) | (u & ((1U << d.l) - 1));
+          tile[u] = amp[gidx[u]];
+        }
+        for (u32 u = 0; u < kTilePoints; ++u) {
+          // Point u of the tile is point I of thread tid (u = tid + 1024 I).
+          const u32 tid = u & 1023U, I = u >> 10;
+          double2 c = make_double2(0, 0), e = make_double2(0, 0);
+          for (u32 r = pl.sigma_begin; r < pl.sigma_end; ++r) {
+            const SigmaKey& k = recs[r];
+            const u32 xl = k.xl & ~kSigmaLast;
+            const bool sgn = (host_parity(k.zl & tid) ^ host_parity(k.z & rep) ^
+                              host_parity(k.zl & (I << 10))) != 0;
+            c.x += sgn ? -k.c.x : k.c.x;
+            c.y += sgn ? -k.c.y : k.c.y;
+            e.x += sgn ? -k.cd.x : k.cd.x;
+            e.y += sgn ? -k.cd.y : k.cd.y;
+            if (k.xl & kSigmaLast) {
+              const double2 g = tile[u ^ xl];
+              double2& o = out[gidx[u]];
+              o.x += c.x * g.x - c.y * g.y;
+              o.y += c.x * g.y + c.y * g.x;
+              double2& od = out_d[gidx[u]];
+              od.x += e.x * g.x - e.y * g.y;
+              od.y += e.x * g.y + e.y * g.x;
+              c = e = make_double2(0, 0);
+            }
           }
         }
       }

@@ -645,14 +645,17 @@ class DevicePauliSum:
     def expectation_adjoint(
         self, psi_ptr: int, length: int, term_values_ptr: int | None, sigma_ptr: int | None,
         sigma_dagger_ptr: int | None, stream: Any = None, coeffs_ptr: int | None = None,
-    ) -> complex:
-        """Raw device-pointer forward + adjoint (see include/wolfgang_amd.h)."""
-        out = (C.c_double * 2)()
+        want_energy: bool = True,
+    ) -> complex | None:
+        """Raw device-pointer forward + adjoint (see include/wolfgang_amd.h).
+        With want_energy=False no energy is formed (None is returned), so a
+        sigma-only call on a tiled state runs only the sigma sweep."""
+        out = (C.c_double * 2)() if want_energy else None
         _native.call(
             "wg_expectation_adjoint", self._handle(), psi_ptr, length, coeffs_ptr, term_values_ptr,
             sigma_ptr, sigma_dagger_ptr, _stream_arg(stream), out,
         )
-        return complex(out[0], out[1])
+        return complex(out[0], out[1]) if want_energy else None
 
     def __repr__(self) -> str:
         return f"DevicePauliSum(num_qubits={self._n}, num_terms={self._t}, device={self._dev}, backend='hip')"
