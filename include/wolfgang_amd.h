@@ -190,23 +190,28 @@ int wg_expectation_adjoint(const wg_pauli_sum* op, const double* psi, size_t len
  * optional coeffs[i] (num_terms complex128, device; NULL = the operator's own),
  * optional outputs term_values[i], sigma[i] = H psi, sigma_dagger[i] =
  * H^dagger psi; energies holds 2*count doubles (device) for E_i. All pointers
- * are fixed at creation: the work of the instances whose plan is direct (small
- * states or few x-groups) is cut into units served by one persistent kernel from
- * a device work queue and that launch sequence is captured into a HIP graph;
- * instances with a tiled plan run the tiled sweeps of wg_expectation_adjoint
- * after it. tunables = {log2 forward chunk, sigma outputs per unit, persistent
- * grid} or NULL to pick them by timing the queued part. */
+ * are fixed at creation; the coefficient buffers are read on every run (a
+ * caller may update them between runs). Instances whose plan is direct (small
+ * states or few x-groups) are cut into units served by one persistent kernel
+ * from a device work queue; instances with a tiled plan run the tiled sweeps of
+ * wg_expectation_adjoint with buffers allocated at creation, spread over
+ * worker streams forked from and joined back to the caller's stream. Both parts
+ * are captured into one HIP graph (or the tiled part is enqueued per run when
+ * that timed faster at creation). tunables = {log2 forward chunk, sigma
+ * outputs per unit, persistent grid, tiled worker streams (1..4), tiled part
+ * inside the graph (0/1)} or NULL to pick them by timing the batch. */
 typedef struct wg_batch wg_batch;
 int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* const* psi,
                     const double* const* coeffs, double* const* term_values, double* const* sigma,
                     double* const* sigma_dagger, double* energies, const int* tunables,
                     void* stream, wg_batch** out);
-/* Enqueues one graph replay on `stream`, then runs the tiled instances on the
- * same stream, synchronizing after each (returns without waiting only when
- * the batch has no tiled instance). */
+/* Enqueues one run (one graph launch) on `stream` and returns without waiting. */
 int wg_batch_run(wg_batch* batch, void* stream);
 int wg_batch_sync(wg_batch* batch, void* stream);
+/* tunables receives the 5 values above. */
 int wg_batch_info(const wg_batch* batch, int* tunables, size_t* units, size_t* items);
+/* Number of instances that run the tiled sweeps. */
+int wg_batch_tiled_count(const wg_batch* batch, size_t* tiled);
 void wg_batch_destroy(wg_batch* batch);
 
 /* Batched Hamiltonian products (no reference counterpart; each pair follows

@@ -1,5 +1,6 @@
-"""Batched forward + adjoint (one persistent, graph-captured launch) against
-the per-instance path and the oracle (1e-12 relative to sum|c| ||psi||^2)."""
+"""Batched forward + adjoint (one graph replay: queued small instances in a
+persistent kernel, tiled instances on worker streams) against the
+per-instance path and the oracle (1e-12 relative to sum|c| ||psi||^2)."""
 
 import numpy as np
 import pytest
@@ -20,7 +21,7 @@ def make_batch(count, seed, n_range=(3, 11), t_range=(5, 300)):
     return ops, psis
 
 
-@pytest.mark.parametrize("tunables", [None, (10, 256, 64)])
+@pytest.mark.parametrize("tunables", [None, (10, 256, 64, 1, 1)])
 def test_batch_matches_oracle_and_plain_path(gpu, orc, tunables, tmp_path, monkeypatch):
     import torch
 
@@ -76,15 +77,18 @@ def test_batch_with_coefficient_overrides(gpu, orc):
             rtol=0, atol=1e-12 * scale)
 
 
-@pytest.mark.parametrize("all_tiled", [False, True])
-def test_batch_with_tiled_instances(gpu, orc, all_tiled):
-    # Instances with a tiled plan (18 qubits and up) bypass the queue and run
-    # the per-instance tiled sweeps after the graph; the rest stay queued.
+@pytest.mark.parametrize("all_tiled,tunables", [(False, None), (True, None), (False, (12, 1024, 256, 2, 1)),
+                                                 (False, (12, 1024, 256, 4, 0))])
+def test_batch_with_tiled_instances(gpu, orc, all_tiled, tunables):
+    # Instances with a tiled plan (18 qubits and up) run the tiled sweeps on
+    # worker streams (inside the graph, or enqueued per run); the rest stay
+    # queued. The coefficient tensors are read on every run: changing them in
+    # place between runs changes E, sigma and sigma^dagger of every instance.
     import torch
 
     from wolfgang_amd.batch import BatchedExpectation, plain_forward_adjoint
 
-    shapes = [(18, 400), (19, 150)] + ([] if all_tiled else [(9, 60), (12, 500), (5, 20)])
+    shapes = [(18, 400), (19, 150), (18, 90)] + ([] if all_tiled else [(9, 60), (12, 500), (5, 20)])
     ops = [random_pauli_masks(n, t, 0.25, 300 + i) if i % 2 else chemistry_masks(n, t, 300 + i)
            for i, (n, t) in enumerate(shapes)]
     psis = [random_statevector(n, 400 + i) for i, (n, _) in enumerate(shapes)]
@@ -92,16 +96,24 @@ def test_batch_with_tiled_instances(gpu, orc, all_tiled):
     new_c = [rng.normal(size=op.num_terms) + 1j * rng.normal(size=op.num_terms) for op in ops]
     b = BatchedExpectation([op.to_device() for op in ops], [torch.from_numpy(p).to("cuda:0") for p in psis],
                            coeffs=[torch.from_numpy(c) for c in new_c], sigma=True, sigma_dagger=True,
-                           use_cache=False)
+                           tunables=tunables, use_cache=False)
     assert (b.units == 0) == all_tiled
-    for _ in range(2):
+    assert b.tiled_instances == 3
+    if tunables is not None:
+        assert b.tunables == list(tunables)
+    for rnd in range(3):
+        if rnd == 2:
+            # New coefficients written into the same tensors.
+            new_c = [rng.normal(size=op.num_terms) + 1j * rng.normal(size=op.num_terms) for op in ops]
+            for t, c in zip(b.coeffs, new_c):
+                t.copy_(torch.from_numpy(c))
         b.run()
         torch.cuda.synchronize()
         for i, (op, psi) in enumerate(zip(ops, psis)):
             n, x, z = op.num_qubits, op.x_words(), op.z_words()
             scale = float(np.abs(new_c[i]).sum())
             want = orc.expectation_compensated(n, x, z, new_c[i], psi)
-            assert abs(complex(b.energies[i].item()) - want) <= 1e-12 * scale
+            assert abs(complex(b.energies[i].item()) - want) <= 1e-12 * scale, (rnd, i)
             np.testing.assert_allclose(b.sigma[i].cpu().numpy(), orc.apply(n, x, z, new_c[i], psi, threads=8),
                                        rtol=0, atol=1e-12 * scale)
             np.testing.assert_allclose(b.sigma_dagger[i].cpu().numpy(),
@@ -109,6 +121,7 @@ def test_batch_with_tiled_instances(gpu, orc, all_tiled):
                                        rtol=0, atol=1e-12 * scale)
             tv = orc.term_values(n, x, z, psi)
             np.testing.assert_allclose(b.term_values[i].cpu().numpy().real, tv, rtol=0, atol=1e-12)
+        _ = torch.randn(100, device="cuda:0").sum().item()
     e_batch = b.energies.clone()
     plain_forward_adjoint(b)
     torch.cuda.synchronize()

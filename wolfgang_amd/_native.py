@@ -76,6 +76,7 @@ SIGNATURES = {
     "wg_batch_run": (C.c_int, [_vp, _vp]),
     "wg_batch_sync": (C.c_int, [_vp, _vp]),
     "wg_batch_info": (C.c_int, [_vp, _vp, C.POINTER(_size), C.POINTER(_size)]),
+    "wg_batch_tiled_count": (C.c_int, [_vp, C.POINTER(_size)]),
     "wg_batch_destroy": (None, [_vp]),
     "wg_group_commuting": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(_size), _vp, _vp]),
     "wg_product_batch_create": (C.c_int, [_size, _vp, _vp, _size, _size, _vp, C.POINTER(_vp)]),

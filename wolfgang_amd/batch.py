@@ -1,15 +1,16 @@
 """Batched forward + adjoint over many (psi, H) pairs in one graph replay.
 
-``BatchedExpectation(ops, psis)`` builds one persistent, queue-driven launch
-over all instances (see include/wolfgang_amd.h, wg_batch_create) and captures
-it into a HIP graph; ``run()`` replays it on the current torch stream. Outputs
+``BatchedExpectation(ops, psis)`` builds the batch (see include/wolfgang_amd.h,
+wg_batch_create): small instances as units of one persistent, queue-driven
+kernel, large (tiled) instances as tiled sweeps on worker streams, both captured
+into one HIP graph; ``run()`` replays it on the current torch stream. Outputs
 per instance: E_i (``energies``), per-term values <P_k> and sigma_i = H_i psi_i
-(and H^dagger psi when asked).
+(and H^dagger psi when asked). The coefficient tensors are read on every run.
 
-The tunables (forward chunk, sigma chunk, persistent grid) are found by timing
-the actual batch once and remembered in a JSON cache keyed by the device and
-the batch's shape signature (``$WOLFGANG_AMD_CACHE`` or
-``~/.cache/wolfgang_amd/autotune.json``).
+The tunables (forward chunk, sigma chunk, persistent grid, tiled worker
+streams, tiled part inside the graph) are found by timing the actual batch once
+and remembered in a JSON cache keyed by the device and the batch's shape
+signature (``$WOLFGANG_AMD_CACHE`` or ``~/.cache/wolfgang_amd/autotune.json``).
 
 ``plain_forward_adjoint`` is the per-instance path over the same outputs (one
 expectation_adjoint call per instance), the baseline the batched launch is
@@ -104,9 +105,11 @@ class BatchedExpectation:
         self.cache_key = f"batch:{sig}"
         if tunables is None and use_cache:
             cached = _load_cache().get(self.cache_key)
-            if cached:
+            if cached and len(cached) == 5:
                 tunables = cached
-        tun = (C.c_int * 3)(*tunables) if tunables is not None else None
+        if tunables is not None and len(tunables) != 5:
+            raise ValueError("batch tunables are 5 integers")
+        tun = (C.c_int * 5)(*tunables) if tunables is not None else None
         handle = C.c_void_p()
         stream = torch.cuda.current_stream(dev).cuda_stream
         _native.call(
@@ -125,12 +128,14 @@ class BatchedExpectation:
         )
         self._h = handle.value
         self._device = dev
-        info = (C.c_int * 3)()
-        units, items = C.c_size_t(), C.c_size_t()
+        info = (C.c_int * 5)()
+        units, items, tiled = C.c_size_t(), C.c_size_t(), C.c_size_t()
         _native.call("wg_batch_info", self._h, info, C.byref(units), C.byref(items))
-        self.tunables = [int(info[0]), int(info[1]), int(info[2])]
+        _native.call("wg_batch_tiled_count", self._h, C.byref(tiled))
+        self.tunables = [int(v) for v in info]
         self.units = units.value
         self.items = items.value
+        self.tiled_instances = tiled.value
         if use_cache and tunables is None:
             _store_cache(self.cache_key, self.tunables)
 

@@ -96,7 +96,8 @@ def matmul(args) -> dict:
     }
 
 
-def batch_speedup(batched_steps: int = 3, plain_steps: int = 2, warmup: int = 1) -> dict:
+def batch_speedup(batched_steps: int = 3, plain_steps: int = 2, warmup: int = 1,
+                  breakdown: bool = False) -> dict:
     """256 (statevector, Hamiltonian) pairs, n ~ U{12..22}, terms log-uniform
     in [100, 50,000] (random masks, density 0.25, complex coefficients):
     forward + adjoint (E, per-term values, sigma and sigma^dagger) in one graph
@@ -122,12 +123,25 @@ def batch_speedup(batched_steps: int = 3, plain_steps: int = 2, warmup: int = 1)
     setup_s = time.perf_counter() - t0
     batched = _timed(b.run, batched_steps, warmup)
     e_batched = b.energies.clone()
+    # The same batch with every launch inside the one graph (one hipGraphLaunch
+    # per run), for comparison with the autotuned choice.
+    forced = list(b.tunables[:4]) + [1]
+    if b.tiled_instances and b.tunables[4] == 0:
+        g_all = BatchedExpectation(ops, psis, sigma=True, sigma_dagger=True, tunables=forced, use_cache=False)
+        all_in_graph_ms = _timed(g_all.run, batched_steps, warmup) * 1000
+        del g_all
+    else:
+        all_in_graph_ms = batched * 1000
     plain = _timed(lambda: plain_forward_adjoint(b), plain_steps, warmup)
     diff = torch.abs(b.energies - e_batched).cpu().numpy()
     pairs = sum(T * (1 << n) for n, T in shapes)
+    extra = {"per_qubit_count": _per_instance_breakdown(b, shapes)} if breakdown else {}
     return {
+        **extra,
+        "tiled_instances": b.tiled_instances,
         "speedup": plain / batched,
         "batched_ms": batched * 1000,
+        "batched_ms_all_in_graph": all_in_graph_ms,
         "plain_ms": plain * 1000,
         "max_abs_energy_difference": float(diff.max()),
         "max_rel_energy_difference": float(np.max(diff / np.asarray(l1))),
@@ -142,8 +156,31 @@ def batch_speedup(batched_steps: int = 3, plain_steps: int = 2, warmup: int = 1)
     }
 
 
+def _per_instance_breakdown(b, shapes) -> dict:
+    """Plain-path time of every instance on its own (synchronized before and
+    after), grouped by qubit count, with whether its plan is tiled."""
+    import torch
+
+    stream = torch.cuda.current_stream().cuda_stream
+    rows = {}
+    for i, (op, psi) in enumerate(zip(b.ops, b.psis)):
+        n, T = shapes[i]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.expectation_adjoint(psi.data_ptr(), psi.numel(), b.term_values[i].data_ptr(), b.sigma[i].data_ptr(),
+                               b.sigma_dagger[i].data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        r = rows.setdefault(n, {"instances": 0, "tiled": 0, "plain_ms": 0.0, "term_amp_pairs": 0})
+        r["instances"] += 1
+        r["tiled"] += int(op.expectation_plan_info()["tiled"])
+        r["plain_ms"] += dt * 1000
+        r["term_amp_pairs"] += T * (1 << n)
+    return {str(n): rows[n] for n in sorted(rows)}
+
+
 def batch(args) -> dict:
-    res = batch_speedup(args.steps, max(1, args.steps // 2), args.warmup)
+    res = batch_speedup(args.steps, max(1, args.steps // 2), args.warmup, breakdown=True)
     return {
         "metric": "batched fwd+adjoint speedup vs plain per-instance GPU path",
         "value": res["speedup"],

@@ -1,26 +1,34 @@
-// Batched forward + adjoint over many (psi, H) instances.
+// Batched forward + adjoint over many (psi, H) instances, one graph replay.
 //
-// All instances' work is cut into units and served from one device work queue
-// by a persistent kernel:
+// Instances whose expectation plan is direct (small states, or few x-groups)
+// are cut into units served from one device work queue by a persistent
+// kernel:
 //   * forward units (instance, item, chunk of its sweep range) produce partial
 //     sums S_z for one x-group slice, as the direct sweep does;
 //   * sigma units (instance, range of output amplitudes) produce sigma = H psi
 //     (and H^dagger psi) for those outputs.
 // Units are ordered by decreasing cost, so the queue behaves as a
 // longest-processing-time-first schedule across instances of very different
-// sizes. A reduce kernel and a per-instance finalize kernel follow. The whole
-// sequence (queue reset, persistent kernel, reduce, finalize) is captured once
-// into a HIP graph and replayed; the queue counter is reset by a one-thread
-// kernel inside the graph (no memset node).
+// sizes. The per-key coefficients of these instances are recomputed from the
+// callers' coefficient buffers at the start of every replay, then the queue
+// counter is reset by a one-thread kernel (no memset node), and a reduce and a
+// per-instance finalize kernel follow the persistent kernel.
 //
-// Instances whose expectation plan is tiled (large states with many x-groups)
-// stay out of the queue: the direct sweeps of a unit read psi once per x-group,
-// the tiled passes once per pass. They run the per-instance tiled path on the
-// same stream after the graph.
+// Instances whose plan is tiled (18 qubits and up with many x-groups) run the
+// tiled sweeps (forward owner sweep, then one sigma launch per pass) with
+// their temporary buffers carved from one allocation made at creation. Their
+// launch sequences are spread over `streams` worker streams,
+// longest-estimated-first onto the least loaded stream, forked from and
+// joined back to the caller's stream by events, so instances too small to
+// fill the GPU on their own (an 18-qubit pass has 64 tiles) run side by side.
+// That fork/join is captured into the same HIP graph as the queue (one
+// hipGraphLaunch per run, no host synchronization), or, when timing at
+// creation shows it faster, enqueued on the worker streams at each run.
 //
-// Tunables (picked by a timing search at creation unless given): forward chunk
-// (amplitude pairs per unit, 2^fwd_log), sigma chunk (outputs per unit) and the
-// persistent grid size.
+// Tunables (picked by timing the actual batch at creation unless given):
+// forward chunk (amplitude pairs per unit, 2^fwd_log), sigma chunk (outputs per
+// unit), persistent grid, worker streams for the tiled instances, and whether
+// the tiled fork/join is inside the graph.
 
 #include <hip/hip_runtime.h>
 
@@ -243,6 +251,27 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const BatchInst* __
   }
 }
 
+// Per-key coefficient sums of the queued instances, recomputed from the
+// instances' coefficient buffers on every replay (keys in the flat group
+// order of the batch, terms of a key in term order: the same additions as the
+// per-instance plan, so the same bits).
+__global__ __launch_bounds__(256) void batch_key_coeff_kernel(const BatchInst* __restrict__ insts,
+                                                              const u32* __restrict__ key_inst,
+                                                              const u32* __restrict__ key_term_begin,
+                                                              const u32* __restrict__ key_terms,
+                                                              u32 num_keys, double2* __restrict__ key_c) {
+  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_keys) return;
+  const double2* c = insts[key_inst[k]].coeffs;
+  const u32 b = key_term_begin[k], e = key_term_begin[k + 1];
+  double2 acc = c[key_terms[b]];
+  for (u32 i = b + 1; i < e; ++i) {
+    acc.x += c[key_terms[i]].x;
+    acc.y += c[key_terms[i]].y;
+  }
+  key_c[k] = acc;
+}
+
 template <typename T>
 T* upload(const std::vector<T>& v, hipStream_t stream) {
   T* d = device_alloc<T>(std::max<std::size_t>(v.size(), 1));
@@ -258,31 +287,38 @@ T* upload(const std::vector<T>& v, hipStream_t stream) {
 
 using namespace wg;
 
+constexpr int kMaxTiledStreams = 4;
+
 struct wg_batch {
   int device = 0;
   std::vector<std::shared_ptr<ExpectationPlan>> plans;  // keeps instance plans alive
+  // ---- instances with a direct plan: the persistent queue
   std::vector<BatchInst> insts;
-  std::vector<ExpItem> items;         // direct items, keys offset to global ids
+  std::vector<std::size_t> item_first;  // first item of each queued instance
+  std::vector<ExpItem> items;           // direct items, keys offset to global ids
   std::vector<u64> group_x;
-  std::vector<u32> group_begin;       // global CSR over keys (flat, group order)
+  std::vector<u32> group_begin;         // global CSR over keys (flat, group order)
   std::vector<u64> key_z;
   std::vector<int> key_y;
-  std::vector<double2> key_c;
-  std::vector<u32> term_key;          // per instance, local key ids
+  std::vector<u32> key_inst;            // queued instance of each flat key
+  std::vector<u32> key_term_begin;      // global CSR: flat key -> its terms
+  std::vector<u32> key_terms;           // term indices local to the instance
+  std::vector<u32> term_key;            // per instance, local key ids
   std::size_t total_keys = 0;
-  // Device copies.
   BatchInst* d_insts = nullptr;
   ExpItem* d_items = nullptr;
   u64* d_group_x = nullptr;
   u32* d_group_begin = nullptr;
   u64* d_key_z = nullptr;
   int* d_key_y = nullptr;
-  double2* d_key_c = nullptr;
+  double2* d_key_c = nullptr;           // written by batch_key_coeff_kernel every replay
+  u32* d_key_inst = nullptr;
+  u32* d_key_term_begin = nullptr;
+  u32* d_key_terms = nullptr;
   u32* d_term_key = nullptr;
   double* d_key_vals = nullptr;
-  double2* d_energies = nullptr;      // caller-owned
+  double2* d_energies = nullptr;        // caller-owned
   u32* d_counter = nullptr;
-  // Schedule for the current tunables.
   int fwd_log = 16;
   u32 sigma_chunk = 4096;
   u32 grid = 0;
@@ -290,14 +326,38 @@ struct wg_batch {
   BatchUnit* d_units = nullptr;
   u32* d_slot_begin = nullptr;
   double2* d_partials = nullptr;
+  // ---- instances with a tiled plan
+  struct TiledInst {
+    const ExpectationPlan* plan;
+    const double2* op_coeffs;
+    const double2* psi;
+    const double2* coeffs;  // caller override or null
+    double2* term_values;
+    double2* sigma;
+    double2* sigma_dag;
+    u32 slot;
+    std::size_t buf_offset;
+    double cost;
+  };
+  std::vector<TiledInst> tiled;  // longest-estimated first
+  void* d_tiled_buf = nullptr;
+  int streams = 1;
+  bool tiled_in_graph = true;
+  hipStream_t worker[kMaxTiledStreams] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join[kMaxTiledStreams] = {};
+  // ---- the graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
-  void free_schedule() {
+  void free_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+  }
+
+  void free_schedule() {
     device_free(d_units);
     device_free(d_slot_begin);
     device_free(d_partials);
@@ -310,21 +370,28 @@ struct wg_batch {
     int prev = -1;
     const hipError_t r = hipGetDevice(&prev);
     (void)hipSetDevice(device);
+    free_graph();
     free_schedule();
-    device_free(d_insts);
-    device_free(d_items);
-    device_free(d_group_x);
-    device_free(d_group_begin);
-    device_free(d_key_z);
-    device_free(d_key_y);
-    device_free(d_key_c);
-    device_free(d_term_key);
-    device_free(d_key_vals);
-    device_free(d_counter);
+    for (void* p : {static_cast<void*>(d_insts), static_cast<void*>(d_items),
+                    static_cast<void*>(d_group_x), static_cast<void*>(d_group_begin),
+                    static_cast<void*>(d_key_z), static_cast<void*>(d_key_y),
+                    static_cast<void*>(d_key_c), static_cast<void*>(d_key_inst),
+                    static_cast<void*>(d_key_term_begin), static_cast<void*>(d_key_terms),
+                    static_cast<void*>(d_term_key), static_cast<void*>(d_key_vals),
+                    static_cast<void*>(d_counter), d_tiled_buf}) {
+      device_free(p);
+    }
+    for (hipStream_t& s : worker) {
+      if (s) (void)hipStreamDestroy(s);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (hipEvent_t& e : join) {
+      if (e) (void)hipEventDestroy(e);
+    }
     if (r == hipSuccess && prev >= 0) (void)hipSetDevice(prev);
   }
 
-  // Builds units for (fwd_log, sigma_chunk) and uploads them.
+  // Builds the queue's units for (fwd_log, sigma_chunk) and uploads them.
   void schedule(int fwd_log_, u32 sigma_chunk_, u32 grid_, hipStream_t stream) {
     free_schedule();
     fwd_log = fwd_log_;
@@ -384,8 +451,15 @@ struct wg_batch {
     sync(stream);
   }
 
-  void launch(hipStream_t stream) {
+  // The queued instances: key coefficients, queue reset, persistent kernel,
+  // reduce, finalize.
+  void launch_queue(hipStream_t stream) {
     if (insts.empty()) return;
+    if (total_keys) {
+      batch_key_coeff_kernel<<<grid_for(key_z.size(), 256), 256, 0, stream>>>(
+          d_insts, d_key_inst, d_key_term_begin, d_key_terms, static_cast<u32>(key_z.size()), d_key_c);
+      check_launch("batch_key_coeff");
+    }
     reset_counter_kernel<<<1, 1, 0, stream>>>(d_counter);
     batch_persistent_kernel<<<grid, kBlock, 0, stream>>>(
         d_units, static_cast<u32>(num_units), d_counter, d_insts, d_items, d_group_x,
@@ -396,35 +470,87 @@ struct wg_batch {
     }
     batch_finalize_kernel<<<static_cast<unsigned>(insts.size()), 256, 0, stream>>>(
         d_insts, d_term_key, d_key_vals, d_energies);
+    check_launch("batch");
   }
 
-  std::vector<std::size_t> item_first;  // first item of each instance
+  // The tiled instances run on `nstreams` worker streams (longest first onto
+  // the least loaded one), forked from `stream` before the queue's launches
+  // and joined back after them, so the queue and the tiled sweeps overlap.
+  void fork(hipStream_t stream, int nstreams) {
+    check_hip(hipEventRecord(fork_ev, stream), "record fork event");
+    for (int w = 0; w < nstreams; ++w) {
+      check_hip(hipStreamWaitEvent(worker[w], fork_ev, 0), "wait fork event");
+    }
+  }
 
-  // Instances whose plan is tiled (the largest ones) run the tiled sweeps of
-  // the per-instance path after the graph: the queue's direct sweeps would
-  // read psi once per x-group.
-  struct TiledInst {
-    wg_pauli_sum* op;
-    const double* psi;
-    const double* coeffs;
-    double* term_values;
-    double* sigma;
-    double* sigma_dag;
-    u32 slot;
-  };
-  std::vector<TiledInst> tiled;
+  void enqueue_tiled(int nstreams) {
+    std::vector<double> load(nstreams, 0.0);
+    for (const TiledInst& t : tiled) {
+      const int w = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      load[w] += t.cost;
+      tiled_adjoint_enqueue(*t.plan, t.op_coeffs, t.psi, t.coeffs, t.term_values, t.sigma,
+                            t.sigma_dag, d_energies + t.slot,
+                            static_cast<unsigned char*>(d_tiled_buf) + t.buf_offset, worker[w]);
+    }
+  }
+
+  void join_back(hipStream_t stream, int nstreams) {
+    for (int w = 0; w < nstreams; ++w) {
+      check_hip(hipEventRecord(join[w], worker[w]), "record join event");
+      check_hip(hipStreamWaitEvent(stream, join[w], 0), "wait join event");
+    }
+  }
+
+  // Queue launches on `stream` (when `queue`) and, when `with_tiled`, the
+  // tiled instances on the worker streams, all ordered after the work already
+  // on `stream` and before what follows it.
+  void launch(hipStream_t stream, bool queue, bool with_tiled, int nstreams) {
+    const bool t = with_tiled && !tiled.empty();
+    if (t) fork(stream, nstreams);
+    if (queue) launch_queue(stream);
+    if (t) {
+      enqueue_tiled(nstreams);
+      join_back(stream, nstreams);
+    }
+  }
+
+  void capture(bool with_tiled) {
+    free_graph();
+    hipStream_t cs;
+    check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "create capture stream");
+    check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+    launch(cs, true, with_tiled, streams);
+    hipGraph_t g = nullptr;
+    const hipError_t r = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    check_hip(r, "end capture");
+    graph = g;
+    check_hip(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "instantiate graph");
+  }
+
+  void run(hipStream_t stream) {
+    if (tiled_in_graph || tiled.empty()) {
+      if (exec) check_hip(hipGraphLaunch(exec, stream), "launch graph");
+      return;
+    }
+    fork(stream, streams);
+    if (exec) check_hip(hipGraphLaunch(exec, stream), "launch graph");
+    enqueue_tiled(streams);
+    join_back(stream, streams);
+  }
 };
 
 namespace {
 
-// One timed launch of the batch (after a warm-up launch when `warm`).
-float time_launch(wg_batch* b, hipStream_t stream, bool warm) {
+// Milliseconds of one call of fn(stream) after `warm` untimed calls.
+template <typename Fn>
+float time_ms(hipStream_t stream, int warm, Fn&& fn) {
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "create event");
   check_hip(hipEventCreate(&e1), "create event");
-  if (warm) b->launch(stream);
+  for (int i = 0; i < warm; ++i) fn(stream);
   check_hip(hipEventRecord(e0, stream), "record event");
-  b->launch(stream);
+  fn(stream);
   check_hip(hipEventRecord(e1, stream), "record event");
   check_hip(hipEventSynchronize(e1), "synchronize event");
   float ms = 0.0f;
@@ -433,20 +559,6 @@ float time_launch(wg_batch* b, hipStream_t stream, bool warm) {
   (void)hipEventDestroy(e1);
   check_launch("batch");
   return ms;
-}
-
-void capture(wg_batch* b, hipStream_t stream) {
-  hipStream_t cs;
-  check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "create capture stream");
-  check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
-  b->launch(cs);
-  hipGraph_t g = nullptr;
-  const hipError_t r = hipStreamEndCapture(cs, &g);
-  (void)hipStreamDestroy(cs);
-  check_hip(r, "end capture");
-  b->graph = g;
-  check_hip(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "instantiate graph");
-  (void)stream;
 }
 
 }  // namespace
@@ -471,27 +583,44 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
       if (ops[i]->num_qubits > 40) fail_invalid("batched expectation supports num_qubits <= 40");
       if (psi[i] == nullptr) fail_invalid("HIP statevector data pointer must be non-null");
     }
+    if (tunables && (tunables[0] < 8 || tunables[0] > 30 || tunables[1] < 1 || tunables[2] < 1 ||
+                     tunables[3] < 1 || tunables[3] > kMaxTiledStreams)) {
+      fail_invalid("batch tunables out of range");
+    }
     const int device = ops[0]->device;
     DeviceScope scope(device);
     hipStream_t stream = as_stream(stream_ptr);
     std::unique_ptr<wg_batch> b(new wg_batch());
     b->device = device;
     b->d_energies = reinterpret_cast<double2*>(energies);
+    std::size_t tiled_bytes = 0;
     for (std::size_t i = 0; i < count; ++i) {
       auto* op = const_cast<wg_pauli_sum*>(ops[i]);
       auto plan = get_plan(op, stream);
       b->plans.push_back(plan);
+      const double2* ci = coeffs && coeffs[i] ? reinterpret_cast<const double2*>(coeffs[i]) : nullptr;
       if (plan->tiled && op->num_terms > 0) {
-        b->tiled.push_back({op, psi[i], coeffs ? coeffs[i] : nullptr,
-                            term_values ? term_values[i] : nullptr, sigma ? sigma[i] : nullptr,
-                            sigma_dagger ? sigma_dagger[i] : nullptr, static_cast<u32>(i)});
+        wg_batch::TiledInst t{};
+        t.plan = plan.get();
+        t.op_coeffs = op->coeffs;
+        t.psi = reinterpret_cast<const double2*>(psi[i]);
+        t.coeffs = ci;
+        t.term_values = term_values ? reinterpret_cast<double2*>(term_values[i]) : nullptr;
+        t.sigma = sigma ? reinterpret_cast<double2*>(sigma[i]) : nullptr;
+        t.sigma_dag = sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger[i]) : nullptr;
+        t.slot = static_cast<u32>(i);
+        t.buf_offset = tiled_bytes;
+        tiled_bytes += (tiled_adjoint_bytes(*plan) + 255) & ~std::size_t{255};
+        // Work estimate: psi (and sigma) streamed once per pass.
+        t.cost = static_cast<double>(plan->passes.size()) * static_cast<double>(u64{1} << op->num_qubits);
+        b->tiled.push_back(t);
         continue;
       }
       BatchInst in{};
       in.slot = static_cast<u32>(i);
       in.psi = reinterpret_cast<const double2*>(psi[i]);
       in.len = u64{1} << op->num_qubits;
-      in.coeffs = coeffs && coeffs[i] ? reinterpret_cast<const double2*>(coeffs[i]) : op->coeffs;
+      in.coeffs = ci ? ci : op->coeffs;
       in.term_values = term_values ? reinterpret_cast<double2*>(term_values[i]) : nullptr;
       in.sigma = sigma ? reinterpret_cast<double2*>(sigma[i]) : nullptr;
       in.sigma_dag = sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger[i]) : nullptr;
@@ -501,6 +630,7 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
       in.num_terms = static_cast<u32>(op->num_terms);
       in.key_base = static_cast<u32>(b->total_keys);
       const u32 flat_base = static_cast<u32>(b->key_z.size());
+      const u32 inst = static_cast<u32>(b->insts.size());
       b->item_first.push_back(b->items.size());
       for (ExpItem it : plan->direct_items) {
         for (int j = 0; j < it.nz; ++j) it.key[j] += in.key_base;
@@ -512,84 +642,103 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
       }
       b->key_z.insert(b->key_z.end(), plan->h_key_z.begin(), plan->h_key_z.end());
       b->key_y.insert(b->key_y.end(), plan->h_key_y.begin(), plan->h_key_y.end());
-      if (coeffs && coeffs[i]) {
-        // Per-key sums of the given coefficients, keys in group order.
-        std::vector<double2> hc(op->num_terms);
-        check_hip(hipMemcpy(hc.data(), coeffs[i], op->num_terms * sizeof(double2),
-                            hipMemcpyDeviceToHost),
-                  "copy device buffer to host");
-        std::vector<double2> kc(plan->num_keys, make_double2(0, 0));
-        std::vector<char> seen(plan->num_keys, 0);
-        for (std::size_t t = 0; t < op->num_terms; ++t) {
-          const u32 k = plan->h_term_key[t];
-          if (!seen[k]) {
-            kc[k] = hc[t];
-            seen[k] = 1;
-          } else {
-            kc[k].x += hc[t].x;
-            kc[k].y += hc[t].y;
-          }
-        }
-        // Map first-appearance key ids to group order through the items.
-        std::vector<double2> flat(plan->h_key_z.size());
-        std::size_t pos = 0;
-        for (const ExpItem& it : plan->direct_items) {
-          for (int j = 0; j < it.nz; ++j) flat[pos++] = kc[it.key[j]];
-        }
-        b->key_c.insert(b->key_c.end(), flat.begin(), flat.end());
-      } else {
-        b->key_c.insert(b->key_c.end(), plan->h_key_c.begin(), plan->h_key_c.end());
+      // Flat keys of this instance: the terms to sum per replay.
+      const u32 term_off = static_cast<u32>(b->key_terms.size());
+      for (std::size_t k = 0; k < plan->h_key_z.size(); ++k) {
+        b->key_inst.push_back(inst);
+        b->key_term_begin.push_back(term_off + plan->h_key_term_begin[k]);
       }
+      b->key_terms.insert(b->key_terms.end(), plan->h_key_terms.begin(), plan->h_key_terms.end());
       b->term_key.insert(b->term_key.end(), plan->h_term_key.begin(), plan->h_term_key.end());
       b->total_keys += plan->num_keys;
       b->insts.push_back(in);
     }
     b->group_begin.push_back(static_cast<u32>(b->key_z.size()));
+    b->key_term_begin.push_back(static_cast<u32>(b->key_terms.size()));
+    // Longest-estimated tiled instances first.
+    std::stable_sort(b->tiled.begin(), b->tiled.end(),
+                     [](const wg_batch::TiledInst& x, const wg_batch::TiledInst& y) { return x.cost > y.cost; });
     b->d_insts = upload(b->insts, stream);
     b->d_items = upload(b->items, stream);
     b->d_group_x = upload(b->group_x, stream);
     b->d_group_begin = upload(b->group_begin, stream);
     b->d_key_z = upload(b->key_z, stream);
     b->d_key_y = upload(b->key_y, stream);
-    b->d_key_c = upload(b->key_c, stream);
+    b->d_key_inst = upload(b->key_inst, stream);
+    b->d_key_term_begin = upload(b->key_term_begin, stream);
+    b->d_key_terms = upload(b->key_terms, stream);
     b->d_term_key = upload(b->term_key, stream);
+    b->d_key_c = device_alloc<double2>(std::max<std::size_t>(1, b->key_z.size()));
     b->d_key_vals = device_alloc<double>(std::max<std::size_t>(1, b->total_keys));
     b->d_counter = device_alloc<u32>(1);
+    if (tiled_bytes) b->d_tiled_buf = device_alloc<unsigned char>(tiled_bytes);
+    for (int w = 0; w < kMaxTiledStreams; ++w) {
+      check_hip(hipStreamCreateWithFlags(&b->worker[w], hipStreamNonBlocking), "create worker stream");
+      check_hip(hipEventCreateWithFlags(&b->join[w], hipEventDisableTiming), "create event");
+    }
+    check_hip(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "create event");
     sync(stream);
 
     hipDeviceProp_t prop{};
     check_hip(hipGetDeviceProperties(&prop, device), "get device properties");
     const u32 cus = static_cast<u32>(prop.multiProcessorCount);
-    if (b->insts.empty()) {
-      // Every instance is tiled: nothing for the queue.
-    } else if (tunables) {
-      b->schedule(tunables[0], static_cast<u32>(tunables[1]), static_cast<u32>(tunables[2]), stream);
+    if (tunables) {
+      b->streams = tunables[3];
+      b->tiled_in_graph = tunables[4] != 0;
+      if (!b->insts.empty()) {
+        b->schedule(tunables[0], static_cast<u32>(tunables[1]), static_cast<u32>(tunables[2]), stream);
+      }
     } else {
-      // Timing search on the real batch, one tunable at a time (forward
-      // chunk, then sigma chunk, then grid), each from the best so far.
-      int best_f = 16;
-      u32 best_s = 4096, best_g = cus * 8;
-      b->schedule(best_f, best_s, best_g, stream);
-      float best = time_launch(b.get(), stream, true);
-      auto trial = [&](int f, u32 s, u32 g) {
-        if (f == best_f && s == best_s && g == best_g) return;
-        b->schedule(f, s, g, stream);
-        const float ms = time_launch(b.get(), stream, best < 50.0f);
-        if (ms < best) {
-          best = ms;
-          best_f = f;
-          best_s = s;
-          best_g = g;
+      if (!b->insts.empty()) {
+        // Timing search on the queued part, one tunable at a time (forward
+        // chunk, then sigma chunk, then grid), each from the best so far.
+        int best_f = 16;
+        u32 best_s = 4096, best_g = cus * 8;
+        b->schedule(best_f, best_s, best_g, stream);
+        auto queue = [&](hipStream_t s) { b->launch(s, true, false, 1); };
+        float best = time_ms(stream, 1, queue);
+        auto trial = [&](int f, u32 s, u32 g) {
+          if (f == best_f && s == best_s && g == best_g) return;
+          b->schedule(f, s, g, stream);
+          const float ms = time_ms(stream, best < 50.0f ? 1 : 0, queue);
+          if (ms < best) {
+            best = ms;
+            best_f = f;
+            best_s = s;
+            best_g = g;
+          }
+        };
+        for (int f : {14, 18}) trial(f, best_s, best_g);
+        const int f_pick = best_f;
+        for (u32 s : {1024u, 16384u}) trial(f_pick, s, best_g);
+        const u32 s_pick = best_s;
+        for (u32 g : {cus * 4, cus * 16}) trial(f_pick, s_pick, g);
+        b->schedule(best_f, best_s, best_g, stream);
+      }
+      if (!b->tiled.empty()) {
+        // Worker streams for the tiled instances: 1, 2 or 4, timed directly
+        // (one warm-up run first).
+        float best = 0.0f;
+        for (int k : {1, 2, 4}) {
+          const float ms =
+              time_ms(stream, k == 1 ? 1 : 0, [&](hipStream_t s) { b->launch(s, false, true, k); });
+          if (k == 1 || ms < best) {
+            best = ms;
+            b->streams = k;
+          }
         }
-      };
-      for (int f : {14, 18}) trial(f, best_s, best_g);
-      const int f_pick = best_f;
-      for (u32 s : {1024u, 16384u}) trial(f_pick, s, best_g);
-      const u32 s_pick = best_s;
-      for (u32 g : {cus * 4, cus * 16}) trial(f_pick, s_pick, g);
-      b->schedule(best_f, best_s, best_g, stream);
+        // Inside the graph or enqueued per run: time whole runs both ways.
+        b->tiled_in_graph = true;
+        b->capture(true);
+        const float in_graph = time_ms(stream, 0, [&](hipStream_t s) { b->run(s); });
+        b->tiled_in_graph = false;
+        b->capture(false);
+        const float outside = time_ms(stream, 0, [&](hipStream_t s) { b->run(s); });
+        b->tiled_in_graph = in_graph <= outside;
+      }
     }
-    if (!b->insts.empty()) capture(b.get(), stream);
+    if (!b->insts.empty() || b->tiled_in_graph) b->capture(b->tiled_in_graph);
+    sync(stream);
     *out = b.release();
   });
 }
@@ -599,18 +748,8 @@ int wg_batch_run(wg_batch* b, void* stream_ptr) {
     if (b == nullptr) fail_runtime("batch is empty or destroyed");
     DeviceScope scope(b->device);
     hipStream_t stream = as_stream(stream_ptr);
-    if (b->exec) {
-      ProfileScope prof("batch_graph", stream);
-      check_hip(hipGraphLaunch(b->exec, stream), "launch graph");
-    }
-    // Tiled instances behind the graph on the same stream, one at a time
-    // (each synchronizes before its scratch buffers are released).
-    for (const auto& t : b->tiled) {
-      Scratch scratch(b->device);
-      adjoint_enqueue(t.op, t.psi, std::size_t{1} << t.op->num_qubits, t.coeffs, t.term_values,
-                      t.sigma, t.sigma_dag, stream, scratch, b->d_energies + t.slot);
-      sync(stream);
-    }
+    ProfileScope prof("batch_graph", stream);
+    b->run(stream);
   });
 }
 
@@ -629,9 +768,18 @@ int wg_batch_info(const wg_batch* b, int* tunables, size_t* units, size_t* items
       tunables[0] = b->fwd_log;
       tunables[1] = static_cast<int>(b->sigma_chunk);
       tunables[2] = static_cast<int>(b->grid);
+      tunables[3] = b->streams;
+      tunables[4] = b->tiled_in_graph ? 1 : 0;
     }
     if (units) *units = b->num_units;
     if (items) *items = b->items.size();
+  });
+}
+
+int wg_batch_tiled_count(const wg_batch* b, size_t* tiled) {
+  return guard([&] {
+    if (b == nullptr) fail_runtime("batch is empty or destroyed");
+    *tiled = b->tiled.size();
   });
 }
 

@@ -2437,6 +2437,8 @@ std::shared_ptr<ExpectationPlan> build_plan(const wg_pauli_sum* op, hipStream_t 
   plan->h_key_y = flat_y;
   plan->h_key_c = flat_c;
   plan->h_term_key = term_key;
+  plan->h_key_term_begin = key_term_begin;
+  plan->h_key_terms = key_terms;
   plan->tiled = plan->n >= kTiledMinQubits && !items.empty();
   std::vector<u32> offset, count;
   std::vector<SigmaKey> sigma_keys;
@@ -2714,6 +2716,92 @@ int wg_expectation(const wg_pauli_sum* op_c, const void* psi, int dtype, size_t 
 
 namespace wg {
 
+namespace {
+
+__global__ __launch_bounds__(256) void clear_c128_kernel(double2* __restrict__ out, std::size_t n) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    out[i] = make_double2(0.0, 0.0);
+  }
+}
+
+std::size_t align256(std::size_t b) { return (b + 255) & ~std::size_t{255}; }
+
+}  // namespace
+
+std::size_t tiled_adjoint_bytes(const ExpectationPlan& plan) {
+  const std::size_t K = std::max<std::size_t>(plan.num_keys, 1);
+  return forward_scratch_bytes(plan) + 3 * align256(K * sizeof(double2)) +
+         align256(std::max<std::size_t>(plan.sigma_keys, 1) * sizeof(SigmaKey));
+}
+
+void tiled_adjoint_enqueue(const ExpectationPlan& plan, const double2* op_coeffs, const double2* psi,
+                           const double2* coeffs, double2* term_values, double2* sigma,
+                           double2* sigma_dagger, double2* dout, void* buffer, hipStream_t stream) {
+  const std::size_t K = std::max<std::size_t>(plan.num_keys, 1);
+  auto* base = static_cast<unsigned char*>(buffer);
+  const std::size_t key_bytes = align256(K * sizeof(double));
+  auto* key_vals = reinterpret_cast<double*>(base);
+  auto* partials = reinterpret_cast<double2*>(base + key_bytes);
+  base += forward_scratch_bytes(plan);
+  auto* kc = reinterpret_cast<double2*>(base);
+  auto* cy = reinterpret_cast<double2*>(base + align256(K * sizeof(double2)));
+  auto* cy_dag = reinterpret_cast<double2*>(base + 2 * align256(K * sizeof(double2)));
+  auto* skeys = reinterpret_cast<SigmaKey*>(base + 3 * align256(K * sizeof(double2)));
+  const double2* c = coeffs ? coeffs : op_coeffs;
+  const std::size_t len = std::size_t{1} << plan.n;
+  // Forward sweeps for E and the per-key values (only when either is wanted),
+  // then sigma over the same passes (one read of psi and one sigma update per
+  // pass).
+  if (dout || term_values) {
+    forward_into(c, plan, psi, term_values, dout, key_vals, plan.num_items ? partials : nullptr, stream);
+  }
+  if (!sigma && !sigma_dagger) return;
+  const double2* key_c = plan.d_key_c;
+  if (coeffs) {
+    key_coeff_kernel<<<grid_for(plan.num_keys, 256), 256, 0, stream>>>(
+        c, plan.d_key_term_begin, plan.d_key_terms, plan.num_keys, kc);
+    check_launch("key_coeff");
+    key_c = kc;
+  }
+  key_cy_kernel<<<grid_for(plan.num_keys, 256), 256, 0, stream>>>(key_c, plan.d_key_y, plan.d_flat_key,
+                                                                   plan.num_keys, cy, cy_dag);
+  check_launch("key_cy");
+  // Per-call key records, then one sweep per pass: sigma and sigma^dagger
+  // together when both are wanted. The first launched pass writes sigma
+  // without reading it.
+  sigma_keys_kernel<<<grid_for(plan.sigma_keys, 256), 256, 0, stream>>>(
+      plan.d_sigma_keys, plan.sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
+  check_launch("sigma_keys");
+  double2* out = sigma ? sigma : sigma_dagger;
+  double2* out_dag = sigma && sigma_dagger ? sigma_dagger : nullptr;
+  const unsigned grid = 1U << plan.sigma_log_blocks;
+  u32 init = 1;
+  for (std::size_t p = 0; p < plan.passes.size(); ++p) {
+    const PassLaunch& pl = plan.passes[p];
+    if (pl.sigma_end == pl.sigma_begin) continue;
+    ProfileScope prof("sigma_tiled", stream);
+    if (out_dag) {
+      sigma_tiled_kernel<true><<<grid, kSigmaBlock, 0, stream>>>(
+          psi, plan.d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan.sigma_log_blocks, init, out,
+          out_dag);
+    } else {
+      sigma_tiled_kernel<false><<<grid, kSigmaBlock, 0, stream>>>(
+          psi, plan.d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan.sigma_log_blocks, init, out,
+          nullptr);
+    }
+    check_launch("sigma_tiled");
+    init = 0;
+  }
+  if (init) {
+    // No key records at all: sigma is zero (a fill kernel, so the sequence
+    // can be captured into a graph).
+    if (sigma) clear_c128_kernel<<<stride_grid(len, 256), 256, 0, stream>>>(sigma, len);
+    if (sigma_dagger) clear_c128_kernel<<<stride_grid(len, 256), 256, 0, stream>>>(sigma_dagger, len);
+    check_launch("clear_sigma");
+  }
+}
+
 void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const double* coeffs,
                      double* term_values, double* sigma, double* sigma_dagger, hipStream_t stream,
                      Scratch& scratch, double2* dout) {
@@ -2721,56 +2809,10 @@ void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const
   const double2* dpsi = reinterpret_cast<const double2*>(psi);
   const double2* c = coeffs ? reinterpret_cast<const double2*>(coeffs) : op->coeffs;
   if ((sigma || sigma_dagger) && plan->tiled) {
-    // Forward sweeps for E and the per-key values (only when either is
-    // wanted), then sigma over the same passes (one read of psi and one sigma
-    // update per pass).
-    if (dout || term_values) {
-      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
-    }
-    const double2* key_c = plan->d_key_c;
-    if (coeffs) {
-      double2* kc = scratch.take<double2>(plan->num_keys);
-      key_coeff_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
-          c, plan->d_key_term_begin, plan->d_key_terms, plan->num_keys, kc);
-      check_launch("key_coeff");
-      key_c = kc;
-    }
-    double2* cy = scratch.take<double2>(plan->num_keys);
-    double2* cy_dag = scratch.take<double2>(plan->num_keys);
-    key_cy_kernel<<<grid_for(plan->num_keys, 256), 256, 0, stream>>>(
-        key_c, plan->d_key_y, plan->d_flat_key, plan->num_keys, cy, cy_dag);
-    check_launch("key_cy");
-    // Per-call key records, then one sweep per pass: sigma and sigma^dagger
-    // together when both are wanted. The first launched pass writes sigma
-    // without reading it.
-    SigmaKey* skeys = scratch.take<SigmaKey>(plan->sigma_keys);
-    sigma_keys_kernel<<<grid_for(plan->sigma_keys, 256), 256, 0, stream>>>(
-        plan->d_sigma_keys, plan->sigma_keys, cy, cy_dag, sigma ? 0 : 1, skeys);
-    check_launch("sigma_keys");
-    double2* out = reinterpret_cast<double2*>(sigma ? sigma : sigma_dagger);
-    double2* out_dag = sigma && sigma_dagger ? reinterpret_cast<double2*>(sigma_dagger) : nullptr;
-    const unsigned grid = 1U << plan->sigma_log_blocks;
-    u32 init = 1;
-    for (std::size_t p = 0; p < plan->passes.size(); ++p) {
-      const PassLaunch& pl = plan->passes[p];
-      if (pl.sigma_end == pl.sigma_begin) continue;
-      ProfileScope prof("sigma_tiled", stream);
-      if (out_dag) {
-        sigma_tiled_kernel<true><<<grid, kSigmaBlock, 0, stream>>>(
-            dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
-            init, out, out_dag);
-      } else {
-        sigma_tiled_kernel<false><<<grid, kSigmaBlock, 0, stream>>>(
-            dpsi, plan->d_passes + p, skeys, pl.sigma_begin, pl.sigma_end, plan->sigma_log_blocks,
-            init, out, nullptr);
-      }
-      check_launch("sigma_tiled");
-      init = 0;
-    }
-    if (init) {
-      if (sigma) check_hip(hipMemsetAsync(sigma, 0, len * 16, stream), "clear sigma");
-      if (sigma_dagger) check_hip(hipMemsetAsync(sigma_dagger, 0, len * 16, stream), "clear sigma");
-    }
+    void* buf = scratch.take_bytes(tiled_adjoint_bytes(*plan));
+    tiled_adjoint_enqueue(*plan, op->coeffs, dpsi, reinterpret_cast<const double2*>(coeffs),
+                          reinterpret_cast<double2*>(term_values), reinterpret_cast<double2*>(sigma),
+                          reinterpret_cast<double2*>(sigma_dagger), dout, buf, stream);
   } else if (sigma || sigma_dagger) {
     // One fused sweep: sigma, sigma^dagger and the per-key values.
     const double2* key_c = plan->d_key_c;

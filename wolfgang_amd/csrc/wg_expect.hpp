@@ -191,6 +191,8 @@ struct ExpectationPlan {
   std::vector<u64> h_key_z;
   std::vector<int> h_key_y;
   std::vector<double2> h_key_c;
+  std::vector<u32> h_key_term_begin;  // [num_keys + 1] CSR over keys in group order
+  std::vector<u32> h_key_terms;       // [T] terms of each key, in term order
   std::vector<u32> h_term_key;
   ~ExpectationPlan();
 };
@@ -217,6 +219,18 @@ void forward_into(const double2* coeffs, const ExpectationPlan& plan, const doub
 void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const double* coeffs,
                      double* term_values, double* sigma, double* sigma_dagger, hipStream_t stream,
                      Scratch& scratch, double2* dout);
+
+// Device bytes tiled_adjoint_enqueue needs for `plan` (one buffer).
+std::size_t tiled_adjoint_bytes(const ExpectationPlan& plan);
+
+// Forward + adjoint of an operator with a tiled plan, all outputs optional:
+// E into the device location dout, per-term values, sigma = H psi and
+// H^dagger psi (coeffs, if non-null, replaces op_coeffs). Temporary buffers
+// come from `buffer` (tiled_adjoint_bytes bytes). No allocation, no memset
+// node, no synchronization: the sequence can be captured into a graph.
+void tiled_adjoint_enqueue(const ExpectationPlan& plan, const double2* op_coeffs, const double2* psi,
+                           const double2* coeffs, double2* term_values, double2* sigma,
+                           double2* sigma_dagger, double2* dout, void* buffer, hipStream_t stream);
 
 // Builds the tiled passes for the given items (host only; exposed for tests of
 // the planner through wg_expectation_plan_info). With diag_separate the

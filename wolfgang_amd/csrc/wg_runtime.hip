@@ -333,6 +333,13 @@ void resolve_pending_locked() {
 ProfileScope::ProfileScope(const char* name, hipStream_t stream) : name_(name), stream_(stream) {
   std::lock_guard<std::mutex> lk(g_profile_mutex);
   if (!g_profile_enabled) return;
+  // Launches being captured into a graph are not timed here.
+  hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream_, &capturing) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  if (capturing != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&start_) != hipSuccess || hipEventRecord(start_, stream_) != hipSuccess) {
     (void)hipGetLastError();
     start_ = nullptr;
