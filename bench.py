@@ -68,11 +68,12 @@ def parse_args() -> argparse.Namespace:
     p.add_argument(
         "--workload",
         default="expectation",
-        choices=["expectation", "commutation", "matmul", "batch", "products", "adjoint", "grouping",
-                 "copy"],
+        choices=["expectation", "expectation-parity", "commutation", "matmul", "batch", "products",
+                 "adjoint", "grouping", "copy"],
         help="expectation (the headline line, default); commutation / matmul: the 128-qubit "
         "algebra workloads; batch: batched forward+adjoint vs the per-instance path; products: "
-        "64 batched Hamiltonian products vs per-pair matmul; adjoint: "
+        "64 batched Hamiltonian products vs per-pair matmul; expectation-parity: the headline "
+        "energy against the oracle's x-grouped compensated evaluation on all host cores; adjoint: "
         "forward + fused adjoint at 24 qubits, 20,000 terms; copy: a "
         "plain device copy of known size, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE",
     )
@@ -80,15 +81,16 @@ def parse_args() -> argparse.Namespace:
 
 
 TRAFFIC_PROFILE = "profiles/expect_sweep_n30_chem100k.md"
+PARITY_PROFILE = "profiles/expect_full_energy_parity_n30_chem100k.md"
 
 
-def profile_figures(args) -> dict:
-    """key=value figures (HBM bytes per launch of the dominant sweep kernel,
-    its counter-derived VALU-busy and wait fractions) from the committed
-    rocprofv3 summary of the default workload; empty for any other
-    configuration."""
+def profile_figures(args, name: str = TRAFFIC_PROFILE) -> dict:
+    """key=value figures from a committed summary of the default workload (the
+    rocprofv3 summary: HBM bytes per launch of the dominant sweep kernel, its
+    counter-derived VALU-busy and wait fractions; the full-energy check: the
+    device and oracle energies); empty for any other configuration."""
     default = (args.num_qubits, args.num_terms, args.op_seed, args.psi_seed) == (30, 100_000, 3001, 3002)
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), name)
     out = {}
     if not default or not os.path.exists(path):
         return out
@@ -347,6 +349,50 @@ CPU_BASELINES = {"commutation": cpu_baseline_commutation, "matmul": cpu_baseline
                  "grouping": cpu_baseline_grouping, "adjoint": cpu_baseline_adjoint}
 
 
+def expectation_parity(args) -> int:
+    """The headline workload's energy on the device against the oracle's
+    x-grouped, chunk-paired double-double evaluation of every key over the
+    full state on the host's cores (a one-off check of a few minutes; progress
+    goes to stderr)."""
+    import torch
+
+    from wolfgang_amd.workloads import chemistry_masks, random_statevector_device
+
+    oracle_mod = _oracle_module()
+    n, T = args.num_qubits, args.num_terms
+    op = chemistry_masks(n, T, args.op_seed)
+    dop = op.to_device(0)
+    psi = random_statevector_device(n, args.psi_seed, 0)
+    e = dop.expectation_statevector(psi)
+    torch.cuda.synchronize()
+    psi_h = psi.cpu().numpy()
+    del psi
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+
+    def progress(done, total):
+        print(f"[parity] chunks {done}/{total} after {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+              flush=True)
+
+    ref = oracle_mod.oracle().expectation_grouped(n, op.x_words(), op.z_words(), op.coeffs(), psi_h,
+                                                  threads=threads, progress=progress, pieces=64)
+    dt = time.perf_counter() - t0
+    scale = float(np.abs(op.coeffs()).sum()) * float(np.vdot(psi_h, psi_h).real)
+    print(json.dumps({
+        "workload": "expectation-parity",
+        "config": {"num_qubits": n, "num_terms": T, "op_seed": args.op_seed, "psi_seed": args.psi_seed},
+        "gpu_energy": [e.real, e.imag],
+        "oracle_energy": [ref.real, ref.imag],
+        "abs_error": abs(e - ref),
+        "rel_error_vs_sum_abs_c_norm2": abs(e - ref) / scale,
+        "oracle": "x-grouped, chunk-paired double-double key values (oracle/wolfgang_oracle.cpp), "
+                  "products summed exactly rounded",
+        "oracle_threads": threads,
+        "oracle_s": dt,
+    }), flush=True)
+    return 0
+
+
 def copy_calibration(args) -> int:
     """A plain 4 GiB device-to-device copy (torch's copy kernel): the reference
     point for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -374,6 +420,8 @@ def main() -> int:
         return 2
     if args.workload == "copy":
         return copy_calibration(args)
+    if args.workload == "expectation-parity":
+        return expectation_parity(args)
     if args.workload != "expectation":
         from wolfgang_amd import bench_workloads
 
@@ -498,6 +546,19 @@ def main() -> int:
             "evaluation_definition_equivalent_GBs": bytes_per_step / kernel_s_per_step / 1e9,
         },
     }
+    full = profile_figures(args, PARITY_PROFILE)
+    if full:
+        # The full-energy check recorded for this workload applies to this run
+        # when the device reproduces the recorded energy bit for bit.
+        recorded = complex(full["gpu_energy_re"], full["gpu_energy_im"])
+        oracle_e = complex(full["oracle_energy_re"], full["oracle_energy_im"])
+        line["full_energy_check"] = {
+            "source": PARITY_PROFILE,
+            "oracle_energy": [oracle_e.real, oracle_e.imag],
+            "abs_error": abs(result - oracle_e),
+            "rel_error_vs_sum_abs_c": abs(result - oracle_e) / float(np.abs(op.coeffs()).sum()),
+            "energy_matches_recorded_bits": result == recorded,
+        }
     if not args.no_cpu_baseline:
         tv = torch.empty(T, dtype=torch.complex128, device=psi.device)
         dop.expectation_adjoint(psi.data_ptr(), psi.numel(), tv.data_ptr(), None, None,

@@ -15,7 +15,8 @@ the host's cores:
   (owner sweep) on a stratified sample -- at least one key of every tiled pass
   and every owner-bundle kind, plus diagonal keys (Walsh-Hadamard sweep) --
   within 1e-12 * ||psi||^2 of the oracle's compensated values; E equals the
-  per-term values summed with the coefficients.
+  per-term values summed with the coefficients, and E is within 1e-12 * sum|c|
+  of the oracle's x-grouped double-double evaluation of every key.
 * 24 qubits, 20,000 random terms: central finite differences (h = 1e-6) of a
   real loss through the autograd function on 4 coefficients and 4 amplitudes
   (real and imaginary parts), the same gradients against the oracle's
@@ -191,6 +192,10 @@ def test_c3_expectation_30_qubits_stratified_keys(gpu, orc):
     c = op.coeffs()
     tot = complex(math.fsum((c.real * got.real).tolist()), math.fsum((c.imag * got.real).tolist()))
     assert abs(e - tot) <= RTOL * scale(c)
+    say("C3 full energy: every key by the oracle's x-grouped evaluation")
+    want_e = orc.expectation_grouped(n, op.x_words(), op.z_words(), c, psi_h, threads=THREADS, pieces=8)
+    say(f"C3 energy error {abs(e - want_e):.3e}")
+    assert abs(e - want_e) <= RTOL * scale(c)
 
 
 @pytest.mark.timeout(300)

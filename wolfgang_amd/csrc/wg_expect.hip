@@ -19,7 +19,7 @@
 //   * direct (small states): every item streams psi[b], psi[b^x] from global
 //     memory; the whole state is cache resident.
 //   * tiled (large states): psi is cut into tiles that are cosets of a subspace
-//     U of the index space (the low 6 bits, some further coordinate bits and
+//     U of the index space (the low 3 bits, some further coordinate bits and
 //     some generator masks); a block stages one tile in LDS and each of its
 //     waves applies one item whose x lies in U. A pass covers every x-group in
 //     U with one read of psi from HBM, so the HBM traffic is (#passes) x |psi|
@@ -198,12 +198,13 @@ __device__ __forceinline__ double2 lds_at(const double2* tile, u32 byte_off) {
 }
 
 // Thread t stages points u = t + i * kTileBlock (i < 8) of a tile, i.e. the
-// segments wave + 8 i. Segment offsets are linear over GF(2) in the segment
-// index, so seg_off[wave + 8 i] = seg_off[wave] ^ (the offsets of the set bits
-// of 8 i): four uniform values per wave instead of eight.
+// segments (t >> 3) + 64 i at offset t & 7. Segment offsets are linear over
+// GF(2) in the segment index, so seg_off[(t >> 3) + 64 i] = seg_off[t >> 3] ^
+// (the offsets of the set bits of 64 i): one per-thread value and three
+// uniform ones instead of eight.
 constexpr u32 kSegStride = kTileBlock >> kTileLowLog;
-static_assert(kTilePoints / kTileBlock == 8 && kSegStride == 8,
-              "staging is written for 8 points per thread and 8 waves");
+static_assert(kTilePoints / kTileBlock == 8 && kSegStride == 64,
+              "staging is written for 8 points per thread and 64 segments per block");
 
 // The staged points live in eight named registers (an array or a struct
 // passed by reference may be left in scratch memory by the compiler).
@@ -232,8 +233,9 @@ __device__ __forceinline__ void sweep_tiles(const double2* __restrict__ psi,
                                             Work& work) {
   const u64 free_mask = pd->free_mask;
   const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
-  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[8], sb_e1 = pd->seg_off[16],
-            sb_e2 = pd->seg_off[32];
+  (void)wave;
+  const u64 sb_w = pd->seg_off[threadIdx.x >> kTileLowLog], sb_e0 = pd->seg_off[kSegStride],
+            sb_e1 = pd->seg_off[2 * kSegStride], sb_e2 = pd->seg_off[4 * kSegStride];
   const u32 lo = threadIdx.x & ((1U << kTileLowLog) - 1U);
   double2 st0, st1, st2, st3, st4, st5, st6, st7;
   u64 t = t0;
@@ -630,7 +632,7 @@ const std::array<TiledKernel, kVariants> kTiledKernels =
 //    representatives are registers i = j < 4 and partners
 //    (tid ^ x[0..8], j ^ x[9..11]) lie in registers 4..7, so only that half
 //    of the tile goes to LDS (owner_half_sweep);
-//  * lane pivot (x inside local bits 0..5): representatives are registers
+//  * lane pivot (x inside local bits 0..2): representatives are registers
 //    i = j + 4 h, h = bit `hbit` of tid, partners (tid ^ x, i); the whole tile
 //    is staged (owner_lane_sweep).
 // For representative j and key k the pair's sign is (-1)^{parity(z & rep(t))}
@@ -698,12 +700,16 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
   const u32 lane = tid & 63U;
   const u64 free_mask = pd->free_mask;
   const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  // Segment part of this thread's points: bits 0..2 of the segment index are
+  // lane bits 3..5, bits 3..5 the wave; bits 6..8 (the register index) are
+  // the uniform sb_e0..sb_e2.
   u64 sb_w = 0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if ((wave >> i) & 1U) sb_w ^= B.seg_basis[i];
+  for (int i = 0; i < 6; ++i) {
+    if ((tid >> (kTileLowLog + i)) & 1U) sb_w ^= B.seg_basis[i];
   }
-  const u64 sb_e0 = B.seg_basis[3], sb_e1 = B.seg_basis[4], sb_e2 = B.seg_basis[5];
+  const u64 sb_e0 = B.seg_basis[6], sb_e1 = B.seg_basis[7], sb_e2 = B.seg_basis[8];
+  const u32 lane_lo = lane & ((1U << kTileLowLog) - 1U);
   const u32 tid16 = tid << 4;
   u32 addr0[M], kw[M][NZ];
   u32 hmask = 0;  // bit m = bit hbit_m of tid
@@ -731,14 +737,14 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
   do {                                                                      \
     const u64 r0_ = (rep) ^ sb_w, r1_ = r0_ ^ sb_e0, r2_ = r0_ ^ sb_e1;      \
     const u64 r3_ = r1_ ^ sb_e1;                                            \
-    st0 = psi[r0_ | lane];                                                  \
-    st1 = psi[r1_ | lane];                                                  \
-    st2 = psi[r2_ | lane];                                                  \
-    st3 = psi[r3_ | lane];                                                  \
-    st4 = psi[(r0_ ^ sb_e2) | lane];                                        \
-    st5 = psi[(r1_ ^ sb_e2) | lane];                                        \
-    st6 = psi[(r2_ ^ sb_e2) | lane];                                        \
-    st7 = psi[(r3_ ^ sb_e2) | lane];                                        \
+    st0 = psi[r0_ | lane_lo];                                               \
+    st1 = psi[r1_ | lane_lo];                                               \
+    st2 = psi[r2_ | lane_lo];                                               \
+    st3 = psi[r3_ | lane_lo];                                               \
+    st4 = psi[(r0_ ^ sb_e2) | lane_lo];                                     \
+    st5 = psi[(r1_ ^ sb_e2) | lane_lo];                                     \
+    st6 = psi[(r2_ ^ sb_e2) | lane_lo];                                     \
+    st7 = psi[(r3_ ^ sb_e2) | lane_lo];                                     \
   } while (0)
   WG_OWNER_FETCH(rep);
   for (u64 step = 0; step < run; ++step) {
@@ -765,7 +771,7 @@ __device__ __forceinline__ void owner_lane_sweep(const double2* __restrict__ psi
     for (int m = 0; m < M; ++m) {
       const bool h = (hmask >> m) & 1U;
       const double2 a[4] = {h ? own4 : own0, h ? own5 : own1, h ? own6 : own2, h ? own7 : own3};
-      // The partners share the representatives' half (x is inside the lane bits).
+      // The partners share the representatives' half (x is inside the low bits).
       const double2* half = h ? hi : lo;
       const u32 base = tid16 ^ addr0[m];
       double2 q[4];
@@ -847,12 +853,16 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
   const u32 lane = tid & 63U;
   const u64 free_mask = pd->free_mask;
   const u64 ntiles = 1ULL << (pd->n - kTileMaxLog);
+  // Segment part of this thread's points: bits 0..2 of the segment index are
+  // lane bits 3..5, bits 3..5 the wave; bits 6..8 (the register index) are
+  // the uniform sb_e0..sb_e2.
   u64 sb_w = 0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    if ((wave >> i) & 1U) sb_w ^= B.seg_basis[i];
+  for (int i = 0; i < 6; ++i) {
+    if ((tid >> (kTileLowLog + i)) & 1U) sb_w ^= B.seg_basis[i];
   }
-  const u64 sb_e0 = B.seg_basis[3], sb_e1 = B.seg_basis[4], sb_e2 = B.seg_basis[5];
+  const u64 sb_e0 = B.seg_basis[6], sb_e1 = B.seg_basis[7], sb_e2 = B.seg_basis[8];
+  const u32 lane_lo = lane & ((1U << kTileLowLog) - 1U);
   const u32 tid16 = tid << 4;
   // Partner of representative j: byte (tid << 4) ^ haddr[m] ^ (j << 13) of the
   // half-buffer (register j ^ x[9..11] minus 4; x bit 11 is set).
@@ -874,15 +884,15 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
   double2 nx0, nx1, nx2, nx3;
   auto issue = [&](u64 r, double2* buf) {
     const u64 r0 = r ^ sb_w, r1 = r0 ^ sb_e0, r2 = r0 ^ sb_e1, r3 = r1 ^ sb_e1;
-    nx0 = psi[r0 | lane];
-    nx1 = psi[r1 | lane];
-    nx2 = psi[r2 | lane];
-    nx3 = psi[r3 | lane];
+    nx0 = psi[r0 | lane_lo];
+    nx1 = psi[r1 | lane_lo];
+    nx2 = psi[r2 | lane_lo];
+    nx3 = psi[r3 | lane_lo];
     double2* half = buf + 64U * wave;
-    __builtin_amdgcn_global_load_lds(psi + ((r0 ^ sb_e2) | lane), (lds_void*)(half + 0 * kOwnerBlock), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(psi + ((r1 ^ sb_e2) | lane), (lds_void*)(half + 1 * kOwnerBlock), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(psi + ((r2 ^ sb_e2) | lane), (lds_void*)(half + 2 * kOwnerBlock), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(psi + ((r3 ^ sb_e2) | lane), (lds_void*)(half + 3 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r0 ^ sb_e2) | lane_lo), (lds_void*)(half + 0 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r1 ^ sb_e2) | lane_lo), (lds_void*)(half + 1 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r2 ^ sb_e2) | lane_lo), (lds_void*)(half + 2 * kOwnerBlock), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(psi + ((r3 ^ sb_e2) | lane_lo), (lds_void*)(half + 3 * kOwnerBlock), 16, 0, 0);
   };
   // One step: this wave's loads of tile `step` have landed; the barrier makes
   // every wave's half-buffer writes visible and retires tile step - 1's reads
@@ -1036,7 +1046,7 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
 // owner sweep). The tile is double-buffered in LDS (2 x 64 KiB): global_load_lds
 // fills the other buffer with the next tile while this one is evaluated, so a
 // tile costs one barrier. Thread tid owns the points u = tid + 1024 I (I < 4),
-// which are segment wave + 16 I, lane `lane`: the same addresses it copies into
+// which are segment (tid >> 3) + 128 I, offset tid & 7: the same addresses it copies into
 // LDS, reads sigma from and writes sigma back to (one read-modify-write per
 // point per pass; the first launched pass writes without reading). The current
 // sigma values are loaded at the start of a tile and the sums are stored after
@@ -1049,8 +1059,8 @@ void expect_owner_kernel(const double2* __restrict__ psi, const PassDesc* __rest
 constexpr int kSigmaBlock = 1024;
 constexpr int kSigmaPoints = kTilePoints / kSigmaBlock;
 
-static_assert(kSigmaPoints == 4 && (kSigmaBlock >> kTileLowLog) == 16,
-              "sigma sweep is written for 4 points per thread and 16 waves");
+static_assert(kSigmaPoints == 4 && (kSigmaBlock >> kTileLowLog) == 128,
+              "sigma sweep is written for 4 points per thread and 128 segments per block");
 
 __device__ __forceinline__ double flip_sign(double v, u32 m) {
   return __hiloint2double(__double2hiint(v) ^ static_cast<int>(m), __double2loint(v));
@@ -1090,12 +1100,15 @@ __global__ __launch_bounds__(kSigmaBlock) void sigma_tiled_kernel(
   const u32 lane = tid & 63U;
   const u64 free_mask = pd->free_mask;
   const u64 run = (1ULL << (pd->n - kTileMaxLog)) >> log_blocks;
-  const u64 sb_w = pd->seg_off[wave], sb_e0 = pd->seg_off[16], sb_e1 = pd->seg_off[32];
+  // Point I of thread tid is local point tid + 1024 I: segment (tid >> 3) +
+  // 128 I at offset tid & 7.
+  const u64 sb_w = pd->seg_off[tid >> kTileLowLog], sb_e0 = pd->seg_off[128], sb_e1 = pd->seg_off[256];
+  const u32 lane_lo = lane & ((1U << kTileLowLog) - 1U);
   if (tid < 32U) step_mask[tid] = tile_rep(1ULL << (tid + log_blocks), free_mask);
   u64 rep = tile_rep(blockIdx.x, free_mask);
   // Global index of point I of this thread in the tile with representative r.
 #define WG_PT(r, I) \
-  ((((r) ^ sb_w ^ (((I) & 1) ? sb_e0 : 0) ^ (((I) & 2) ? sb_e1 : 0))) | lane)
+  ((((r) ^ sb_w ^ (((I) & 1) ? sb_e0 : 0) ^ (((I) & 2) ? sb_e1 : 0))) | lane_lo)
   auto issue = [&](u64 r, u32 buf) {
     double2* seg = tile + buf * kTilePoints + 64U * wave;
     __builtin_amdgcn_global_load_lds(psi + WG_PT(r, 0), (lds_void*)(seg + 0 * kSigmaBlock), 16, 0, 0);
@@ -1973,8 +1986,8 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
         ExpItem e = it;
         e.xl = is_diag ? 0 : x_local(it.x);
         // Pivot: the highest local bit of x. Any set bit of x selects one
-        // representative per pair {b, b ^ x}; a pivot above the low 6 bits
-        // keeps each wave's LDS reads on 64 consecutive points.
+        // representative per pair {b, b ^ x}; a pivot above the low 6 local
+        // bits keeps each wave's LDS reads on 64 consecutive points.
         e.pbit = is_diag ? 0 : (1ULL << (63 - __builtin_clzll(e.xl)));
         for (int j = 0; j < e.nz; ++j) e.zl[j] = z_local(it.z[j]);
         assign_step_classes(e, 0);
@@ -2044,49 +2057,52 @@ namespace {
 // s = A s' the basis images are A e'_i = e_{pi(i)} (+ e_p if lam has bit
 // pi(i)) for i < 5 and A e'_5 = e_p.
 struct SegLayout {
-  u32 img[6];  // A e'_i as 6-bit old segment vectors
+  u32 img[kSegBits];  // A e'_i as old segment vectors
   u32 lam;
   int p;
-  int pi[5];
+  int pi[kSegBits - 1];
 };
+
+constexpr u32 kSegMask = (1U << kSegBits) - 1;
 
 SegLayout seg_layout(u32 lam) {
   SegLayout L{};
   L.lam = lam;
   if (lam == 0) {
-    for (int i = 0; i < 6; ++i) L.img[i] = 1U << i;
+    for (int i = 0; i < kSegBits; ++i) L.img[i] = 1U << i;
     L.p = -1;
     return L;
   }
   L.p = 31 - __builtin_clz(lam);
   int c = 0;
-  for (int i = 0; i < 6; ++i) {
+  for (int i = 0; i < kSegBits; ++i) {
     if (i != L.p) L.pi[c++] = i;
   }
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < kSegBits - 1; ++i) {
     L.img[i] = (1U << L.pi[i]) | (((lam >> L.pi[i]) & 1U) << L.p);
   }
-  L.img[5] = 1U << L.p;
+  L.img[kSegBits - 1] = 1U << L.p;
   return L;
 }
 
-// x (12-bit pass-local) in the bundle layout.
+// x (12-bit pass-local) in the bundle layout: the segment part's last
+// coordinate becomes lam(x).
 u32 layout_x(const SegLayout& L, u32 xl) {
   if (L.p < 0) return xl;
-  const u32 s = (xl >> 6) & 63U;
+  const u32 s = (xl >> kTileLowLog) & kSegMask;
   u32 s2 = 0;
-  for (int i = 0; i < 5; ++i) s2 |= ((s >> L.pi[i]) & 1U) << i;
-  s2 |= static_cast<u32>(host_parity(L.lam & s)) << 5;
-  return (xl & 63U) | (s2 << 6);
+  for (int i = 0; i < kSegBits - 1; ++i) s2 |= ((s >> L.pi[i]) & 1U) << i;
+  s2 |= static_cast<u32>(host_parity(L.lam & s)) << (kSegBits - 1);
+  return (xl & ((1U << kTileLowLog) - 1U)) | (s2 << kTileLowLog);
 }
 
 // z character (12-bit pass-local) in the bundle layout: bit i of the new
 // segment part is parity(zl_seg & A e'_i).
 u32 layout_z(const SegLayout& L, u32 zl) {
-  const u32 s = (zl >> 6) & 63U;
+  const u32 s = (zl >> kTileLowLog) & kSegMask;
   u32 s2 = 0;
-  for (int i = 0; i < 6; ++i) s2 |= static_cast<u32>(host_parity(s & L.img[i])) << i;
-  return (zl & 63U) | (s2 << 6);
+  for (int i = 0; i < kSegBits; ++i) s2 |= static_cast<u32>(host_parity(s & L.img[i])) << i;
+  return (zl & ((1U << kTileLowLog) - 1U)) | (s2 << kTileLowLog);
 }
 
 u64 host_compress(u64 v, u64 mask) {
@@ -2148,7 +2164,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
           c.ncols = static_cast<int>(std::min(per, cols.size() - s));
           for (int q = 0; q < c.ncols; ++q) c.cols[q] = cols[s + q];
           c.wide = wide;
-          c.lanex = ((c.xl >> 6) & 63U) == 0 ? 1 : 0;
+          c.lanex = ((c.xl >> kTileLowLog) & kSegMask) == 0 ? 1 : 0;
           cands.push_back(c);
         }
       }
@@ -2171,7 +2187,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
         const int kind = shape * 4 + base_kind;
         const bool lanex = (base_kind >> 1) & 1;
         OwnerBundle bd{};
-        for (int i = 0; i < 6; ++i) bd.seg_basis[i] = d.seg_off[L.img[i]];
+        for (int i = 0; i < kSegBits; ++i) bd.seg_basis[i] = d.seg_off[L.img[i]];
         bd.kind = static_cast<u32>(kind);
         for (int m = 0; m < M; ++m) {
           OwnerSlot& S = bd.slot[m];
@@ -2186,7 +2202,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
           S.item = c.item;
           const u32 xt16 = (xl & 511U) << 4, xr = (xl >> 9) & 7U;
           if (c.lanex) {
-            if (xl >= 64U) fail_runtime("lane-pivot item has x outside the lane bits");
+            if (xl >= (1U << kTileLowLog)) fail_runtime("lane-pivot item has x outside the low bits");
             S.hbit = static_cast<u32>(31 - __builtin_clz(xl));
             S.addr0 = xt16;
           } else {
@@ -2231,9 +2247,9 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
         while (!pool.empty()) {
           u32 best = 1;
           std::size_t best_n = 0;
-          for (u32 lam = 1; lam < 64; ++lam) {
+          for (u32 lam = 1; lam <= kSegMask; ++lam) {
             std::size_t cnt = 0;
-            for (const Cand& c : pool) cnt += host_parity(lam & ((c.xl >> 6) & 63U));
+            for (const Cand& c : pool) cnt += host_parity(lam & ((c.xl >> kTileLowLog) & kSegMask));
             if (cnt > best_n) {
               best_n = cnt;
               best = lam;
@@ -2242,7 +2258,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
           const std::size_t quota = best_n >= full ? best_n / full * full : best_n;
           std::vector<Cand> take, rest;
           for (const Cand& c : pool) {
-            const bool covered = host_parity(best & ((c.xl >> 6) & 63U));
+            const bool covered = host_parity(best & ((c.xl >> kTileLowLog) & kSegMask));
             (covered && take.size() < quota ? take : rest).push_back(c);
           }
           emit(take, seg_layout(best), base_kind, wide);
@@ -3253,16 +3269,16 @@ int wg_debug_tiled_plan(size_t num_qubits, size_t num_terms, const uint64_t* x, 
           const bool lanex = (bd.kind >> 1) & 1, odd = bd.kind & 1;
           auto point = [&](u32 u) {
             u64 off = 0;
-            for (int i = 0; i < 6; ++i) {
-              if ((u >> (6 + i)) & 1U) off ^= bd.seg_basis[i];
+            for (int i = 0; i < kSegBits; ++i) {
+              if ((u >> (kTileLowLog + i)) & 1U) off ^= bd.seg_basis[i];
             }
-            return (rep ^ off) | (u & 63U);
+            return (rep ^ off) | (u & ((1U << kTileLowLog) - 1U));
           };
           if (t == 0) {
             std::vector<u64> pts(kTilePoints), ref(kTilePoints);
             for (u32 u = 0; u < kTilePoints; ++u) {
               pts[u] = point(u);
-              ref[u] = (rep ^ d.seg_off[u >> d.l]) | (u & 63U);
+              ref[u] = (rep ^ d.seg_off[u >> d.l]) | (u & ((1U << d.l) - 1U));
             }
             std::sort(pts.begin(), pts.end());
             std::sort(ref.begin(), ref.end());

@@ -15,11 +15,15 @@ constexpr int kMaxClassLog = 3;
 constexpr unsigned char kNoClasses = 0xFF;
 
 // Largest tile of the tiled pass: 2^kTileMaxLog amplitudes (64 KiB of complex128
-// in LDS, two blocks per CU) and the contiguous low-bit run inside it
-// (2^kTileLowLog amplitudes = 1 KiB per segment).
+// in LDS) and the contiguous low-bit run inside it (2^kTileLowLog amplitudes =
+// 128 bytes, one full cache line, per segment). The other kTileMaxLog -
+// kTileLowLog local bits (the segment coordinates) map to any directions of the
+// pass's subspace, including the thread-lane bits 3..5: every memory
+// instruction of a wave still moves whole 128-byte lines.
 constexpr int kTileMaxLog = 12;
-constexpr int kTileLowLog = 6;
-constexpr int kMaxSegments = 1 << (kTileMaxLog - kTileLowLog);
+constexpr int kTileLowLog = 3;
+constexpr int kSegBits = kTileMaxLog - kTileLowLog;
+constexpr int kMaxSegments = 1 << kSegBits;
 
 // One x-group (or a slice of up to kZ of its keys).
 struct ExpItem {
@@ -61,9 +65,9 @@ struct PassDesc {
 // 512-thread block owns the tile points u = tid + 512 i (i < 8, local bits
 // 9..11 = i) in registers and evaluates every pair {u, u ^ x} of an item once,
 // from the point whose pivot bit is clear, reading only the partner from LDS.
-// Each bundle may change the basis of the 6 segment coordinates (local bits
-// 6..11) so that every item of a register-pivot bundle has local bit 11 set;
-// items inside the low 6 bits use a lane pivot.
+// Each bundle may change the basis of the 9 segment coordinates (local bits
+// 3..11) so that every item of a register-pivot bundle has local bit 11 set;
+// items inside the low 3 bits use a lane pivot.
 constexpr int kOwnerMaxSlots = 9;
 constexpr int kOwnerMaxKeys = 4;
 
@@ -78,7 +82,7 @@ struct OwnerKey {
 
 struct OwnerSlot {
   u32 item;   // item index relative to the pass's first item
-  u32 hbit;   // lane pivot: the pivot bit of x (< 6)
+  u32 hbit;   // lane pivot: the pivot bit of x (< 3)
   u32 addr0;  // (x local bits 0..8) << 4 | (x local bits 9..11) << 13: the partner of
               // representative j of thread tid is at byte (tid << 4) ^ addr0 ^ (j << 13)
               // (register pivot) or ((tid << 4) ^ addr0 | h << 15) + (j << 13) (lane pivot)
@@ -87,7 +91,7 @@ struct OwnerSlot {
 };
 
 struct OwnerBundle {
-  u64 seg_basis[6];  // global offsets of the 6 segment coordinates in this layout
+  u64 seg_basis[kSegBits];  // global offsets of the 9 segment coordinates in this layout
   u32 kind;          // bit 0 odd |x & z|, bit 1 lane pivot, bits 2.. shape
   u32 pad;
   OwnerSlot slot[kOwnerMaxSlots];
