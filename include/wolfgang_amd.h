@@ -57,6 +57,21 @@ int wg_device_info(int ordinal, char* name, size_t name_len, char* gfx_target, s
                    size_t* total_memory_bytes, int* compute_units);
 int wg_runtime_versions(int* runtime_version, int* driver_version);
 
+/* Device memory cache (no reference counterpart; the reference allocates and
+ * frees per call). Device buffers of operators, matrices, plans, batches and
+ * scratch are kept after they are freed, up to a byte limit per device
+ * (default: a quarter of the device's memory), and reused for requests of the
+ * same size class; a block re-enters the cache only after the device has
+ * finished its queued work. wg_memory_cache_limit sets the limit for every
+ * device (SIZE_MAX: back to the default) and trims the caches to it;
+ * wg_memory_cache_release returns every cached block of a device to the
+ * runtime; wg_memory_cache_info reports cached bytes and blocks and the bytes
+ * handed out and not yet freed. */
+int wg_memory_cache_limit(size_t bytes, size_t* previous);
+int wg_memory_cache_release(int device);
+int wg_memory_cache_info(int device, size_t* cached_bytes, size_t* cached_blocks,
+                         size_t* live_bytes);
+
 /* DevicePauliSum::from_host (reference include/wolfgang/device_pauli_sum.hpp:104-108;
  * src/hip/device_pauli_sum.hip.cpp:116-157). coeffs holds 2*num_terms doubles. */
 int wg_pauli_sum_create(size_t num_qubits, size_t num_terms, const uint64_t* x, const uint64_t* z,
@@ -123,6 +138,14 @@ int wg_comm_matrix_to_host_u8(const wg_comm_matrix* m, uint8_t* out, size_t out_
 /* Packed words to host: out_words = rows*row_words. */
 int wg_comm_matrix_to_host_packed(const wg_comm_matrix* m, uint64_t* out, size_t out_words,
                                   void* stream);
+
+/* copy_device_commutation_matrix_from_host_for_testing (reference
+ * include/wolfgang/device_commutation_matrix.hpp:64-66,
+ * src/hip/device_commutation_matrix.hip.cpp:372-391): overwrite the matrix
+ * with len = rows*cols host bytes, row-major; a nonzero byte sets its bit.
+ * Synchronizes `stream` before returning. */
+int wg_comm_matrix_from_host_u8(wg_comm_matrix* m, const uint8_t* values, size_t len,
+                                void* stream);
 
 /* count_commuting / count_commuting_rows / count_commuting_cols and
  * conflict_degrees (reference include/wolfgang/device_commutation_matrix.hpp:37-39;

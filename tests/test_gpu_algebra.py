@@ -77,6 +77,30 @@ def test_commutation_edges_and_errors(gpu):
     assert z0.commutes_with(z0).all()
 
 
+def test_commutation_overwrites_poisoned_output(gpu, orc):
+    """The reference's poison check (benchmarks/bench_metal_kernels.py:946):
+    a matrix filled from the host through the test hook is fully overwritten
+    by commutes_with_device(output=...)."""
+    from wolfgang_amd.pauli_sum import _copy_device_commutation_matrix_from_host_for_testing as poison
+
+    a = random_pauli_masks(100, 70, 0.3, 5)
+    b = random_pauli_masks(100, 130, 0.3, 6)
+    da, db = a.to_device(), b.to_device()
+    m = DeviceCommutationMatrix.empty((70, 130))
+    rng = np.random.default_rng(7)
+    flags = rng.integers(0, 3, size=(70, 130), dtype=np.uint8)
+    poison(m, flags)
+    np.testing.assert_array_equal(m.to_host(), flags != 0)
+    assert m.count_commuting() == int((flags != 0).sum())
+    # Bits past the last column stay zero: the per-row counts see only real columns.
+    np.testing.assert_array_equal(m.count_commuting(axis=1), (flags != 0).sum(axis=1))
+    poison(m, np.ones((70, 130), dtype=np.uint8))
+    da.commutes_with_device(db, output=m)
+    np.testing.assert_array_equal(m.to_host(), orc.commutes_with(100, xz(a), xz(b)).astype(bool))
+    with pytest.raises(ValueError, match="testing copy size does not match matrix entries"):
+        poison(m, np.ones(5, dtype=np.uint8))
+
+
 # ------------------------------------------------------------ products
 
 @pytest.mark.parametrize("n", [1, 16, 64, 65, 128, 200])

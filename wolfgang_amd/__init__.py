@@ -19,6 +19,9 @@ from .pauli_sum import (
     hip_status,
     metal_available,
     metal_devices,
+    memory_cache_info,
+    empty_cache,
+    set_memory_cache_limit,
 )
 
 __all__ = [
@@ -34,6 +37,9 @@ __all__ = [
     "metal_devices",
     "checked_matmul_intermediate_terms",
     "checked_commutation_entries",
+    "memory_cache_info",
+    "empty_cache",
+    "set_memory_cache_limit",
 ]
 
 __version__ = "0.1.0"

@@ -58,6 +58,10 @@ SIGNATURES = {
     "wg_commutes_with_device": (C.c_int, [_vp, _vp, _vp, _size, _vp]),
     "wg_comm_matrix_to_host_u8": (C.c_int, [_vp, _vp, _size, _vp]),
     "wg_comm_matrix_to_host_packed": (C.c_int, [_vp, _vp, _size, _vp]),
+    "wg_comm_matrix_from_host_u8": (C.c_int, [_vp, _vp, _size, _vp]),
+    "wg_memory_cache_limit": (C.c_int, [_size, C.POINTER(_size)]),
+    "wg_memory_cache_release": (C.c_int, [C.c_int]),
+    "wg_memory_cache_info": (C.c_int, [C.c_int, C.POINTER(_size), C.POINTER(_size), C.POINTER(_size)]),
     "wg_comm_matrix_count": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _size, _vp]),
     "wg_expectation": (C.c_int, [_vp, _vp, C.c_int, _size, C.c_int, _vp, _dblp]),
     "wg_expectation_adjoint": (C.c_int, [_vp, _vp, _size, _vp, _vp, _vp, _vp, _vp, _dblp]),

@@ -81,9 +81,19 @@ def build_oracle(reference: str = "/root/reference") -> None:
         )
 
 
+def build_facade_check(reference: str = "/root/reference") -> None:
+    """Test program linking the C++ facade into the reference's own API
+    (oracle/_ref/facade_check); needs the reference tree and the built library."""
+    if pathlib.Path(reference, "include", "wolfgang", "device_pauli_sum.hpp").exists():
+        subprocess.run(
+            ["make", "-s", "-C", str(ROOT / "oracle"), "facade", f"REF={reference}"], check=True
+        )
+
+
 def main() -> int:
     build_oracle()
     lib = build_library()
+    build_facade_check()
     print(f"built {lib}")
     return 0
 

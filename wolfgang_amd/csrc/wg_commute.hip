@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "wg_internal.hpp"
 
@@ -413,6 +414,33 @@ int wg_comm_matrix_to_host_packed(const wg_comm_matrix* m, uint64_t* out, size_t
     hipStream_t stream = as_stream(stream_ptr);
     check_hip(hipMemcpyAsync(out, m->bits, words * sizeof(u64), hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
+    sync(stream);
+  });
+}
+
+int wg_comm_matrix_from_host_u8(wg_comm_matrix* m, const uint8_t* values, size_t len,
+                                void* stream_ptr) {
+  return guard([&] {
+    if (m == nullptr) fail_runtime("DeviceCommutationMatrix is empty or moved-from");
+    if (len != m->rows * m->cols) {
+      fail_invalid("DeviceCommutationMatrix testing copy size does not match matrix entries");
+    }
+    if (len == 0) return;
+    if (values == nullptr) fail_invalid("testing copy source must be non-null");
+    // Pack on the host: entry (i, j) nonzero -> bit j % 64 of word j / 64 of row i.
+    std::vector<u64> packed(m->rows * m->row_words, 0);
+    for (std::size_t i = 0; i < m->rows; ++i) {
+      const uint8_t* row = values + i * m->cols;
+      u64* dst = packed.data() + i * m->row_words;
+      for (std::size_t j = 0; j < m->cols; ++j) {
+        if (row[j] != 0) dst[j >> 6] |= u64{1} << (j & 63);
+      }
+    }
+    DeviceScope scope(m->device);
+    hipStream_t stream = as_stream(stream_ptr);
+    check_hip(hipMemcpyAsync(m->bits, packed.data(), packed.size() * sizeof(u64),
+                             hipMemcpyHostToDevice, stream),
+              "DeviceCommutationMatrix testing copy");
     sync(stream);
   });
 }

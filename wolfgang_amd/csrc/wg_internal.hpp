@@ -105,18 +105,29 @@ inline std::size_t checked_mul(std::size_t a, std::size_t b, const char* what) {
 inline std::size_t words_for(std::size_t n) { return n == 0 ? 0 : (n + 63) / 64; }
 
 // ------------------------------------------------------------------ memory
+// Device memory goes through a per-device block cache: a freed block is kept
+// (up to a byte limit per device) and handed to the next request of the same
+// size class, so steady-state loops (products, batches, operators rebuilt per
+// step) make no hipMalloc / hipFree calls; mapping and unmapping large buffers
+// costs milliseconds each. A block is returned to the cache only after the
+// device has finished all queued work (as hipFree would), so reuse never races
+// a kernel still reading it. `action` names the allocation in the error.
+void* device_alloc_bytes(std::size_t bytes, const char* action = "allocate device buffer");
+void device_free(void* p);
+
 template <typename T>
 T* device_alloc(std::size_t count) {
   if (count == 0) return nullptr;
-  void* p = nullptr;
-  check_hip(hipMalloc(&p, checked_mul(count, sizeof(T), "device buffer")),
-            "allocate device buffer");
-  return static_cast<T*>(p);
+  return static_cast<T*>(device_alloc_bytes(checked_mul(count, sizeof(T), "device buffer")));
 }
 
-inline void device_free(void* p) {
-  if (p != nullptr) (void)hipFree(p);
-}
+// Cache control: give every cached block of `device` back to the runtime; set
+// the per-device byte limit (default: a quarter of the device's memory; returns
+// the previous override, SIZE_MAX if none); cached bytes, cached blocks and
+// bytes handed out and not yet freed on `device`.
+void memory_cache_release(int device);
+std::size_t memory_cache_set_limit(std::size_t bytes);
+void memory_cache_info(int device, std::size_t* cached, std::size_t* blocks, std::size_t* live);
 
 // Per-device growable scratch arena. A call takes the device lock for its whole
 // duration (all entry points synchronize before returning), carves aligned

@@ -47,6 +47,179 @@ void require_same_device(const char* operation, const wg_pauli_sum* a, const wg_
                std::to_string(b->device) + ", failure_stage=pre_allocation)");
 }
 
+// ------------------------------------------------------------ block cache
+
+namespace {
+
+struct BlockCache {
+  std::mutex mutex;
+  struct Block {
+    int device;
+    std::size_t size;  // size class actually allocated
+  };
+  std::map<void*, Block> live;                                     // every block we own
+  std::map<std::pair<int, std::size_t>, std::vector<void*>> free;  // cached, by (device, class)
+  std::map<int, std::size_t> cached_bytes;
+  std::map<int, std::size_t> limit;  // per device; set on first use
+  std::size_t limit_override = SIZE_MAX;  // wg_memory_cache_limit
+};
+
+BlockCache& block_cache() {
+  static auto* c = new BlockCache();  // lives until process exit
+  return *c;
+}
+
+// Size classes: 256-byte steps up to 4 KiB, then 8 steps per power of two
+// (at most 12.5 % rounding).
+std::size_t size_class(std::size_t b) {
+  if (b <= 4096) return (b + 255) & ~std::size_t{255};
+  const int hi = 64 - __builtin_clzll(b - 1);  // 2^(hi-1) < b <= 2^hi
+  const std::size_t step = std::size_t{1} << (hi - 3);
+  return (b + step - 1) & ~(step - 1);
+}
+
+std::size_t default_limit(int device) {
+  std::size_t free_b = 0, total_b = 0;
+  DeviceScope scope(device);
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return std::size_t{8} << 30;
+  }
+  return total_b / 4;
+}
+
+// Frees cached blocks of `device` (largest first) until at most `keep` bytes
+// stay cached. Caller holds the cache mutex.
+void trim_locked(BlockCache& c, int device, std::size_t keep) {
+  auto& cached = c.cached_bytes[device];
+  auto it = c.free.lower_bound({device + 1, 0});
+  while (cached > keep && it != c.free.begin()) {
+    --it;
+    if (it->first.first != device) break;
+    auto& list = it->second;
+    while (cached > keep && !list.empty()) {
+      void* p = list.back();
+      list.pop_back();
+      cached -= it->first.second;
+      c.live.erase(p);
+      (void)hipFree(p);
+    }
+    if (list.empty()) it = c.free.erase(it);
+  }
+  (void)hipGetLastError();
+}
+
+}  // namespace
+
+void* device_alloc_bytes(std::size_t bytes, const char* action) {
+  if (bytes == 0) return nullptr;
+  int device = 0;
+  check_hip(hipGetDevice(&device), "get current device");
+  const std::size_t cls = size_class(bytes);
+  BlockCache& c = block_cache();
+  {
+    std::lock_guard<std::mutex> lk(c.mutex);
+    auto it = c.free.find({device, cls});
+    if (it != c.free.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      c.cached_bytes[device] -= cls;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t r = hipMalloc(&p, cls);
+  if (r == hipErrorOutOfMemory) {
+    // Give the cached blocks of this device back to the runtime and retry once.
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> lk(c.mutex);
+      trim_locked(c, device, 0);
+    }
+    r = hipMalloc(&p, cls);
+  }
+  check_hip(r, action);
+  std::lock_guard<std::mutex> lk(c.mutex);
+  c.live[p] = BlockCache::Block{device, cls};
+  return p;
+}
+
+void device_free(void* p) {
+  if (p == nullptr) return;
+  BlockCache& c = block_cache();
+  BlockCache::Block b{};
+  {
+    std::lock_guard<std::mutex> lk(c.mutex);
+    auto it = c.live.find(p);
+    if (it == c.live.end()) {
+      (void)hipFree(p);  // not from this cache
+      (void)hipGetLastError();
+      return;
+    }
+    b = it->second;
+  }
+  {
+    // Work queued on the device may still use the block: wait for it, as
+    // hipFree does, before the block can be handed out again.
+    int prev = -1;
+    const bool have_prev = hipGetDevice(&prev) == hipSuccess;
+    if (!have_prev || prev != b.device) (void)hipSetDevice(b.device);
+    (void)hipDeviceSynchronize();
+    if (have_prev && prev != b.device) (void)hipSetDevice(prev);
+    (void)hipGetLastError();
+  }
+  std::lock_guard<std::mutex> lk(c.mutex);
+  auto lim = c.limit.find(b.device);
+  if (lim == c.limit.end()) {
+    lim = c.limit.emplace(b.device, default_limit(b.device)).first;
+  }
+  const std::size_t limit = std::min(lim->second, c.limit_override);
+  if (b.size > limit) {
+    c.live.erase(p);
+    (void)hipFree(p);
+    (void)hipGetLastError();
+    return;
+  }
+  c.free[{b.device, b.size}].push_back(p);
+  c.cached_bytes[b.device] += b.size;
+  if (c.cached_bytes[b.device] > limit) trim_locked(c, b.device, limit);
+}
+
+void memory_cache_release(int device) {
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mutex);
+  trim_locked(c, device, 0);
+}
+
+std::size_t memory_cache_set_limit(std::size_t bytes) {
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mutex);
+  const std::size_t prev = c.limit_override;
+  c.limit_override = bytes;
+  for (auto& [device, cached] : c.cached_bytes) {
+    const auto lim = c.limit.find(device);
+    const std::size_t l = std::min(lim == c.limit.end() ? SIZE_MAX : lim->second, bytes);
+    if (cached > l) trim_locked(c, device, l);
+  }
+  return prev;
+}
+
+void memory_cache_info(int device, std::size_t* cached, std::size_t* blocks, std::size_t* live) {
+  BlockCache& c = block_cache();
+  std::lock_guard<std::mutex> lk(c.mutex);
+  std::size_t owned = 0, free_blocks = 0;
+  for (const auto& [key, list] : c.free) {
+    if (key.first == device) free_blocks += list.size();
+  }
+  for (const auto& [ptr, blk] : c.live) {
+    if (blk.device == device) owned += blk.size;
+  }
+  const auto it = c.cached_bytes.find(device);
+  *cached = it == c.cached_bytes.end() ? 0 : it->second;
+  *blocks = free_blocks;
+  *live = owned - *cached;  // bytes in blocks handed out and not yet freed
+}
+
 struct Scratch::Arena {
   std::mutex mutex;
   void* base = nullptr;
@@ -76,14 +249,13 @@ Scratch::~Scratch() {
     device_free(arena_->base);
     arena_->base = nullptr;
     arena_->capacity = 0;
-    void* p = nullptr;
     const std::size_t top = std::max(high_, used_);
     const std::size_t want = top + top / 4;
-    if (hipMalloc(&p, want) == hipSuccess) {
-      arena_->base = p;
+    try {
+      arena_->base = device_alloc_bytes(want, "allocate scratch buffer");
       arena_->capacity = want;
-    } else {
-      (void)hipGetLastError();
+    } catch (...) {
+      // The arena stays empty; the next call overflows again.
     }
   }
 }
@@ -96,8 +268,7 @@ void* Scratch::take_bytes(std::size_t bytes) {
   if (overflow_.empty() && used_ <= arena_->capacity) {
     return static_cast<char*>(arena_->base) + offset;
   }
-  void* p = nullptr;
-  check_hip(hipMalloc(&p, rounded), "allocate scratch buffer");
+  void* p = device_alloc_bytes(rounded, "allocate scratch buffer");
   overflow_.push_back(p);
   overflow_at_.push_back(offset);
   overflowed_ = true;
@@ -402,6 +573,31 @@ int wg_device_info(int ordinal, char* name, size_t name_len, char* gfx_target, s
     }
     if (total_memory_bytes) *total_memory_bytes = prop.totalGlobalMem;
     if (compute_units) *compute_units = prop.multiProcessorCount;
+  });
+}
+
+int wg_memory_cache_limit(size_t bytes, size_t* previous) {
+  return guard([&] {
+    const std::size_t prev = memory_cache_set_limit(bytes);
+    if (previous) *previous = prev;
+  });
+}
+
+int wg_memory_cache_release(int device) {
+  return guard([&] {
+    validate_device(device);
+    memory_cache_release(device);
+  });
+}
+
+int wg_memory_cache_info(int device, size_t* cached_bytes, size_t* cached_blocks,
+                         size_t* live_bytes) {
+  return guard([&] {
+    std::size_t c = 0, b = 0, l = 0;
+    memory_cache_info(device, &c, &b, &l);
+    if (cached_bytes) *cached_bytes = c;
+    if (cached_blocks) *cached_blocks = b;
+    if (live_bytes) *live_bytes = l;
   });
 }
 

@@ -865,6 +865,38 @@ class DeviceCommutationMatrix:
         return f"DeviceCommutationMatrix(shape={self.shape}, device={self._dev}, backend='hip')"
 
 
+def _copy_device_commutation_matrix_from_host_for_testing(matrix: DeviceCommutationMatrix,
+                                                          values: Any) -> None:
+    """Overwrite a device matrix with host flags (row-major, rows*cols entries;
+    nonzero = commutes): the test-only hook of reference
+    bindings/python/internal_bindings.cpp:721,980 over
+    src/hip/device_commutation_matrix.hip.cpp:372-391."""
+    flat = np.ascontiguousarray(np.asarray(values).reshape(-1) != 0, dtype=np.uint8)
+    _native.call("wg_comm_matrix_from_host_u8", matrix._handle(), _ptr(flat), flat.size, None)
+
+
+def memory_cache_info(device: int = 0) -> dict[str, int]:
+    """The library's device-memory cache on `device`: bytes and blocks kept for
+    reuse, and bytes handed out and not yet freed."""
+    cached, blocks, live = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    _native.call("wg_memory_cache_info", int(device), C.byref(cached), C.byref(blocks), C.byref(live))
+    return {"cached_bytes": cached.value, "cached_blocks": blocks.value, "live_bytes": live.value}
+
+
+def empty_cache(device: int = 0) -> None:
+    """Return every cached device block of `device` to the HIP runtime."""
+    _native.call("wg_memory_cache_release", int(device))
+
+
+def set_memory_cache_limit(limit_bytes: int | None) -> int | None:
+    """Per-device byte limit of the cache (None: the default, a quarter of the
+    device's memory). Returns the previous explicit limit, or None."""
+    new = (1 << 64) - 1 if limit_bytes is None else _checked_size(limit_bytes, "limit_bytes")
+    prev = C.c_size_t()
+    _native.call("wg_memory_cache_limit", new, C.byref(prev))
+    return None if prev.value == (1 << 64) - 1 else prev.value
+
+
 def hip_devices() -> list[dict[str, Any]]:
     lib = _native.load()
     count = C.c_int()
