@@ -81,3 +81,33 @@ def test_grouped_key_values_dense_closed_form(orc):
     for t, label in enumerate(labels):
         m = functools.reduce(np.kron, [mats[ch] for ch in label])
         assert abs(got[key[t]] - np.vdot(psi, m @ psi)) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_threaded_matmul_simplify_is_bitwise_serial_and_reference(orc, ref_lib, n):
+    """The threaded canonical sort and run reduction used for the 8,192^2
+    products (parts of at least 65,536 terms): bit-identical to the serial
+    oracle and to the compiled reference at 90,000 products, with duplicate
+    keys spread across the part boundaries."""
+    # Operands drawn from small pools of masks with distinct coefficients:
+    # 90,000 products over at most 40 x 40 distinct keys, so every run of
+    # equal keys is long and crosses part boundaries.
+    rng = np.random.default_rng(n)
+    pa = random_pauli_masks(n, 40, 0.3, 70 + n)
+    pb = random_pauli_masks(n, 40, 0.3, 71 + n)
+    ia, ib = rng.integers(0, 40, 300), rng.integers(0, 40, 300)
+    ca = rng.normal(size=300) + 1j * rng.normal(size=300)
+    cb = rng.normal(size=300) + 1j * rng.normal(size=300)
+    a = PauliSum.from_packed(n, pa.x_words()[ia], pa.z_words()[ia], ca)
+    b = PauliSum.from_packed(n, pb.x_words()[ib], pb.z_words()[ib], cb)
+    want = ref_lib.matmul(n, packed(a)[1:], packed(b)[1:], simplify=True)
+    for threads in (1, 4, 7):
+        orc.set_threads(threads)
+        try:
+            got = orc.matmul(n, packed(a)[1:], packed(b)[1:], simplify=True)
+        finally:
+            orc.set_threads(1)
+        assert got[0] == want[0]
+        for g, w in zip(got[1:], want[1:]):
+            np.testing.assert_array_equal(g.view(np.uint64), w.view(np.uint64))
+    assert want[3].shape[0] <= 1600
