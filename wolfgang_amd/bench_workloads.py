@@ -326,10 +326,10 @@ def adjoint(args) -> dict:
         _native.profile_reset()
         t_adj = _timed(fwd_adj, args.steps, args.warmup)
         calls = args.steps + args.warmup
-        # sigma: the tiled sweep (n >= 18, one launch per pass) or the fused
+        # sigma: the tiled sweep (n >= 18, one launch per pass) or the direct
         # sweep (smaller states).
         tiled_ms, tiled_launches = _native.profile_get("sigma_tiled")
-        sig_ms = (tiled_ms + _native.profile_get("adjoint_fused")[0]) / calls
+        sig_ms = (tiled_ms + _native.profile_get("sigma_direct")[0]) / calls
         _native.profile_enable(False)
         amp = 1 << n
         passes = tiled_launches // calls

@@ -1515,113 +1515,86 @@ __device__ inline double2 times_i_pow(double2 v, int k) {
   }
 }
 
-// Fused adjoint sweep: sigma = H psi, sigma_dag = H^dagger psi and, per key,
-// <P> = sum_o conj(psi[o]) (P psi)[o], in one pass. A block owns chunks of
-// kFuseOut * kFuseBlock consecutive outputs (grid-stride); for each x-group it
-// gathers psi[o ^ x] once and applies all the group's keys. Per-key partial
-// sums are reduced over the block every kFuseChunk keys and accumulated into
-// the block's own row of `partials` (row = block, one entry per key in group
-// order), so no two blocks write the same entry and the order is fixed.
-constexpr int kFuseBlock = 256;
-constexpr int kFuseOut = 8;
-constexpr int kFuseChunk = 32;
+// Direct sigma sweep (states swept without tiles): sigma = H psi and
+// sigma_dag = H^dagger psi, with
+//   sigma[o] = sum_g psi[o ^ x_g] sum_{k in g} c_k i^{y_k} (-1)^{|z_k & (o ^ x_g)|}
+// (sigma_dag: conj(c_k)). The work is split two ways so that small states with
+// many terms still fill the GPU: blockIdx.x picks a chunk of kSigOut *
+// kSigBlock consecutive outputs, blockIdx.y a slice of consecutive x-groups.
+// Each block writes its slice's partial sums to its own row (row = slice,
+// row_stride = len); sigma_rows_sum_kernel then adds the rows in slice order,
+// so the result does not depend on scheduling. With one slice the block writes
+// sigma directly.
+constexpr int kSigBlock = 256;
+constexpr int kSigOut = 8;
+constexpr u64 kSigChunk = static_cast<u64>(kSigBlock) * kSigOut;
 
-__global__ __launch_bounds__(kFuseBlock) void adjoint_fused_kernel(
+__global__ __launch_bounds__(kSigBlock) void sigma_direct_kernel(
     const double2* __restrict__ psi, u64 len, const u64* __restrict__ group_x,
-    const u32* __restrict__ group_begin, u32 num_groups, const u64* __restrict__ key_z,
-    const int* __restrict__ key_y, const double2* __restrict__ key_c, u32 num_keys,
-    double2* __restrict__ sigma, double2* __restrict__ sigma_dag, double2* __restrict__ partials) {
-  __shared__ double2 red[kFuseBlock / 64][kFuseChunk];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
-  const u64 nchunks = (len + chunk_out - 1) / chunk_out;
-  double2* row = partials + static_cast<std::size_t>(blockIdx.x) * num_keys;
-  bool first = true;
-  for (u64 ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    const u64 base = ch * chunk_out;
-    double2 pv[kFuseOut], sg[kFuseOut], sd[kFuseOut];
+    const u32* __restrict__ group_begin, u32 num_groups, u32 groups_per_slice,
+    const u64* __restrict__ key_z, const int* __restrict__ key_y, const double2* __restrict__ key_c,
+    double2* __restrict__ rows_s, double2* __restrict__ rows_d) {
+  const u64 base = static_cast<u64>(blockIdx.x) * kSigChunk;
+  const u32 g0 = blockIdx.y * groups_per_slice;
+  const u32 g1 = min(num_groups, g0 + groups_per_slice);
+  double2 sg[kSigOut], sd[kSigOut];
 #pragma unroll
-    for (int i = 0; i < kFuseOut; ++i) {
-      const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
-      pv[i] = o < len ? psi[o] : make_double2(0.0, 0.0);
-      sg[i] = make_double2(0.0, 0.0);
-      sd[i] = make_double2(0.0, 0.0);
-    }
-    for (u32 g = 0; g < num_groups; ++g) {
-      const u64 x = group_x[g];
-      double2 tv[kFuseOut];
-#pragma unroll
-      for (int i = 0; i < kFuseOut; ++i) {
-        const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
-        tv[i] = o < len ? psi[o ^ x] : make_double2(0.0, 0.0);
-      }
-      const u32 kb = group_begin[g], ke = group_begin[g + 1];
-      for (u32 k = kb; k < ke; ++k) {
-        const u64 z = key_z[k];
-        const int y = key_y[k];
-        const double2 c = key_c[k];
-        double pr = 0.0, pi = 0.0;
-#pragma unroll
-        for (int i = 0; i < kFuseOut; ++i) {
-          const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
-          double2 w = times_i_pow(tv[i], y);
-          if (__popcll(z & (o ^ x)) & 1) w = make_double2(-w.x, -w.y);
-          sg[i].x = fma(c.x, w.x, fma(-c.y, w.y, sg[i].x));
-          sg[i].y = fma(c.x, w.y, fma(c.y, w.x, sg[i].y));
-          sd[i].x = fma(c.x, w.x, fma(c.y, w.y, sd[i].x));
-          sd[i].y = fma(c.x, w.y, fma(-c.y, w.x, sd[i].y));
-          // conj(psi[o]) * w
-          pr = fma(pv[i].x, w.x, fma(pv[i].y, w.y, pr));
-          pi = fma(pv[i].x, w.y, fma(-pv[i].y, w.x, pi));
-        }
-        for (int d = 32; d > 0; d >>= 1) {
-          pr += __shfl_xor(pr, d, 64);
-          pi += __shfl_xor(pi, d, 64);
-        }
-        const u32 slot = k % kFuseChunk;
-        if (lane == 0) red[wave][slot] = make_double2(pr, pi);
-        if (slot == kFuseChunk - 1 || k + 1 == num_keys) {
-          __syncthreads();
-          const u32 k0 = k - slot;
-          if (threadIdx.x <= slot) {
-            double2 acc = first ? make_double2(0.0, 0.0) : row[k0 + threadIdx.x];
-            for (int w2 = 0; w2 < kFuseBlock / 64; ++w2) {
-              acc.x += red[w2][threadIdx.x].x;
-              acc.y += red[w2][threadIdx.x].y;
-            }
-            row[k0 + threadIdx.x] = acc;
-          }
-          __syncthreads();
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kFuseOut; ++i) {
-      const u64 o = base + threadIdx.x + static_cast<u64>(i) * kFuseBlock;
-      if (o < len) {
-        if (sigma) sigma[o] = sg[i];
-        if (sigma_dag) sigma_dag[o] = sd[i];
-      }
-    }
-    first = false;
+  for (int i = 0; i < kSigOut; ++i) {
+    sg[i] = make_double2(0.0, 0.0);
+    sd[i] = make_double2(0.0, 0.0);
   }
-  if (first) {
-    // A block with no chunk still owns a row: clear it.
-    for (u32 k = threadIdx.x; k < num_keys; k += kFuseBlock) row[k] = make_double2(0.0, 0.0);
+  for (u32 g = g0; g < g1; ++g) {
+    const u64 x = group_x[g];
+    double2 tv[kSigOut];
+#pragma unroll
+    for (int i = 0; i < kSigOut; ++i) {
+      const u64 o = base + threadIdx.x + static_cast<u64>(i) * kSigBlock;
+      tv[i] = o < len ? psi[o ^ x] : make_double2(0.0, 0.0);
+    }
+    const u32 kb = group_begin[g], ke = group_begin[g + 1];
+    for (u32 k = kb; k < ke; ++k) {
+      const u64 z = key_z[k];
+      // c i^y and conj(c) i^y, uniform per key.
+      const double2 c = key_c[k];
+      const double2 cy = times_i_pow(c, key_y[k]);
+      const double2 cdy = times_i_pow(make_double2(c.x, -c.y), key_y[k]);
+#pragma unroll
+      for (int i = 0; i < kSigOut; ++i) {
+        const u64 o = base + threadIdx.x + static_cast<u64>(i) * kSigBlock;
+        double2 w = tv[i];
+        if (__popcll(z & (o ^ x)) & 1) w = make_double2(-w.x, -w.y);
+        sg[i].x = fma(cy.x, w.x, fma(-cy.y, w.y, sg[i].x));
+        sg[i].y = fma(cy.x, w.y, fma(cy.y, w.x, sg[i].y));
+        sd[i].x = fma(cdy.x, w.x, fma(-cdy.y, w.y, sd[i].x));
+        sd[i].y = fma(cdy.x, w.y, fma(cdy.y, w.x, sd[i].y));
+      }
+    }
+  }
+  const std::size_t row = static_cast<std::size_t>(blockIdx.y) * len;
+#pragma unroll
+  for (int i = 0; i < kSigOut; ++i) {
+    const u64 o = base + threadIdx.x + static_cast<u64>(i) * kSigBlock;
+    if (o < len) {
+      if (rows_s) rows_s[row + o] = sg[i];
+      if (rows_d) rows_d[row + o] = sd[i];
+    }
   }
 }
 
-// Per-key values from the fused sweep's per-block rows, summed in block order;
-// flat (group-order) key k maps to distinct-key id flat_key[k].
-__global__ __launch_bounds__(256) void fused_reduce_kernel(const double2* __restrict__ partials,
-                                                           u32 rows, u32 num_keys,
-                                                           const u32* __restrict__ flat_key,
-                                                           double* __restrict__ key_vals) {
-  const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_keys) return;
-  double r = 0.0;
-  for (u32 b = 0; b < rows; ++b) r += partials[static_cast<std::size_t>(b) * num_keys + k].x;
-  key_vals[flat_key[k]] = r;
+// out[o] = sum over the rows of rows[r * len + o], rows in order.
+__global__ __launch_bounds__(256) void sigma_rows_sum_kernel(const double2* __restrict__ rows,
+                                                             u32 num_rows, u64 len,
+                                                             double2* __restrict__ out) {
+  for (u64 o = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; o < len;
+       o += static_cast<u64>(gridDim.x) * blockDim.x) {
+    double2 acc = rows[o];
+    for (u32 r = 1; r < num_rows; ++r) {
+      const double2 v = rows[static_cast<std::size_t>(r) * len + o];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    out[o] = acc;
+  }
 }
 
 // Per-key coefficient sums from per-term coefficients, each key's terms in term
@@ -2818,6 +2791,48 @@ void tiled_adjoint_enqueue(const ExpectationPlan& plan, const double2* op_coeffs
   }
 }
 
+namespace {
+
+// Slices of x-groups for the direct sigma sweep: enough blocks to fill the GPU
+// (about 2,048), each slice's partial rows bounded to 2^22 entries per output.
+void sigma_direct_enqueue(const ExpectationPlan& plan, const double2* psi, u64 len,
+                          const double2* key_c, double2* sigma, double2* sigma_dag,
+                          Scratch& scratch, hipStream_t stream) {
+  const u32 G = static_cast<u32>(plan.num_groups);
+  const u64 chunks = (len + kSigChunk - 1) / kSigChunk;
+  if (chunks > 0x7fffffffULL) fail_invalid("request exceeds HIP launch grid limits");
+  u64 slices = std::max<u64>(1, 2048 / chunks);
+  slices = std::min<u64>(slices, std::max<u64>(1, (u64{1} << 22) / len));
+  slices = std::min<u64>(slices, std::max<u32>(G, 1));
+  const u32 per = static_cast<u32>((G + slices - 1) / slices);
+  const u32 S = static_cast<u32>((G + per - 1) / std::max<u32>(per, 1));
+  double2* rs = sigma;
+  double2* rd = sigma_dag;
+  if (S > 1) {
+    rs = sigma ? scratch.take<double2>(checked_mul(S, len, "sigma partial rows")) : nullptr;
+    rd = sigma_dag ? scratch.take<double2>(checked_mul(S, len, "sigma partial rows")) : nullptr;
+  }
+  {
+    ProfileScope prof("sigma_direct", stream);
+    sigma_direct_kernel<<<dim3(static_cast<unsigned>(chunks), std::max<u32>(S, 1)), kSigBlock, 0,
+                          stream>>>(psi, len, plan.d_group_x, plan.d_group_begin, G,
+                                    std::max<u32>(per, 1), plan.d_key_z, plan.d_key_y, key_c, rs, rd);
+    check_launch("sigma_direct");
+  }
+  if (S > 1) {
+    if (sigma) {
+      sigma_rows_sum_kernel<<<stride_grid(len, 256), 256, 0, stream>>>(rs, S, len, sigma);
+      check_launch("sigma_rows_sum");
+    }
+    if (sigma_dag) {
+      sigma_rows_sum_kernel<<<stride_grid(len, 256), 256, 0, stream>>>(rd, S, len, sigma_dag);
+      check_launch("sigma_rows_sum");
+    }
+  }
+}
+
+}  // namespace
+
 void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const double* coeffs,
                      double* term_values, double* sigma, double* sigma_dagger, hipStream_t stream,
                      Scratch& scratch, double2* dout) {
@@ -2830,7 +2845,11 @@ void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const
                           reinterpret_cast<double2*>(term_values), reinterpret_cast<double2*>(sigma),
                           reinterpret_cast<double2*>(sigma_dagger), dout, buf, stream);
   } else if (sigma || sigma_dagger) {
-    // One fused sweep: sigma, sigma^dagger and the per-key values.
+    // E and the per-term values from the forward sweep; sigma and sigma^dagger
+    // from the direct sigma sweep over output chunks x x-group slices.
+    if (term_values || dout) {
+      run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
+    }
     const double2* key_c = plan->d_key_c;
     if (coeffs) {
       double2* kc = scratch.take<double2>(plan->num_keys);
@@ -2839,26 +2858,8 @@ void adjoint_enqueue(wg_pauli_sum* op, const double* psi, std::size_t len, const
       check_launch("key_coeff");
       key_c = kc;
     }
-    const u64 chunk_out = static_cast<u64>(kFuseBlock) * kFuseOut;
-    const u64 nchunks = (len + chunk_out - 1) / chunk_out;
-    const u32 rows = static_cast<u32>(std::min<u64>(nchunks, 1024));
-    const u32 K = static_cast<u32>(plan->num_keys);
-    double2* rowbuf = scratch.take<double2>(checked_mul(rows, K, "adjoint partials"));
-    double* key_vals = scratch.take<double>(K);
-    {
-      ProfileScope prof("adjoint_fused", stream);
-      adjoint_fused_kernel<<<rows, kFuseBlock, 0, stream>>>(
-          dpsi, len, plan->d_group_x, plan->d_group_begin, static_cast<u32>(plan->num_groups),
-          plan->d_key_z, plan->d_key_y, key_c, K, reinterpret_cast<double2*>(sigma),
-          reinterpret_cast<double2*>(sigma_dagger), rowbuf);
-      check_launch("adjoint_fused");
-    }
-    fused_reduce_kernel<<<grid_for(K, 256), 256, 0, stream>>>(rowbuf, rows, K, plan->d_flat_key,
-                                                              key_vals);
-    check_launch("fused_reduce");
-    expect_finalize_kernel<<<1, 1024, 0, stream>>>(c, plan->d_term_key, plan->T, key_vals,
-                                                   reinterpret_cast<double2*>(term_values), dout);
-    check_launch("expect_finalize");
+    sigma_direct_enqueue(*plan, dpsi, len, key_c, reinterpret_cast<double2*>(sigma),
+                         reinterpret_cast<double2*>(sigma_dagger), scratch, stream);
   } else {
     run_forward(c, *plan, dpsi, reinterpret_cast<double2*>(term_values), dout, scratch, stream);
   }
