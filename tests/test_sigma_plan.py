@@ -38,7 +38,8 @@ def one_x_many_z(n, keys, seed):
     return PauliSum.from_packed(n, x, z, c)
 
 
-@pytest.mark.parametrize("case", ["random18", "chemistry19", "diag_heavy18", "long_pass18", "random20"])
+@pytest.mark.parametrize("case", ["random18", "chemistry19", "diag_heavy18", "long_pass18", "random20",
+                                  "random_wide18"])
 def test_sigma_records_replay_matches_oracle(orc, case):
     if case == "random18":
         op = random_pauli_masks(18, 300, 0.3, 81)
@@ -51,6 +52,9 @@ def test_sigma_records_replay_matches_oracle(orc, case):
         op = PauliSum.from_packed(18, xs, r.z_words(), r.coeffs())
     elif case == "long_pass18":
         op = one_x_many_z(18, 300, 84)
+    elif case == "random_wide18":
+        # More than 1,024 distinct x-masks: the greedy coset planner.
+        op = random_pauli_masks(18, 2500, 0.3, 86)
     else:
         op = random_pauli_masks(20, 120, 0.25, 85)
     n = op.num_qubits

@@ -39,6 +39,7 @@
 #include <utility>
 #include <numeric>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "wg_expect.hpp"
@@ -1784,7 +1785,23 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
   std::vector<Span> spans;
   std::vector<std::vector<u64>> pass_xs;
   const bool structured = xs.size() <= 1024;
-  std::size_t cursor = 0;
+  // Unstructured sets: the uncovered x-groups keyed by their part above the l
+  // contiguous low bits (those bits are coordinates of every pass, so an
+  // x-group lies in U exactly when its high part lies in the span of the
+  // generators' high parts), and the distinct high parts in first-appearance
+  // order (the candidate generators).
+  std::unordered_map<u64, std::vector<u32>> by_high;
+  std::vector<u64> highs;
+  if (!structured) {
+    by_high.reserve(xs.size() * 2);
+    for (std::size_t i = 0; i < xs.size(); ++i) {
+      auto [it, fresh] = by_high.try_emplace(xs[i] >> l);
+      if (fresh) highs.push_back(xs[i] >> l);
+      it->second.push_back(static_cast<u32>(i));
+    }
+  }
+  std::size_t hcursor = 0;
+  std::vector<u64> span_elems;  // high parts of the current pass's generator span
   while (remaining > 0 || spans.empty()) {
     Span sp;
     sp.coord = lowmask;
@@ -1827,12 +1844,40 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
           if (best_is_gen) sp.add_gen(best_bits); else sp.add_coords(best_bits);
         }
       } else {
-        // Unstructured x-masks: one generator per x-group, in order.
-        while (sp.dim() < kmax && cursor < xs.size()) {
-          if (!covered[cursor] && !sp.contains(xs[cursor])) {
-            sp.add_gen(sp.reduce(xs[cursor], nullptr) & ~sp.coord);
+        // Unstructured x-masks: greedy generators. Each step takes, among the
+        // first kUnstructuredCandidates uncovered high parts outside the
+        // current span, the one whose coset of the span holds the most
+        // uncovered x-groups (2^d hash lookups per candidate), so coincidences
+        // x_a ^ x_b = x_c, plentiful in large random sets, share passes.
+        constexpr std::size_t kUnstructuredCandidates = 48;
+        span_elems.assign(1, 0);
+        std::unordered_set<u64> in_span{0};
+        while (sp.dim() < kmax) {
+          while (hcursor < highs.size() && by_high.find(highs[hcursor]) == by_high.end()) ++hcursor;
+          u64 best = 0;
+          std::size_t best_gain = 0;
+          std::size_t seen_c = 0;
+          for (std::size_t j = hcursor; j < highs.size() && seen_c < kUnstructuredCandidates; ++j) {
+            const u64 c = highs[j];
+            if (by_high.find(c) == by_high.end() || in_span.count(c)) continue;
+            ++seen_c;
+            std::size_t gain = 0;
+            for (u64 e : span_elems) {
+              const auto it = by_high.find(e ^ c);
+              if (it != by_high.end()) gain += it->second.size();
+            }
+            if (gain > best_gain) {
+              best_gain = gain;
+              best = c;
+            }
           }
-          ++cursor;
+          if (best_gain == 0) break;
+          const std::size_t d0 = span_elems.size();
+          for (std::size_t i = 0; i < d0; ++i) {
+            span_elems.push_back(span_elems[i] ^ best);
+            in_span.insert(span_elems[i] ^ best);
+          }
+          sp.add_gen(sp.reduce(best << l, nullptr) & ~sp.coord);
         }
       }
     }
@@ -1844,17 +1889,36 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
       for (u64 g : sp.gens) occupied |= 1ULL << Span::top(g);
       if (!((occupied >> q) & 1ULL)) sp.add_coords(1ULL << q);
     }
-    // Membership scan: all x-groups for structured sets; for unstructured
-    // sets only a window after the generators (the span rarely contains more),
-    // which keeps planning linear in the number of x-groups.
+    // Membership: a scan of all x-groups for structured sets; for
+    // unstructured sets, lookups of every high part of U (the generator span
+    // plus the padding coordinates above the low bits: 2^(kmax - l) keys).
     std::vector<u64> in_pass;
-    const std::size_t scan_end =
-        structured ? xs.size() : std::min<std::size_t>(xs.size(), cursor + 4096);
-    for (std::size_t j = 0; j < scan_end; ++j) {
-      if (!covered[j] && sp.contains(xs[j])) {
-        covered[j] = 1;
-        --remaining;
-        in_pass.push_back(xs[j]);
+    if (structured) {
+      for (std::size_t j = 0; j < xs.size(); ++j) {
+        if (!covered[j] && sp.contains(xs[j])) {
+          covered[j] = 1;
+          --remaining;
+          in_pass.push_back(xs[j]);
+        }
+      }
+    } else if (remaining > 0) {
+      std::vector<u64> keys = span_elems;
+      for (int q = l; q < 64; ++q) {
+        if (!((sp.coord >> q) & 1ULL)) continue;
+        const u64 bit = 1ULL << (q - l);
+        const std::size_t k0 = keys.size();
+        for (std::size_t i = 0; i < k0; ++i) keys.push_back(keys[i] ^ bit);
+      }
+      for (u64 key : keys) {
+        const auto it = by_high.find(key);
+        if (it == by_high.end()) continue;
+        for (u32 j : it->second) {
+          if (!sp.contains(xs[j])) fail_runtime("tiled pass planning lost an x-group");
+          covered[j] = 1;
+          --remaining;
+          in_pass.push_back(xs[j]);
+        }
+        by_high.erase(it);
       }
     }
     spans.push_back(sp);
