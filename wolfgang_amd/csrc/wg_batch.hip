@@ -33,9 +33,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "wg_expect.hpp"
@@ -563,6 +566,57 @@ float time_ms(hipStream_t stream, int warm, Fn&& fn) {
 
 }  // namespace
 
+namespace wg {
+namespace {
+
+// The instances' expectation plans are independent host work (planning, then
+// uploads): the ones not built yet are built on up to 16 host threads, each
+// with a stream of its own. Errors are rethrown on the calling thread.
+void build_plans_parallel(const wg_pauli_sum* const* ops, std::size_t count, int device) {
+  std::vector<wg_pauli_sum*> todo;
+  for (std::size_t i = 0; i < count; ++i) {
+    auto* op = const_cast<wg_pauli_sum*>(ops[i]);
+    std::lock_guard<std::mutex> lk(op->plan_mutex);
+    if (!op->plan && op->num_terms > 0) todo.push_back(op);
+  }
+  std::sort(todo.begin(), todo.end());
+  todo.erase(std::unique(todo.begin(), todo.end()), todo.end());
+  const std::size_t hw = std::max(1U, std::thread::hardware_concurrency());
+  const std::size_t nthreads = std::min<std::size_t>({todo.size(), hw, 16});
+  if (nthreads < 2) return;  // the caller's loop builds them
+  std::atomic<std::size_t> next{0};
+  std::vector<std::exception_ptr> errors(nthreads);
+  std::vector<std::thread> pool;
+  pool.reserve(nthreads);
+  for (std::size_t t = 0; t < nthreads; ++t) {
+    pool.emplace_back([&, t] {
+      try {
+        DeviceScope scope(device);
+        hipStream_t st = nullptr;
+        check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "create stream");
+        struct StreamGuard {
+          hipStream_t s;
+          ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard_stream{st};
+        for (;;) {
+          const std::size_t k = next.fetch_add(1);
+          if (k >= todo.size()) break;
+          (void)get_plan(todo[k], st);
+        }
+      } catch (...) {
+        errors[t] = std::current_exception();
+      }
+    });
+  }
+  for (std::thread& th : pool) th.join();
+  for (const std::exception_ptr& e : errors) {
+    if (e) std::rethrow_exception(e);
+  }
+}
+
+}  // namespace
+}  // namespace wg
+
 extern "C" {
 
 int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* const* psi,
@@ -593,6 +647,7 @@ int wg_batch_create(size_t count, const wg_pauli_sum* const* ops, const double* 
     std::unique_ptr<wg_batch> b(new wg_batch());
     b->device = device;
     b->d_energies = reinterpret_cast<double2*>(energies);
+    build_plans_parallel(ops, count, device);
     std::size_t tiled_bytes = 0;
     for (std::size_t i = 0; i < count; ++i) {
       auto* op = const_cast<wg_pauli_sum*>(ops[i]);
