@@ -463,6 +463,14 @@ __global__ __launch_bounds__(256) void mask_records_kernel(const u64* __restrict
 
 constexpr std::size_t record_words(std::size_t W) { return W == 1 ? 2 : W == 2 ? 4 : 8; }
 
+// Records written by the prologue (2 <= W <= 4): 64 bytes. For W <= 3 the
+// term's coefficient rides in words 2W and 2W + 1, so the final gather reads
+// masks and coefficient from one record.
+constexpr bool record_has_coeff(std::size_t W) { return W >= 2 && W <= 3; }
+constexpr std::size_t prologue_record_words(std::size_t W) {
+  return record_has_coeff(W) ? 8 : record_words(W);
+}
+
 // One pass over the terms of a multi-word operator (2 <= W <= 4, whose keys
 // never pack into one word): OR / AND of every key word ([x0, z0, x1, z1,
 // ...], combined with atomics), the block's max |c|, the word-0 sort records
@@ -475,7 +483,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(
     double* __restrict__ block_max, u64* __restrict__ keys, u32* __restrict__ idx,
     u64* __restrict__ rec) {
   static_assert(W >= 2 && W <= 4, "prologue for multi-word keys");
-  constexpr int RW = static_cast<int>(record_words(W));
+  constexpr int RW = static_cast<int>(prologue_record_words(W));
   __shared__ double lds[4];
   u64 o[2 * W], a[2 * W];
 #pragma unroll
@@ -503,6 +511,10 @@ __global__ __launch_bounds__(256) void prologue_kernel(
     }
     const double2 ct = c[t];
     m = fmax(m, hypot(ct.x, ct.y));
+    if constexpr (record_has_coeff(W)) {
+      r[2 * W] = static_cast<u64>(__double_as_longlong(ct.x));
+      r[2 * W + 1] = static_cast<u64>(__double_as_longlong(ct.y));
+    }
     keys[t] = r[0];
     idx[t] = static_cast<u32>(t);
 #pragma unroll
@@ -527,6 +539,85 @@ __global__ __launch_bounds__(256) void prologue_kernel(
   __syncthreads();
   if (threadIdx.x == 0) {
     block_max[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+  }
+}
+
+// Segment sums and output in one gather (records with the coefficient, W =
+// 2, 3): segment s (sorted order) takes its head's record (masks and
+// coefficient), adds the other members' coefficients in sorted (=
+// construction) order, the reference's +0.0 seed when it applies, and writes
+// masks and sum at output position s; keep[s] flags |sum| > threshold and the
+// survivors are counted (one atomic per block). When every segment survives
+// the output is final; otherwise the survivors are compacted afterwards.
+template <int W>
+__global__ __launch_bounds__(256) void segment_emit_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx,
+    const u64* __restrict__ rec, int zero_seed, double threshold, u64* __restrict__ ox,
+    u64* __restrict__ oz, double2* __restrict__ oc, u32* __restrict__ keep,
+    u32* __restrict__ survivors) {
+  static_assert(record_has_coeff(W), "records carry the coefficient");
+  constexpr int RW = static_cast<int>(prologue_record_words(W));
+  __shared__ u32 lds[4];
+  u32 kept = 0;
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const u32 b = seg_start[s], e = seg_start[s + 1];
+    const u64* head = rec + static_cast<std::size_t>(idx[b]) * RW;
+    u64 r[2 * W + 2];
+#pragma unroll
+    for (int k = 0; k < 2 * W + 2; k += 2) {
+      const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(head + k);
+      r[k] = v.x;
+      r[k + 1] = v.y;
+    }
+    double2 acc = make_double2(__longlong_as_double(static_cast<long long>(r[2 * W])),
+                               __longlong_as_double(static_cast<long long>(r[2 * W + 1])));
+    for (u32 j = b + 1; j < e; ++j) {
+      const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(
+          rec + static_cast<std::size_t>(idx[j]) * RW + 2 * W);
+      acc.x += __longlong_as_double(static_cast<long long>(v.x));
+      acc.y += __longlong_as_double(static_cast<long long>(v.y));
+    }
+    if (zero_seed) {
+      acc.x += 0.0;
+      acc.y += 0.0;
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ox[s * W + w] = r[w];
+      oz[s * W + w] = r[W + w];
+    }
+    oc[s] = acc;
+    const u32 k = hypot(acc.x, acc.y) > threshold ? 1U : 0U;
+    keep[s] = k;
+    kept += k;
+  }
+  for (int d = 32; d > 0; d >>= 1) kept += static_cast<u32>(__shfl_xor(static_cast<int>(kept), d, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = kept;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32 total = lds[0] + lds[1] + lds[2] + lds[3];
+    if (total) atomicAdd(survivors, total);
+  }
+}
+
+__global__ void clear_count_kernel(u32* count) { *count = 0; }
+
+// Survivors of segment_emit_kernel's output (every segment at its position)
+// moved to their compacted positions.
+__global__ __launch_bounds__(256) void compact_emitted_kernel(
+    const u32* __restrict__ keep, const u32* __restrict__ out_pos, std::size_t S, std::size_t W,
+    const u64* __restrict__ sx, const u64* __restrict__ sz, const double2* __restrict__ sc,
+    u64* __restrict__ ox, u64* __restrict__ oz, double2* __restrict__ oc) {
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (!keep[s]) continue;
+    const std::size_t o = out_pos[s];
+    for (std::size_t w = 0; w < W; ++w) {
+      ox[o * W + w] = sx[s * W + w];
+      oz[o * W + w] = sz[s * W + w];
+    }
+    oc[o] = sc[s];
   }
 }
 
@@ -700,7 +791,8 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   u32* idx_a = scratch.take<u32>(T);
   u64* keys_b = scratch.take<u64>(T);
   u32* idx_b = scratch.take<u32>(T);
-  u64* rec = fused ? scratch.take<u64>(checked_mul(T, record_words(W), "mask records")) : nullptr;
+  u64* rec = fused ? scratch.take<u64>(checked_mul(T, prologue_record_words(W), "mask records"))
+                   : nullptr;
   const unsigned mgrid = std::min<unsigned>(stride_grid(T), 1024);
   double* dmax = scratch.take<double>(mgrid);
   std::vector<unsigned long long> ors(2 * W, 0ULL), ands(2 * W, ~0ULL);
@@ -836,9 +928,37 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
             "copy device buffer to host");
   sync(stream);
 
-  double2* sums = scratch.take<double2>(S);
   u32* keep = scratch.take<u32>(S);
   u32* out_pos = scratch.take<u32>(S);
+  if (fused && record_has_coeff(W)) {
+    // Sums and output in one gather of the 64-byte records, written at each
+    // segment's position; compacted only when some segment is dropped.
+    std::unique_ptr<wg_pauli_sum> staged(new_pauli_sum(n, S, device));
+    clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+    check_launch("clear_count");
+    auto emit = [&](auto kernel) {
+      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, rec, hash_seed ? 0 : 1,
+                                                 threshold, staged->x, staged->z, staged->coeffs,
+                                                 keep, d_count + 1);
+    };
+    if (W == 2) emit(segment_emit_kernel<2>); else emit(segment_emit_kernel<3>);
+    check_launch("segment_emit");
+    u32 U = 0;
+    check_hip(hipMemcpyAsync(&U, d_count + 1, sizeof(u32), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+    if (U == S) return staged.release();
+    exclusive_scan_u32(keep, out_pos, S, nullptr, scratch, stream);
+    std::unique_ptr<wg_pauli_sum> out(new_pauli_sum(n, U, device));
+    if (U) {
+      compact_emitted_kernel<<<stride_grid(S), 256, 0, stream>>>(
+          keep, out_pos, S, W, staged->x, staged->z, staged->coeffs, out->x, out->z, out->coeffs);
+      check_launch("compact_emitted");
+    }
+    sync(stream);
+    return out.release();
+  }
+  double2* sums = scratch.take<double2>(S);
   segment_sum_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, c,
                                                          hash_seed ? 0 : 1, threshold, sums, keep);
   check_launch("segment_sum");
@@ -865,12 +985,8 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
       kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, idx_a, rec, sums,
                                                  out->x, out->z, out->coeffs);
     };
-    switch (W) {
-      case 1: gather(compact_records_kernel<1>); break;
-      case 2: gather(compact_records_kernel<2>); break;
-      case 3: gather(compact_records_kernel<3>); break;
-      default: gather(compact_records_kernel<4>); break;
-    }
+    // (Two and three words took segment_emit_kernel above.)
+    if (W == 1) gather(compact_records_kernel<1>); else gather(compact_records_kernel<4>);
     check_launch("compact");
   } else if (U) {
     compact_gather_kernel<<<stride_grid(S), 256, 0, stream>>>(
