@@ -19,6 +19,7 @@
 #include <string>
 
 #include "wg_internal.hpp"
+#include "wg_product_term.hpp"
 #include "wg_simplify.hpp"
 
 namespace wg {
@@ -27,15 +28,6 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPerThread = 4;
 constexpr std::size_t kChunk = static_cast<std::size_t>(kBlock) * kPerThread;
-
-__device__ inline double2 apply_phase(double2 v, int k) {
-  switch (k & 3) {
-    case 0: return v;
-    case 1: return make_double2(-v.y, v.x);
-    case 2: return make_double2(-v.x, -v.y);
-    default: return make_double2(v.y, -v.x);
-  }
-}
 
 template <int WC>
 __global__ __launch_bounds__(kBlock) void product_kernel(
@@ -64,8 +56,7 @@ __global__ __launch_bounds__(kBlock) void product_kernel(
       k += __popcll(oxw & ozw) - __popcll(rxw & rzw) + 2 * __popcll(lxw & rzw);
     }
     const double2 b = rc[j];
-    const double2 prod = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-    oc[p] = apply_phase(prod, k);
+    oc[p] = apply_phase(complex_product(a, b), k);
   }
 }
 
@@ -138,16 +129,22 @@ int wg_matmul(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, int simplify_out
       return;
     }
     Scratch scratch(lhs->device);
-    u64* px = scratch.take<u64>(checked_mul(P, W, "matmul products"));
-    u64* pz = scratch.take<u64>(P * W);
-    double2* pc = scratch.take<double2>(P);
-    launch_products(lhs, rhs, px, pz, pc, stream);
     // Reference src/multiply.cpp:107-261: with W in {1, 2} and a
     // duplicate-heavy operand the reference accumulates in a hash map seeded by
     // the first product, with a fixed 1e-12 drop; otherwise it calls
     // simplify() with its defaults on the materialized products.
     const bool hash = (W == 1 || W == 2) &&
                       (operand_duplicate_heavy(lhs, stream) || operand_duplicate_heavy(rhs, stream));
+    if (W >= 1 && W <= 4) {
+      // Up to 256 qubits the products are never materialized: the simplify
+      // reads them through the operands.
+      *out = simplify_products(lhs, rhs, 1.0e-12, 0.0, hash ? 1 : -1, scratch, stream);
+      return;
+    }
+    u64* px = scratch.take<u64>(checked_mul(P, W, "matmul products"));
+    u64* pz = scratch.take<u64>(P * W);
+    double2* pc = scratch.take<double2>(P);
+    launch_products(lhs, rhs, px, pz, pc, stream);
     *out = simplify_device(px, pz, pc, lhs->num_qubits, P, lhs->device, 1.0e-12, 0.0,
                            hash ? 1 : -1, scratch, stream);
   });

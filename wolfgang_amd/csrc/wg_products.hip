@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "wg_internal.hpp"
+#include "wg_product_term.hpp"
 #include "wg_simplify.hpp"
 
 namespace wg {
@@ -47,15 +48,6 @@ struct ProductPair {
   u64 unit_begin;  // first unit of this pair in the chunk
   u64 chunks;      // rhs blocks per lhs term
 };
-
-__device__ inline double2 apply_phase(double2 v, int k) {
-  switch (k & 3) {
-    case 0: return v;
-    case 1: return make_double2(-v.y, v.x);
-    case 2: return make_double2(-v.x, -v.y);
-    default: return make_double2(v.y, -v.x);
-  }
-}
 
 __global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0; }
 
@@ -100,8 +92,7 @@ __global__ __launch_bounds__(kBlock) void product_batch_kernel(
         k += __popcll(oxw & ozw) - __popcll(rxw & rzw) + 2 * __popcll(lxw & rzw);
       }
       const double2 b = p.rc[j];
-      const double2 prod = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-      p.oc[o] = apply_phase(prod, k);
+      p.oc[o] = apply_phase(complex_product(a, b), k);
     }
   }
 }

@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "wg_internal.hpp"
+#include "wg_product_term.hpp"
 #include "wg_simplify.hpp"
 
 namespace wg {
@@ -170,8 +171,10 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   constexpr int kWaves = kBlock / 64;
   constexpr int kRows = kTile / kBlock;
   static_assert(kBlock == 256, "one thread per digit");
-  __shared__ u32 counts[kWaves][256];  // running counts per wave, then the waves' offsets
-  __shared__ u32 tstart[256];          // digit's first position in the sorted tile
+  // (16-bit counts and tile positions keep the block under a third of the LDS,
+  // three blocks per CU.)
+  __shared__ unsigned short counts[kWaves][256];  // running counts per wave, then the waves' offsets
+  __shared__ unsigned short tstart[256];          // digit's first position in the sorted tile
   __shared__ u32 gbase[256];           // digit's first output position for this tile
   __shared__ u32 wsum[kWaves];
   __shared__ u64 skeys[kTile * KW];
@@ -217,7 +220,9 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     dig[r] = d;
     const u32 before = valid ? counts[wave][d] : 0U;
     rank[r] = before + static_cast<u32>(__popcll(peers & lt));
-    if (valid && (peers & lt) == 0) counts[wave][d] = before + static_cast<u32>(__popcll(peers));
+    if (valid && (peers & lt) == 0) {
+      counts[wave][d] = static_cast<unsigned short>(before + static_cast<u32>(__popcll(peers)));
+    }
   }
   __syncthreads();
   // Thread tid handles digit tid: the waves' offsets, then the tile's
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) {
     const u32 c = counts[w][tid];
-    counts[w][tid] = total;
+    counts[w][tid] = static_cast<unsigned short>(total);
     total += c;
   }
   u32 incl = total;
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   __syncthreads();
   u32 before_wave = 0;
   for (u32 w = 0; w < wave; ++w) before_wave += wsum[w];
-  tstart[tid] = before_wave + incl - total;
+  tstart[tid] = static_cast<unsigned short>(before_wave + incl - total);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
@@ -334,11 +339,18 @@ __global__ __launch_bounds__(256) void compact_kernel(
 // instead of the whole key, and for distinct-ish keys (random products) the
 // later words touch few items.
 
-__device__ inline u64 key_word(const u64* __restrict__ x, const u64* __restrict__ z,
-                               std::size_t W, u32 t, int w) {
-  const std::size_t at = static_cast<std::size_t>(t) * W + static_cast<std::size_t>(w >> 1);
-  return (w & 1) ? z[at] : x[at];
-}
+// Materialized terms: key word w of term t in the canonical order
+// [x0, z0, x1, z1, ...]. (ProductTerms reads the same words through the
+// operands of a product.)
+struct ArrayTerms {
+  const u64* x;
+  const u64* z;
+  std::size_t W;
+  __device__ u64 word(u32 t, int w) const {
+    const std::size_t at = static_cast<std::size_t>(t) * W + static_cast<std::size_t>(w >> 1);
+    return (w & 1) ? z[at] : x[at];
+  }
+};
 
 __global__ __launch_bounds__(256) void word_records_kernel(const u64* __restrict__ x,
                                                            const u64* __restrict__ z,
@@ -378,12 +390,12 @@ __global__ __launch_bounds__(256) void tied_kernel(const u32* __restrict__ head,
 // Refinement records for the tied items of the current list (list == null:
 // the identity over all positions): global position, run rank (monotone in
 // position) and word w of the item's term.
+template <class Src>
 __global__ __launch_bounds__(256) void refine_records_kernel(
     const u32* __restrict__ list, std::size_t n, const u32* __restrict__ tied,
     const u32* __restrict__ tpos, const u32* __restrict__ head, const u32* __restrict__ hscan,
-    const u32* __restrict__ idx, const u64* __restrict__ x, const u64* __restrict__ z,
-    std::size_t W, int w, u32* __restrict__ plist, u64* __restrict__ keys2,
-    u32* __restrict__ idx2) {
+    const u32* __restrict__ idx, Src src, int w, u32* __restrict__ plist,
+    u64* __restrict__ keys2, u32* __restrict__ idx2) {
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     if (!tied[i]) continue;
@@ -392,7 +404,7 @@ __global__ __launch_bounds__(256) void refine_records_kernel(
     const u32 t = idx[pos];
     plist[j] = pos;
     keys2[2 * static_cast<std::size_t>(j)] = hscan[i] + head[i] - 1U;
-    keys2[2 * static_cast<std::size_t>(j) + 1] = key_word(x, z, W, t, w);
+    keys2[2 * static_cast<std::size_t>(j) + 1] = src.word(t, w);
     idx2[j] = t;
   }
 }
@@ -408,18 +420,17 @@ __global__ __launch_bounds__(256) void refine_scatter_kernel(const u32* __restri
 
 // Segment heads of the fully sorted order: word 0 from the sorted word-0 keys,
 // the other words compared through the terms only where word 0 ties.
+template <class Src>
 __global__ __launch_bounds__(256) void full_heads_kernel(const u64* __restrict__ k0,
-                                                         const u32* __restrict__ idx,
-                                                         const u64* __restrict__ x,
-                                                         const u64* __restrict__ z, std::size_t W,
-                                                         std::size_t T, u32* __restrict__ flags) {
+                                                         const u32* __restrict__ idx, Src src,
+                                                         int KW, std::size_t T,
+                                                         u32* __restrict__ flags) {
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < T;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     bool h = i == 0 || k0[i] != k0[i - 1];
     if (!h) {
-      const std::size_t a = static_cast<std::size_t>(idx[i]) * W;
-      const std::size_t b = static_cast<std::size_t>(idx[i - 1]) * W;
-      for (std::size_t w = 0; w < W && !h; ++w) h = x[a + w] != x[b + w] || z[a + w] != z[b + w];
+      const u32 a = idx[i], b = idx[i - 1];
+      for (int w = 1; w < KW && !h; ++w) h = src.word(a, w) != src.word(b, w);
     }
     flags[i] = h ? 1U : 0U;
   }
@@ -649,6 +660,189 @@ __global__ __launch_bounds__(256) void compact_records_kernel(
   }
 }
 
+// ---------------------------------------------------------------- products
+//
+// Simplify of lhs @ rhs without materializing the L*R products. One pass
+// computes every product's sort key (word 0; x << 32 | z when the masks fit
+// 32 bits), the key statistics and max |c|, and writes only the (key, index)
+// records; the sort runs on those. The tie refinement, the segment heads and
+// the segment sums read masks and coefficients through the operands
+// (ProductTerms), which stay in L2, instead of gathering materialized products
+// from HBM in sorted (random) order. The coefficient arithmetic is the product
+// kernel's, so the sums are bit-identical to simplifying materialized products.
+
+constexpr u32 kProdPerThread = 4;
+constexpr u32 kProdChunk = 256 * kProdPerThread;
+
+// Operand records (product_record_words(W) words per term: x words, z words,
+// coefficient bits, zero padding).
+__global__ __launch_bounds__(256) void operand_records_kernel(const u64* __restrict__ x,
+                                                              const u64* __restrict__ z,
+                                                              const double2* __restrict__ c,
+                                                              std::size_t T, u32 W,
+                                                              u64* __restrict__ rec) {
+  const std::size_t RW = product_record_words(W);
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < T;
+       t += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    u64* r = rec + t * RW;
+    for (std::size_t w = 0; w < W; ++w) {
+      r[w] = x[t * W + w];
+      r[W + w] = z[t * W + w];
+    }
+    r[2 * W] = static_cast<u64>(__double_as_longlong(c[t].x));
+    r[2 * W + 1] = static_cast<u64>(__double_as_longlong(c[t].y));
+    for (std::size_t w = 2 * W + 2; w < RW; ++w) r[w] = 0;
+  }
+}
+
+template <int W, bool kPacked, bool kMax>
+__global__ __launch_bounds__(256) void product_keys_kernel(ProductTerms P, u32 chunks,
+                                                           u64* __restrict__ block_stats,
+                                                           u64* __restrict__ keys,
+                                                           u32* __restrict__ idx) {
+  static_assert(W >= 1 && W <= 4 && (!kPacked || W == 1), "product key shapes");
+  constexpr int NS = 4 * W + 1;  // OR of the 2W key words, AND of them, max |c| (bits)
+  __shared__ u64 lds[4][NS];
+  // One block per (lhs term, 1,024 rhs terms), as the product kernel.
+  const u32 i = blockIdx.x / chunks;
+  const u32 j0 = (blockIdx.x - i * chunks) * kProdChunk;
+  u64 o[2 * W], a[2 * W];
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    o[k] = 0;
+    a[k] = ~0ULL;
+  }
+  double m = 0.0;
+#pragma unroll
+  for (u32 q = 0; q < kProdPerThread; ++q) {
+    const u32 j = j0 + q * 256U + threadIdx.x;
+    if (j >= P.R) break;
+    u64 ox[W], oz[W];
+    const double2 c = P.term_ij<W>(i, j, ox, oz);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      o[2 * w] |= ox[w];
+      a[2 * w] &= ox[w];
+      o[2 * w + 1] |= oz[w];
+      a[2 * w + 1] &= oz[w];
+    }
+    if constexpr (kMax) m = fmax(m, hypot(c.x, c.y));
+    const std::size_t t = static_cast<std::size_t>(i) * P.R + j;
+    keys[t] = kPacked ? ((ox[0] << 32) | oz[0]) : ox[0];
+    idx[t] = static_cast<u32>(t);
+  }
+  // Block partials: waves by shuffles, then the 4 waves through LDS.
+  const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    u64 ok = o[k], ak = a[k];
+    for (int d = 32; d > 0; d >>= 1) {
+      ok |= __shfl_xor(ok, d, 64);
+      ak &= __shfl_xor(ak, d, 64);
+    }
+    if (lane == 0) {
+      lds[wave][k] = ok;
+      lds[wave][2 * W + k] = ak;
+    }
+  }
+  if constexpr (kMax) {
+    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+  }
+  // |c| >= 0, so the bit patterns order like the values.
+  if (lane == 0) lds[wave][4 * W] = static_cast<u64>(__double_as_longlong(m));
+  __syncthreads();
+  if (threadIdx.x < static_cast<u32>(NS)) {
+    const u32 k = threadIdx.x;
+    u64 v = lds[0][k];
+    for (int w = 1; w < 4; ++w) {
+      const u64 u = lds[w][k];
+      v = k < 2 * W ? (v | u) : k < 4 * W ? (v & u) : (u > v ? u : v);
+    }
+    block_stats[static_cast<std::size_t>(k) * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// Combines the per-block statistics of product_keys_kernel (word k of block b
+// at k * blocks + b): block k of this kernel reduces word k (OR for k < n_or,
+// AND for k < 2 n_or, max after that). Same-address atomics from every block
+// would serialize.
+__global__ __launch_bounds__(1024) void stats_reduce_kernel(const u64* __restrict__ block_stats,
+                                                            u32 blocks, u32 n_or,
+                                                            u64* __restrict__ out) {
+  __shared__ u64 part[16];
+  const u32 k = blockIdx.x;
+  const int op = k < n_or ? 0 : k < 2 * n_or ? 1 : 2;
+  auto combine = [op](u64 v, u64 u) { return op == 0 ? (v | u) : op == 1 ? (v & u) : (u > v ? u : v); };
+  u64 v = op == 1 ? ~0ULL : 0ULL;
+  const u64* row = block_stats + static_cast<std::size_t>(k) * blocks;
+  for (u32 b = threadIdx.x; b < blocks; b += blockDim.x) v = combine(v, row[b]);
+  for (int d = 32; d > 0; d >>= 1) v = combine(v, __shfl_xor(v, d, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) v = combine(v, part[w]);
+    out[k] = v;
+  }
+}
+
+// Masks of the first `count` products (the duplicate-rate sample of the
+// simplify seed rule).
+__global__ __launch_bounds__(256) void product_sample_kernel(ProductTerms P, u32 count,
+                                                             u64* __restrict__ x,
+                                                             u64* __restrict__ z) {
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  for (u32 w = 0; w < P.W; ++w) {
+    x[static_cast<std::size_t>(t) * P.W + w] = P.word(t, static_cast<int>(2 * w));
+    z[static_cast<std::size_t>(t) * P.W + w] = P.word(t, static_cast<int>(2 * w + 1));
+  }
+}
+
+// Segment sums and output of the products: segment s (sorted order) takes its
+// head's masks, adds its members' coefficients in sorted (= construction)
+// order with the reference's +0.0 seed when it applies, and writes masks and
+// sum at output position s; keep[s] flags |sum| > threshold and the survivors
+// are counted (one atomic per block).
+template <int W>
+__global__ __launch_bounds__(256) void product_emit_kernel(
+    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx, ProductTerms P,
+    int zero_seed, double threshold, u64* __restrict__ ox, u64* __restrict__ oz,
+    double2* __restrict__ oc, u32* __restrict__ keep, u32* __restrict__ survivors) {
+  __shared__ u32 lds[4];
+  u32 kept = 0;
+  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
+       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const u32 b = seg_start[s], e = seg_start[s + 1];
+    u64 mx[W], mz[W];
+    double2 acc = P.term<W>(idx[b], mx, mz);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      ox[s * W + w] = mx[w];
+      oz[s * W + w] = mz[w];
+    }
+    for (u32 q = b + 1; q < e; ++q) {
+      const double2 v = P.coeff<W>(idx[q]);
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    if (zero_seed) {
+      acc.x += 0.0;
+      acc.y += 0.0;
+    }
+    oc[s] = acc;
+    const u32 k = hypot(acc.x, acc.y) > threshold ? 1U : 0U;
+    keep[s] = k;
+    kept += k;
+  }
+  for (int d = 32; d > 0; d >>= 1) kept += static_cast<u32>(__shfl_xor(static_cast<int>(kept), d, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = kept;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32 total = lds[0] + lds[1] + lds[2] + lds[3];
+    if (total) atomicAdd(survivors, total);
+  }
+}
+
 unsigned stride_grid(std::size_t work) {
   const std::size_t g = (work + 255) / 256;
   return static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g, 65536)));
@@ -747,7 +941,8 @@ std::vector<DigitPass> digit_passes(const std::vector<u64>& varying) {
 // Stable LSD radix sort of (KW-word key, index) records; the sorted records
 // end up in (keys_a, idx_a).
 void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_t n, int KW,
-                const std::vector<DigitPass>& passes, Scratch& scratch, hipStream_t stream) {
+                const std::vector<DigitPass>& passes, Scratch& scratch, hipStream_t stream,
+                const char* prof_name = "radix_scatter") {
   if (passes.empty() || n < 2) return;
   const std::size_t nblocks = (n + kTile - 1) / kTile;
   u32* hist = scratch.take<u32>(256 * nblocks);
@@ -757,7 +952,7 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
                                                                    p.shift, nblocks, hist);
     check_launch("radix_hist");
     exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
-    ProfileScope prof("radix_scatter", stream);
+    ProfileScope prof(prof_name, stream);
     if (KW == 1) {
       radix_scatter_kernel<1><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
           keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs);
@@ -771,6 +966,67 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
     std::swap(keys_a, keys_b);
     std::swap(idx_a, idx_b);
   }
+}
+
+// Segment heads of the full key order (flags[i] = 1 where a run of equal keys
+// starts), given the items sorted stably by key word 0 (k0, idx): the items
+// whose word-0 values tie are re-sorted by (run rank, next word), word by word,
+// and written back into their positions, so equal keys end up adjacent in
+// construction order. src reads the KW key words of a term; ors / ands hold
+// each word's bitwise OR / AND over all terms (their XOR: the varying bits).
+template <class Src>
+void word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
+                      const std::vector<unsigned long long>& ors,
+                      const std::vector<unsigned long long>& ands, u32* flags, Scratch& scratch,
+                      hipStream_t stream) {
+  u32* head = flags;
+  run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, T, 1, head);
+  check_launch("run_heads");
+  const u32* list = nullptr;  // current list of positions (null: all of [0, T))
+  std::size_t n_cur = T;
+  u32* tied = scratch.take<u32>(T);
+  u32* tpos = scratch.take<u32>(T);
+  u32* hscan = scratch.take<u32>(T);
+  u32* d_tied = scratch.take<u32>(1);
+  for (std::size_t w = 1; w < KW; ++w) {
+    tied_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(head, n_cur, tied);
+    check_launch("tied");
+    exclusive_scan_u32(tied, tpos, n_cur, d_tied, scratch, stream);
+    u32 A = 0;
+    check_hip(hipMemcpyAsync(&A, d_tied, sizeof(u32), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+    if (A == 0) break;
+    exclusive_scan_u32(head, hscan, n_cur, nullptr, scratch, stream);
+    u32* plist = scratch.take<u32>(A);
+    u64* r_keys_a = scratch.take<u64>(2 * static_cast<std::size_t>(A));
+    u64* r_keys_b = scratch.take<u64>(2 * static_cast<std::size_t>(A));
+    u32* r_idx_a = scratch.take<u32>(A);
+    u32* r_idx_b = scratch.take<u32>(A);
+    refine_records_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(
+        list, n_cur, tied, tpos, head, hscan, idx, src, static_cast<int>(w), plist, r_keys_a,
+        r_idx_a);
+    check_launch("refine_records");
+    // Records [run, word w]: word w's varying bits, then the run ranks
+    // (all bits below the bit length of n_cur).
+    int run_bits = 0;
+    while (run_bits < 64 && (u64{1} << run_bits) < n_cur) ++run_bits;
+    const u64 run_varying = run_bits >= 64 ? ~0ULL : ((u64{1} << run_bits) - 1);
+    radix_sort(r_keys_a, r_idx_a, r_keys_b, r_idx_b, A, 2,
+               digit_passes({run_varying, ors[w] ^ ands[w]}), scratch, stream, "radix_refine");
+    refine_scatter_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, r_idx_a, A, idx);
+    check_launch("refine_scatter");
+    // The next level works on the tied items, in their new order.
+    u32* head2 = scratch.take<u32>(A);
+    run_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(r_keys_a, A, 2, head2);
+    check_launch("run_heads");
+    head = head2;
+    list = plist;
+    n_cur = A;
+  }
+  full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, src, static_cast<int>(KW), T,
+                                                        flags);
+  check_launch("full_heads");
 }
 
 }  // namespace
@@ -865,55 +1121,8 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     }
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
-    u64* k0 = keys_a;
-    u32* idx = idx_a;
-    u32* head = flags;
-    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, T, 1, head);
-    check_launch("run_heads");
-    const u32* list = nullptr;  // current list of positions (null: all of [0, T))
-    std::size_t n_cur = T;
-    u32* tied = scratch.take<u32>(T);
-    u32* tpos = scratch.take<u32>(T);
-    u32* hscan = scratch.take<u32>(T);
-    u32* d_tied = scratch.take<u32>(1);
-    for (std::size_t w = 1; w < 2 * W; ++w) {
-      tied_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(head, n_cur, tied);
-      check_launch("tied");
-      exclusive_scan_u32(tied, tpos, n_cur, d_tied, scratch, stream);
-      u32 A = 0;
-      check_hip(hipMemcpyAsync(&A, d_tied, sizeof(u32), hipMemcpyDeviceToHost, stream),
-                "copy device buffer to host");
-      sync(stream);
-      if (A == 0) break;
-      exclusive_scan_u32(head, hscan, n_cur, nullptr, scratch, stream);
-      u32* plist = scratch.take<u32>(A);
-      u64* r_keys_a = scratch.take<u64>(2 * static_cast<std::size_t>(A));
-      u64* r_keys_b = scratch.take<u64>(2 * static_cast<std::size_t>(A));
-      u32* r_idx_a = scratch.take<u32>(A);
-      u32* r_idx_b = scratch.take<u32>(A);
-      refine_records_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(
-          list, n_cur, tied, tpos, head, hscan, idx, x, z, W, static_cast<int>(w), plist, r_keys_a,
-          r_idx_a);
-      check_launch("refine_records");
-      // Records [run, word w]: word w's varying bits, then the run ranks
-      // (all bits below the bit length of n_cur).
-      int run_bits = 0;
-      while (run_bits < 64 && (u64{1} << run_bits) < n_cur) ++run_bits;
-      const u64 run_varying = run_bits >= 64 ? ~0ULL : ((u64{1} << run_bits) - 1);
-      radix_sort(r_keys_a, r_idx_a, r_keys_b, r_idx_b, A, 2,
-                 digit_passes({run_varying, ors[w] ^ ands[w]}), scratch, stream);
-      refine_scatter_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, r_idx_a, A, idx);
-      check_launch("refine_scatter");
-      // The next level works on the tied items, in their new order.
-      u32* head2 = scratch.take<u32>(A);
-      run_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(r_keys_a, A, 2, head2);
-      check_launch("run_heads");
-      head = head2;
-      list = plist;
-      n_cur = A;
-    }
-    full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, x, z, W, T, flags);
-    check_launch("full_heads");
+    word_major_heads(ArrayTerms{x, z, W}, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch,
+                     stream);
   }
 
   // Segments.
@@ -992,6 +1201,155 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     compact_gather_kernel<<<stride_grid(S), 256, 0, stream>>>(
         seg_start, S, keep, out_pos, idx_a, x, z, W, sums, out->x, out->z, out->coeffs);
     check_launch("compact");
+  }
+  sync(stream);
+  return out.release();
+}
+
+wg_pauli_sum* simplify_products(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, double atol,
+                                double rtol, int seed_mode, Scratch& scratch,
+                                hipStream_t stream) {
+  const std::size_t n = lhs->num_qubits, W = lhs->words;
+  const std::size_t L = lhs->num_terms, R = rhs->num_terms;
+  const int device = lhs->device;
+  if (W < 1 || W > 4) fail_runtime("product simplify handles 1 to 4 mask words");
+  const std::size_t T = checked_mul(L, R, "matmul products");
+  if (T == 0) return new_pauli_sum(n, 0, device);
+  if (T >= 0xFFFFFFFFULL) fail_invalid("simplify supports at most 2^32 - 2 terms per call");
+  const bool packed = W == 1 && n <= 32;
+  const std::size_t RW = product_record_words(static_cast<u32>(W));
+  u64* lrec = scratch.take<u64>(checked_mul(L, RW, "operand records"));
+  u64* rrec = scratch.take<u64>(checked_mul(R, RW, "operand records"));
+  operand_records_kernel<<<stride_grid(L), 256, 0, stream>>>(lhs->x, lhs->z, lhs->coeffs, L,
+                                                             static_cast<u32>(W), lrec);
+  check_launch("operand_records");
+  operand_records_kernel<<<stride_grid(R), 256, 0, stream>>>(rhs->x, rhs->z, rhs->coeffs, R,
+                                                             static_cast<u32>(W), rrec);
+  check_launch("operand_records");
+  const ProductTerms P{lrec, rrec, static_cast<u32>(R), static_cast<u32>(W)};
+
+  u32* flags = scratch.take<u32>(T);
+  u64* keys_a = scratch.take<u64>(T);
+  u32* idx_a = scratch.take<u32>(T);
+  u64* keys_b = scratch.take<u64>(T);
+  u32* idx_b = scratch.take<u32>(T);
+  // Statistics: OR of the 2W key words, AND of them, then max |c| (bits),
+  // per block and then combined. max |c| only matters when rtol != 0.
+  const u32 chunks = static_cast<u32>((R + kProdChunk - 1) / kProdChunk);
+  const std::size_t units = checked_mul(L, chunks, "matmul grid");
+  const unsigned grid = grid_for(units, 1);
+  const u32 ns = static_cast<u32>(4 * W + 1);
+  std::vector<unsigned long long> stat(ns);
+  u64* block_stats = scratch.take<u64>(checked_mul(units, ns, "product statistics"));
+  u64* dstat = scratch.take<u64>(ns);
+  {
+    ProfileScope prof("product_keys", stream);
+    auto launch = [&](auto kernel) {
+      kernel<<<grid, 256, 0, stream>>>(P, chunks, block_stats, keys_a, idx_a);
+    };
+    const bool need_max = rtol != 0.0;
+    if (packed) {
+      if (need_max) launch(product_keys_kernel<1, true, true>);
+      else launch(product_keys_kernel<1, true, false>);
+    } else if (need_max) {
+      switch (W) {
+        case 1: launch(product_keys_kernel<1, false, true>); break;
+        case 2: launch(product_keys_kernel<2, false, true>); break;
+        case 3: launch(product_keys_kernel<3, false, true>); break;
+        default: launch(product_keys_kernel<4, false, true>); break;
+      }
+    } else {
+      switch (W) {
+        case 1: launch(product_keys_kernel<1, false, false>); break;
+        case 2: launch(product_keys_kernel<2, false, false>); break;
+        case 3: launch(product_keys_kernel<3, false, false>); break;
+        default: launch(product_keys_kernel<4, false, false>); break;
+      }
+    }
+    check_launch("product_keys");
+  }
+  stats_reduce_kernel<<<ns, 1024, 0, stream>>>(block_stats, static_cast<u32>(units),
+                                               static_cast<u32>(2 * W), dstat);
+  check_launch("stats_reduce");
+  check_hip(hipMemcpyAsync(stat.data(), dstat, ns * sizeof(u64), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+  const std::vector<unsigned long long> ors(stat.begin(), stat.begin() + 2 * W);
+  const std::vector<unsigned long long> ands(stat.begin() + 2 * W, stat.begin() + 4 * W);
+  double max_abs = 0.0;
+  std::memcpy(&max_abs, &stat[4 * W], sizeof(double));
+  const double threshold = atol + rtol * max_abs;
+
+  bool hash_seed;
+  if (seed_mode < 0) {
+    // The reference's rule samples the first 4,096 products.
+    const u32 count = static_cast<u32>(std::min<std::size_t>(T, 4096));
+    u64* sx = scratch.take<u64>(static_cast<std::size_t>(count) * W);
+    u64* sz = scratch.take<u64>(static_cast<std::size_t>(count) * W);
+    product_sample_kernel<<<(count + 255) / 256, 256, 0, stream>>>(P, count, sx, sz);
+    check_launch("product_sample");
+    hash_seed = simplify_takes_hash_path(sx, sz, T, W, stream);
+  } else {
+    hash_seed = seed_mode != 0;
+  }
+
+  if (packed) {
+    const u64 kor = (ors[0] << 32) | ors[1];
+    const u64 kand = (ands[0] << 32) | ands[1];
+    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
+    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, 1, flags);
+    check_launch("run_heads");
+  } else {
+    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
+               stream);
+    word_major_heads(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch, stream);
+  }
+
+  // Segments.
+  u32* seg_pos = scratch.take<u32>(T);
+  u32* seg_start = scratch.take<u32>(T + 1);
+  u32* d_count = scratch.take<u32>(2);
+  exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
+  segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
+  check_launch("segment_starts");
+  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+  check_launch("clear_count");
+  u32 S = 0;
+  check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+
+  // Sums and output at each segment's position; compacted only when some
+  // segment is dropped.
+  u32* keep = scratch.take<u32>(S);
+  std::unique_ptr<wg_pauli_sum> staged(new_pauli_sum(n, S, device));
+  {
+    ProfileScope prof("product_emit", stream);
+    auto emit = [&](auto kernel) {
+      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, P, hash_seed ? 0 : 1,
+                                                 threshold, staged->x, staged->z, staged->coeffs,
+                                                 keep, d_count + 1);
+    };
+    switch (W) {
+      case 1: emit(product_emit_kernel<1>); break;
+      case 2: emit(product_emit_kernel<2>); break;
+      case 3: emit(product_emit_kernel<3>); break;
+      default: emit(product_emit_kernel<4>); break;
+    }
+    check_launch("product_emit");
+  }
+  u32 U = 0;
+  check_hip(hipMemcpyAsync(&U, d_count + 1, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+  if (U == S) return staged.release();
+  u32* out_pos = scratch.take<u32>(S);
+  exclusive_scan_u32(keep, out_pos, S, nullptr, scratch, stream);
+  std::unique_ptr<wg_pauli_sum> out(new_pauli_sum(n, U, device));
+  if (U) {
+    compact_emitted_kernel<<<stride_grid(S), 256, 0, stream>>>(
+        keep, out_pos, S, W, staged->x, staged->z, staged->coeffs, out->x, out->z, out->coeffs);
+    check_launch("compact_emitted");
   }
   sync(stream);
   return out.release();
