@@ -242,12 +242,16 @@ void wg_batch_destroy(wg_batch* batch);
  * bit-identical to wg_matmul on that pair): lhs[i] @ rhs[i] for `count` pairs
  * of uneven term and qubit counts on one device. Creation validates every pair
  * with matmul's messages and limits (max_intermediate_terms per pair), packs
- * the pairs into chunks whose intermediate products fit budget_bytes (0 =
- * 16 GiB) and captures, per chunk, one persistent product kernel fed by a
- * device work queue into a HIP graph. The operands must outlive the batch.
- * wg_product_batch_run replays the graphs and writes `count` new operators to
- * outs (simplified with matmul's rule when simplify_output != 0); it
- * synchronizes `stream` before returning. */
+ * the pairs into chunks that fit budget_bytes (0 = 16 GiB) and captures, per
+ * chunk, one persistent kernel fed by a device work queue into a HIP graph.
+ * With simplify (pairs of 1 to 256 qubits) the kernel writes each product's
+ * sort key and index and the products are never materialized; the chunks are
+ * sized for those records. Runs without simplify materialize the products in
+ * chunks sized for them (graphs captured on the first such run). The operands
+ * must outlive the batch and are read at every run. wg_product_batch_run
+ * replays the graphs and writes `count` new operators to outs (simplified with
+ * matmul's rule when simplify_output != 0); it synchronizes `stream` before
+ * returning. wg_product_batch_info reports the chunks of the simplify path. */
 typedef struct wg_product_batch wg_product_batch;
 int wg_product_batch_create(size_t count, const wg_pauli_sum* const* lhs,
                             const wg_pauli_sum* const* rhs, size_t max_intermediate_terms,

@@ -84,6 +84,36 @@ void launch_products(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, u64* ox, 
 }
 
 }  // namespace
+
+int matmul_seed_mode(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, hipStream_t stream) {
+  // Reference src/multiply.cpp:107-261: with W in {1, 2} and a
+  // duplicate-heavy operand the reference accumulates in a hash map seeded by
+  // the first product, with a fixed 1e-12 drop; otherwise it calls simplify()
+  // with its defaults on the materialized products.
+  const std::size_t W = lhs->words;
+  const bool hash = (W == 1 || W == 2) && lhs->num_terms != 0 && rhs->num_terms != 0 &&
+                    (operand_duplicate_heavy(lhs, stream) || operand_duplicate_heavy(rhs, stream));
+  return hash ? 1 : -1;
+}
+
+wg_pauli_sum* matmul_device(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, int seed_mode,
+                            Scratch& scratch, hipStream_t stream) {
+  const std::size_t W = lhs->words;
+  const std::size_t P = checked_mul(lhs->num_terms, rhs->num_terms, "matmul products");
+  if (P == 0) return new_pauli_sum(lhs->num_qubits, 0, lhs->device);
+  if (W >= 1 && W <= 4 && P < 0xFFFFFFFFULL) {
+    // Up to 256 qubits the products are never materialized: the simplify
+    // reads them through the operands.
+    return simplify_products(lhs, rhs, 1.0e-12, 0.0, seed_mode, scratch, stream);
+  }
+  u64* px = scratch.take<u64>(checked_mul(P, W, "matmul products"));
+  u64* pz = scratch.take<u64>(P * W);
+  double2* pc = scratch.take<double2>(P);
+  launch_products(lhs, rhs, px, pz, pc, stream);
+  return simplify_device(px, pz, pc, lhs->num_qubits, P, lhs->device, 1.0e-12, 0.0, seed_mode,
+                         scratch, stream);
+}
+
 }  // namespace wg
 
 using namespace wg;
@@ -129,24 +159,7 @@ int wg_matmul(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, int simplify_out
       return;
     }
     Scratch scratch(lhs->device);
-    // Reference src/multiply.cpp:107-261: with W in {1, 2} and a
-    // duplicate-heavy operand the reference accumulates in a hash map seeded by
-    // the first product, with a fixed 1e-12 drop; otherwise it calls
-    // simplify() with its defaults on the materialized products.
-    const bool hash = (W == 1 || W == 2) &&
-                      (operand_duplicate_heavy(lhs, stream) || operand_duplicate_heavy(rhs, stream));
-    if (W >= 1 && W <= 4) {
-      // Up to 256 qubits the products are never materialized: the simplify
-      // reads them through the operands.
-      *out = simplify_products(lhs, rhs, 1.0e-12, 0.0, hash ? 1 : -1, scratch, stream);
-      return;
-    }
-    u64* px = scratch.take<u64>(checked_mul(P, W, "matmul products"));
-    u64* pz = scratch.take<u64>(P * W);
-    double2* pc = scratch.take<double2>(P);
-    launch_products(lhs, rhs, px, pz, pc, stream);
-    *out = simplify_device(px, pz, pc, lhs->num_qubits, P, lhs->device, 1.0e-12, 0.0,
-                           hash ? 1 : -1, scratch, stream);
+    *out = matmul_device(lhs, rhs, matmul_seed_mode(lhs, rhs, stream), scratch, stream);
   });
 }
 

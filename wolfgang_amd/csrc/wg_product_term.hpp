@@ -110,4 +110,81 @@ struct ProductTerms {
   }
 };
 
+// Key pass of the unmaterialized products, one unit = lhs term i times rhs
+// terms [j0, j0 + kProductChunk) on a kProductKeyBlock-thread block: writes
+// each product's word-0 sort key (x << 32 | z when kPacked) and its index
+// t = i * R + j, and the unit's statistics — OR of each of the 2W key words
+// [x0, z0, x1, z1, ...], their AND, and max |c| as bits (|c| >= 0, so the
+// bits order like the values; only with kMax) — at block_stats[k * stride +
+// unit] for k < 4W + 1.
+constexpr u32 kProductKeyBlock = 256;
+constexpr u32 kProductPerThread = 4;
+constexpr u32 kProductChunk = kProductKeyBlock * kProductPerThread;
+constexpr int kProductMaxStats = 4 * 4 + 1;
+
+template <int W, bool kPacked, bool kMax>
+__device__ inline void product_keys_unit(const ProductTerms& P, u32 i, u32 j0,
+                                         u64* __restrict__ keys, u32* __restrict__ idx,
+                                         u64* __restrict__ block_stats, std::size_t stride,
+                                         std::size_t unit,
+                                         u64 (*lds)[kProductMaxStats]) {
+  static_assert(W >= 1 && W <= 4 && (!kPacked || W == 1), "product key shapes");
+  constexpr int NS = 4 * W + 1;
+  u64 o[2 * W], a[2 * W];
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    o[k] = 0;
+    a[k] = ~0ULL;
+  }
+  double m = 0.0;
+#pragma unroll
+  for (u32 q = 0; q < kProductPerThread; ++q) {
+    const u32 j = j0 + q * kProductKeyBlock + threadIdx.x;
+    if (j >= P.R) break;
+    u64 ox[W], oz[W];
+    const double2 c = P.term_ij<W>(i, j, ox, oz);
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      o[2 * w] |= ox[w];
+      a[2 * w] &= ox[w];
+      o[2 * w + 1] |= oz[w];
+      a[2 * w + 1] &= oz[w];
+    }
+    if constexpr (kMax) m = fmax(m, hypot(c.x, c.y));
+    const std::size_t t = static_cast<std::size_t>(i) * P.R + j;
+    keys[t] = kPacked ? ((ox[0] << 32) | oz[0]) : ox[0];
+    idx[t] = static_cast<u32>(t);
+  }
+  // Block partials: waves by shuffles, then the waves through LDS.
+  const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 2 * W; ++k) {
+    u64 ok = o[k], ak = a[k];
+    for (int d = 32; d > 0; d >>= 1) {
+      ok |= __shfl_xor(ok, d, 64);
+      ak &= __shfl_xor(ak, d, 64);
+    }
+    if (lane == 0) {
+      lds[wave][k] = ok;
+      lds[wave][2 * W + k] = ak;
+    }
+  }
+  if constexpr (kMax) {
+    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
+  }
+  if (lane == 0) lds[wave][4 * W] = static_cast<u64>(__double_as_longlong(m));
+  __syncthreads();
+  if (threadIdx.x < static_cast<u32>(NS)) {
+    const u32 k = threadIdx.x;
+    u64 v = lds[0][k];
+    for (u32 w = 1; w < kProductKeyBlock / 64; ++w) {
+      const u64 u = lds[w][k];
+      v = k < 2 * W ? (v | u) : k < 4 * W ? (v & u) : (u > v ? u : v);
+    }
+    block_stats[static_cast<std::size_t>(k) * stride + unit] = v;
+  }
+  // The LDS partials are read before the block's next unit writes them.
+  __syncthreads();
+}
+
 }  // namespace wg

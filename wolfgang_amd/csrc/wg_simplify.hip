@@ -671,8 +671,6 @@ __global__ __launch_bounds__(256) void compact_records_kernel(
 // from HBM in sorted (random) order. The coefficient arithmetic is the product
 // kernel's, so the sums are bit-identical to simplifying materialized products.
 
-constexpr u32 kProdPerThread = 4;
-constexpr u32 kProdChunk = 256 * kProdPerThread;
 
 // Operand records (product_record_words(W) words per term: x words, z words,
 // coefficient bits, zero padding).
@@ -695,71 +693,16 @@ __global__ __launch_bounds__(256) void operand_records_kernel(const u64* __restr
   }
 }
 
+// One block per (lhs term, 1,024 rhs terms), as the product kernel.
 template <int W, bool kPacked, bool kMax>
-__global__ __launch_bounds__(256) void product_keys_kernel(ProductTerms P, u32 chunks,
-                                                           u64* __restrict__ block_stats,
-                                                           u64* __restrict__ keys,
-                                                           u32* __restrict__ idx) {
-  static_assert(W >= 1 && W <= 4 && (!kPacked || W == 1), "product key shapes");
-  constexpr int NS = 4 * W + 1;  // OR of the 2W key words, AND of them, max |c| (bits)
-  __shared__ u64 lds[4][NS];
-  // One block per (lhs term, 1,024 rhs terms), as the product kernel.
+__global__ __launch_bounds__(kProductKeyBlock) void product_keys_kernel(
+    ProductTerms P, u32 chunks, u64* __restrict__ block_stats, u64* __restrict__ keys,
+    u32* __restrict__ idx) {
+  __shared__ u64 lds[kProductKeyBlock / 64][kProductMaxStats];
   const u32 i = blockIdx.x / chunks;
-  const u32 j0 = (blockIdx.x - i * chunks) * kProdChunk;
-  u64 o[2 * W], a[2 * W];
-#pragma unroll
-  for (int k = 0; k < 2 * W; ++k) {
-    o[k] = 0;
-    a[k] = ~0ULL;
-  }
-  double m = 0.0;
-#pragma unroll
-  for (u32 q = 0; q < kProdPerThread; ++q) {
-    const u32 j = j0 + q * 256U + threadIdx.x;
-    if (j >= P.R) break;
-    u64 ox[W], oz[W];
-    const double2 c = P.term_ij<W>(i, j, ox, oz);
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      o[2 * w] |= ox[w];
-      a[2 * w] &= ox[w];
-      o[2 * w + 1] |= oz[w];
-      a[2 * w + 1] &= oz[w];
-    }
-    if constexpr (kMax) m = fmax(m, hypot(c.x, c.y));
-    const std::size_t t = static_cast<std::size_t>(i) * P.R + j;
-    keys[t] = kPacked ? ((ox[0] << 32) | oz[0]) : ox[0];
-    idx[t] = static_cast<u32>(t);
-  }
-  // Block partials: waves by shuffles, then the 4 waves through LDS.
-  const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 2 * W; ++k) {
-    u64 ok = o[k], ak = a[k];
-    for (int d = 32; d > 0; d >>= 1) {
-      ok |= __shfl_xor(ok, d, 64);
-      ak &= __shfl_xor(ak, d, 64);
-    }
-    if (lane == 0) {
-      lds[wave][k] = ok;
-      lds[wave][2 * W + k] = ak;
-    }
-  }
-  if constexpr (kMax) {
-    for (int d = 32; d > 0; d >>= 1) m = fmax(m, __shfl_xor(m, d, 64));
-  }
-  // |c| >= 0, so the bit patterns order like the values.
-  if (lane == 0) lds[wave][4 * W] = static_cast<u64>(__double_as_longlong(m));
-  __syncthreads();
-  if (threadIdx.x < static_cast<u32>(NS)) {
-    const u32 k = threadIdx.x;
-    u64 v = lds[0][k];
-    for (int w = 1; w < 4; ++w) {
-      const u64 u = lds[w][k];
-      v = k < 2 * W ? (v | u) : k < 4 * W ? (v & u) : (u > v ? u : v);
-    }
-    block_stats[static_cast<std::size_t>(k) * gridDim.x + blockIdx.x] = v;
-  }
+  const u32 j0 = (blockIdx.x - i * chunks) * kProductChunk;
+  product_keys_unit<W, kPacked, kMax>(P, i, j0, keys, idx, block_stats, gridDim.x, blockIdx.x,
+                                      lds);
 }
 
 // Combines the per-block statistics of product_keys_kernel (word k of block b
@@ -1206,6 +1149,13 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   return out.release();
 }
 
+void build_operand_records(const wg_pauli_sum* op, u64* rec, hipStream_t stream) {
+  if (op->num_terms == 0) return;
+  operand_records_kernel<<<stride_grid(op->num_terms), 256, 0, stream>>>(
+      op->x, op->z, op->coeffs, op->num_terms, static_cast<u32>(op->words), rec);
+  check_launch("operand_records");
+}
+
 wg_pauli_sum* simplify_products(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, double atol,
                                 double rtol, int seed_mode, Scratch& scratch,
                                 hipStream_t stream) {
@@ -1220,32 +1170,23 @@ wg_pauli_sum* simplify_products(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs
   const std::size_t RW = product_record_words(static_cast<u32>(W));
   u64* lrec = scratch.take<u64>(checked_mul(L, RW, "operand records"));
   u64* rrec = scratch.take<u64>(checked_mul(R, RW, "operand records"));
-  operand_records_kernel<<<stride_grid(L), 256, 0, stream>>>(lhs->x, lhs->z, lhs->coeffs, L,
-                                                             static_cast<u32>(W), lrec);
-  check_launch("operand_records");
-  operand_records_kernel<<<stride_grid(R), 256, 0, stream>>>(rhs->x, rhs->z, rhs->coeffs, R,
-                                                             static_cast<u32>(W), rrec);
-  check_launch("operand_records");
+  build_operand_records(lhs, lrec, stream);
+  build_operand_records(rhs, rrec, stream);
   const ProductTerms P{lrec, rrec, static_cast<u32>(R), static_cast<u32>(W)};
 
-  u32* flags = scratch.take<u32>(T);
-  u64* keys_a = scratch.take<u64>(T);
-  u32* idx_a = scratch.take<u32>(T);
-  u64* keys_b = scratch.take<u64>(T);
-  u32* idx_b = scratch.take<u32>(T);
+  u64* keys = scratch.take<u64>(T);
+  u32* idx = scratch.take<u32>(T);
   // Statistics: OR of the 2W key words, AND of them, then max |c| (bits),
-  // per block and then combined. max |c| only matters when rtol != 0.
-  const u32 chunks = static_cast<u32>((R + kProdChunk - 1) / kProdChunk);
+  // per block; max |c| only matters when rtol != 0.
+  const u32 chunks = static_cast<u32>((R + kProductChunk - 1) / kProductChunk);
   const std::size_t units = checked_mul(L, chunks, "matmul grid");
   const unsigned grid = grid_for(units, 1);
   const u32 ns = static_cast<u32>(4 * W + 1);
-  std::vector<unsigned long long> stat(ns);
   u64* block_stats = scratch.take<u64>(checked_mul(units, ns, "product statistics"));
-  u64* dstat = scratch.take<u64>(ns);
   {
     ProfileScope prof("product_keys", stream);
     auto launch = [&](auto kernel) {
-      kernel<<<grid, 256, 0, stream>>>(P, chunks, block_stats, keys_a, idx_a);
+      kernel<<<grid, kProductKeyBlock, 0, stream>>>(P, chunks, block_stats, keys, idx);
     };
     const bool need_max = rtol != 0.0;
     if (packed) {
@@ -1268,6 +1209,24 @@ wg_pauli_sum* simplify_products(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs
     }
     check_launch("product_keys");
   }
+  return simplify_product_keys(P, n, T, device, keys, idx, block_stats, units, atol, rtol,
+                               seed_mode, scratch, stream);
+}
+
+wg_pauli_sum* simplify_product_keys(const ProductTerms& P, std::size_t n, std::size_t T,
+                                    int device, u64* keys, u32* idx, const u64* block_stats,
+                                    std::size_t units, double atol, double rtol, int seed_mode,
+                                    Scratch& scratch, hipStream_t stream) {
+  const std::size_t W = P.W;
+  const bool packed = W == 1 && n <= 32;
+  const u32 ns = static_cast<u32>(4 * W + 1);
+  std::vector<unsigned long long> stat(ns);
+  u64* dstat = scratch.take<u64>(ns);
+  u64* keys_a = keys;
+  u32* idx_a = idx;
+  u32* flags = scratch.take<u32>(T);
+  u64* keys_b = scratch.take<u64>(T);
+  u32* idx_b = scratch.take<u32>(T);
   stats_reduce_kernel<<<ns, 1024, 0, stream>>>(block_stats, static_cast<u32>(units),
                                                static_cast<u32>(2 * W), dstat);
   check_launch("stats_reduce");
