@@ -163,11 +163,31 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const u64* __restric
 // in LDS, and writes the sorted tile out: consecutive threads write
 // consecutive positions of a digit's run (coalesced), where scattering each
 // record from its original slot would touch one cache line per record.
-template <int KW>
+//
+// With kLook (one-sweep passes, one-word keys) the block's tile is the next
+// number from look.tile_counter and the digits' output offsets come from the
+// upfront digit starts plus a decoupled look-back over the earlier tiles'
+// published digit counts (LookBack below), instead of a per-pass histogram and
+// scan (offsets).
+struct LookBack {
+  const u32* starts;  // the pass's 256 digit starts
+  u32* status;        // [tiles][256] published counts: flag bits 30..31, value 0..29
+  u32* tile_counter;
+  u32* error;
+};
+constexpr u32 kLookAggregate = 1U << 30;  // value = this tile's digit count
+constexpr u32 kLookInclusive = 2U << 30;  // value = digit count of tiles 0..this
+constexpr u32 kLookValue = (1U << 30) - 1;
+// Polls of a predecessor's status word before the pass gives up and reports
+// an error (the wait ends on its own: every tile publishes its counts before
+// it waits, and tiles are numbered in the order their blocks start).
+constexpr u32 kLookSpins = 1U << 16;
+
+template <int KW, bool kLook>
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const u64* __restrict__ keys_in, const u32* __restrict__ idx_in, u64* __restrict__ keys_out,
     u32* __restrict__ idx_out, std::size_t T, int word, int shift, std::size_t nblocks,
-    const u32* __restrict__ offsets) {
+    const u32* __restrict__ offsets, LookBack look) {
   constexpr int kWaves = kBlock / 64;
   constexpr int kRows = kTile / kBlock;
   static_assert(kBlock == 256, "one thread per digit");
@@ -180,11 +200,18 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   __shared__ u64 skeys[kTile * KW];
   __shared__ u32 sidx[kTile];
   const u32 tid = threadIdx.x, lane = tid & 63U, wave = tid >> 6;
+  __shared__ u32 tile_s;
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) counts[w][tid] = 0;
-  gbase[tid] = offsets[static_cast<std::size_t>(tid) * nblocks + blockIdx.x];
+  if constexpr (kLook) {
+    if (tid == 0) tile_s = atomicAdd(look.tile_counter, 1U);
+  } else {
+    gbase[tid] = offsets[static_cast<std::size_t>(tid) * nblocks + blockIdx.x];
+    if (tid == 0) tile_s = blockIdx.x;
+  }
   __syncthreads();
-  const std::size_t tile = static_cast<std::size_t>(blockIdx.x) * kTile;
+  const u32 tile_id = tile_s;
+  const std::size_t tile = static_cast<std::size_t>(tile_id) * kTile;
   const u32 n_tile = static_cast<u32>(std::min<std::size_t>(kTile, T - tile));
   const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
   u64 key[kRows][KW];
@@ -234,6 +261,12 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     counts[w][tid] = static_cast<unsigned short>(total);
     total += c;
   }
+  u32* my_status = nullptr;
+  if constexpr (kLook) {
+    my_status = look.status + static_cast<std::size_t>(tile_id) * 256 + tid;
+    __hip_atomic_store(my_status, (tile_id == 0 ? kLookInclusive : kLookAggregate) | total,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   u32 incl = total;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -245,6 +278,31 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   u32 before_wave = 0;
   for (u32 w = 0; w < wave; ++w) before_wave += wsum[w];
   tstart[tid] = static_cast<unsigned short>(before_wave + incl - total);
+  if constexpr (kLook) {
+    // Digit tid's count in the tiles before this one: walk back until a tile
+    // that has published its inclusive count.
+    u32 prefix = 0;
+    for (u32 t = tile_id; t-- > 0;) {
+      const u32* st = look.status + static_cast<std::size_t>(t) * 256 + tid;
+      u32 v = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (u32 spin = 0; (v >> 30) == 0; ++spin) {
+        if (spin == kLookSpins) {
+          atomicOr(look.error, 1U);
+          v = kLookInclusive;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      prefix += v & kLookValue;
+      if ((v >> 30) == 2) break;
+    }
+    if (tile_id > 0) {
+      __hip_atomic_store(my_status, kLookInclusive | (prefix + total), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    gbase[tid] = look.starts[tid] + prefix;
+  }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
@@ -270,7 +328,115 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   }
 }
 
+// Digit counts of every pass of a one-sweep sort in one read of the keys
+// (counts do not depend on the order the passes leave the keys in):
+// hist[p * 256 + d] for the digit at shifts.s[p].
+struct DigitShifts {
+  int s[8];
+};
+
+__global__ __launch_bounds__(256) void digit_hist_all_kernel(const u64* __restrict__ keys,
+                                                             std::size_t n, DigitShifts shifts,
+                                                             int passes, u32* __restrict__ hist) {
+  // One histogram set per wave: the waves' atomics never collide (the biased
+  // digits of product keys make one bin hot).
+  __shared__ u32 h[4][8][256];
+  const u32 wave = threadIdx.x >> 6;
+  for (int w = 0; w < 4; ++w) {
+    for (int p = 0; p < 8; ++p) h[w][p][threadIdx.x] = 0;
+  }
+  __syncthreads();
+  // Four keys per thread per step, loaded before any of them is counted.
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    u64 k[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) k[q] = keys[i + q * stride];
+    for (int p = 0; p < passes; ++p) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        atomicAdd(&h[wave][p][static_cast<u32>(k[q] >> shifts.s[p]) & 0xFFU], 1U);
+      }
+    }
+  }
+  for (; i < n; i += stride) {
+    const u64 k = keys[i];
+    for (int p = 0; p < passes; ++p) {
+      atomicAdd(&h[wave][p][static_cast<u32>(k >> shifts.s[p]) & 0xFFU], 1U);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p) {
+    const u32 c = h[0][p][threadIdx.x] + h[1][p][threadIdx.x] + h[2][p][threadIdx.x] +
+                  h[3][p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+  }
+}
+
+// starts[p * 256 + d] = exclusive prefix over digits of hist[p * 256 + .]
+// (one block per pass).
+__global__ __launch_bounds__(256) void digit_starts_kernel(const u32* __restrict__ hist,
+                                                           u32* __restrict__ starts) {
+  __shared__ u32 wsum[4];
+  const u32 tid = threadIdx.x, lane = tid & 63U, wave = tid >> 6;
+  const u32 c = hist[blockIdx.x * 256 + tid];
+  u32 incl = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    const u32 t = static_cast<u32>(__shfl_up(static_cast<int>(incl), off, 64));
+    if (lane >= static_cast<u32>(off)) incl += t;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  u32 before = 0;
+  for (u32 w = 0; w < wave; ++w) before += wsum[w];
+  starts[blockIdx.x * 256 + tid] = before + incl - c;
+}
+
+__global__ __launch_bounds__(256) void clear_u32_kernel(u32* __restrict__ p, std::size_t n) {
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    p[i] = 0;
+  }
+}
+
 // ---------------------------------------------------------------- segments
+
+// Bounds of segment s in sorted order. seg_start == nullptr means every key
+// is distinct: segment s is position s alone.
+__device__ inline void segment_bounds(const u32* __restrict__ seg_start, std::size_t s, u32& b,
+                                      u32& e) {
+  if (seg_start) {
+    b = seg_start[s];
+    e = seg_start[s + 1];
+  } else {
+    b = static_cast<u32>(s);
+    e = static_cast<u32>(s + 1);
+  }
+}
+
+// head[i] = 1 where a run of equal one-word keys starts; *ties gains the
+// number of non-heads (one atomic per block that has any, so distinct keys
+// cost no atomics).
+__global__ __launch_bounds__(256) void key_heads_kernel(const u64* __restrict__ keys,
+                                                        std::size_t n, u32* __restrict__ head,
+                                                        u32* __restrict__ ties) {
+  __shared__ u32 lds[4];
+  u32 tied = 0;
+  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const bool h = i == 0 || keys[i] != keys[i - 1];
+    head[i] = h ? 1U : 0U;
+    tied += h ? 0U : 1U;
+  }
+  for (int d = 32; d > 0; d >>= 1) tied += static_cast<u32>(__shfl_xor(static_cast<int>(tied), d, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = tied;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32 total = lds[0] + lds[1] + lds[2] + lds[3];
+    if (total) atomicAdd(ties, total);
+  }
+}
 
 __global__ __launch_bounds__(256) void segment_starts_kernel(const u32* __restrict__ flags,
                                                              const u32* __restrict__ seg_pos,
@@ -289,7 +455,8 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(
     u32* __restrict__ keep) {
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    const u32 b = seg_start[s], e = seg_start[s + 1];
+    u32 b, e;
+    segment_bounds(seg_start, s, b, e);
     double2 acc = c[idx[b]];
     for (u32 j = b + 1; j < e; ++j) {
       const double2 v = c[idx[j]];
@@ -314,7 +481,9 @@ __global__ __launch_bounds__(256) void compact_kernel(
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
-    const std::size_t k = seg_start[s];
+    u32 kb, ke;
+    segment_bounds(seg_start, s, kb, ke);
+    const std::size_t k = kb;
     if (W == 0) {
       // Zero-qubit operators have no mask words (their buffers are null).
     } else if (KW == 1) {
@@ -445,7 +614,9 @@ __global__ __launch_bounds__(256) void compact_gather_kernel(
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
-    const std::size_t t = idx[seg_start[s]];
+    u32 b, e;
+    segment_bounds(seg_start, s, b, e);
+    const std::size_t t = idx[b];
     for (std::size_t w = 0; w < W; ++w) {
       ox[o * W + w] = x[t * W + w];
       oz[o * W + w] = z[t * W + w];
@@ -572,7 +743,8 @@ __global__ __launch_bounds__(256) void segment_emit_kernel(
   u32 kept = 0;
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    const u32 b = seg_start[s], e = seg_start[s + 1];
+    u32 b, e;
+    segment_bounds(seg_start, s, b, e);
     const u64* head = rec + static_cast<std::size_t>(idx[b]) * RW;
     u64 r[2 * W + 2];
 #pragma unroll
@@ -643,7 +815,9 @@ __global__ __launch_bounds__(256) void compact_records_kernel(
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
-    const std::size_t t = idx[seg_start[s]];
+    u32 b, e;
+    segment_bounds(seg_start, s, b, e);
+    const std::size_t t = idx[b];
     u64 r[RW];
 #pragma unroll
     for (int k = 0; k < RW; k += 2) {
@@ -755,7 +929,8 @@ __global__ __launch_bounds__(256) void product_emit_kernel(
   u32 kept = 0;
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    const u32 b = seg_start[s], e = seg_start[s + 1];
+    u32 b, e;
+    segment_bounds(seg_start, s, b, e);
     u64 mx[W], mz[W];
     double2 acc = P.term<W>(idx[b], mx, mz);
 #pragma unroll
@@ -881,13 +1056,57 @@ std::vector<DigitPass> digit_passes(const std::vector<u64>& varying) {
   return passes;
 }
 
+// One-word sorts of at least this many records (and fewer than 2^30) run
+// their digit passes one-sweep.
+constexpr std::size_t kOneSweepMin = std::size_t{1} << 20;
+
 // Stable LSD radix sort of (KW-word key, index) records; the sorted records
-// end up in (keys_a, idx_a).
+// end up in (keys_a, idx_a). Large one-word sorts take one histogram of every
+// pass's digits up front and then one kernel per pass (look-back offsets);
+// the others run histogram -> scan -> scatter per pass.
 void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_t n, int KW,
                 const std::vector<DigitPass>& passes, Scratch& scratch, hipStream_t stream,
                 const char* prof_name = "radix_scatter") {
   if (passes.empty() || n < 2) return;
   const std::size_t nblocks = (n + kTile - 1) / kTile;
+  if (KW == 1 && n >= kOneSweepMin && n < (std::size_t{1} << 30) && passes.size() <= 8) {
+    const std::size_t P = passes.size();
+    // One allocation: status words of every pass, tile counters, error flag,
+    // digit counts and starts; cleared by a kernel.
+    const std::size_t status_words = P * nblocks * 256;
+    const std::size_t words = status_words + P + 1 + 2 * P * 256;
+    u32* mem = scratch.take<u32>(words);
+    u32* status = mem;
+    u32* counters = status + status_words;
+    u32* error = counters + P;
+    u32* hist = error + 1;
+    u32* starts = hist + P * 256;
+    clear_u32_kernel<<<stride_grid(words), 256, 0, stream>>>(mem, words);
+    check_launch("clear");
+    DigitShifts sh{};
+    for (std::size_t p = 0; p < P; ++p) sh.s[p] = passes[p].shift;
+    digit_hist_all_kernel<<<std::min<unsigned>(stride_grid(n), 2048), 256, 0, stream>>>(
+        keys_a, n, sh, static_cast<int>(P), hist);
+    check_launch("digit_hist_all");
+    digit_starts_kernel<<<static_cast<unsigned>(P), 256, 0, stream>>>(hist, starts);
+    check_launch("digit_starts");
+    for (std::size_t p = 0; p < P; ++p) {
+      const LookBack look{starts + p * 256, status + p * nblocks * 256, counters + p, error};
+      ProfileScope prof(prof_name, stream);
+      radix_scatter_kernel<1, true><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, n, passes[p].word, passes[p].shift, nblocks, nullptr,
+          look);
+      check_launch("radix_scatter");
+      std::swap(keys_a, keys_b);
+      std::swap(idx_a, idx_b);
+    }
+    u32 err = 0;
+    check_hip(hipMemcpyAsync(&err, error, sizeof(u32), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+    if (err) fail_runtime("radix sort look-back did not complete");
+    return;
+  }
   u32* hist = scratch.take<u32>(256 * nblocks);
   u32* offs = scratch.take<u32>(256 * nblocks);
   for (const DigitPass& p : passes) {
@@ -896,12 +1115,13 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
     check_launch("radix_hist");
     exclusive_scan_u32(hist, offs, 256 * nblocks, nullptr, scratch, stream);
     ProfileScope prof(prof_name, stream);
+    const LookBack none{};
     if (KW == 1) {
-      radix_scatter_kernel<1><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
-          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs);
+      radix_scatter_kernel<1, false><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs, none);
     } else if (KW == 2) {
-      radix_scatter_kernel<2><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
-          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs);
+      radix_scatter_kernel<2, false><<<grid_for(nblocks, 1), kBlock, 0, stream>>>(
+          keys_a, idx_a, keys_b, idx_b, n, p.word, p.shift, nblocks, offs, none);
     } else {
       fail_runtime("radix sort records hold one or two key words");
     }
@@ -911,20 +1131,53 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
   }
 }
 
+// Head flags of sorted one-word keys (flags[i] = 1 where a run of equal keys
+// starts); true when every key is distinct.
+bool key_heads(const u64* keys, std::size_t T, u32* flags, Scratch& scratch, hipStream_t stream) {
+  u32* ties = scratch.take<u32>(1);
+  clear_count_kernel<<<1, 1, 0, stream>>>(ties);
+  check_launch("clear_count");
+  key_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys, T, flags, ties);
+  check_launch("key_heads");
+  u32 tied = 0;
+  check_hip(hipMemcpyAsync(&tied, ties, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+  return tied == 0;
+}
+
+// Segment starts of the sorted order from its head flags (seg_start[s], s <=
+// S; seg_start[S] = T); returns S and leaves d_count[1] cleared for the
+// emit's survivor count.
+u32 segment_table(const u32* flags, std::size_t T, u32* seg_start, u32* d_count, Scratch& scratch,
+                  hipStream_t stream) {
+  u32* seg_pos = scratch.take<u32>(T);
+  exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
+  segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
+  check_launch("segment_starts");
+  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+  check_launch("clear_count");
+  u32 S = 0;
+  check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
+            "copy device buffer to host");
+  sync(stream);
+  return S;
+}
+
 // Segment heads of the full key order (flags[i] = 1 where a run of equal keys
 // starts), given the items sorted stably by key word 0 (k0, idx): the items
 // whose word-0 values tie are re-sorted by (run rank, next word), word by word,
 // and written back into their positions, so equal keys end up adjacent in
 // construction order. src reads the KW key words of a term; ors / ands hold
 // each word's bitwise OR / AND over all terms (their XOR: the varying bits).
+// Returns true (and skips the refinement) when every word-0 key is distinct.
 template <class Src>
-void word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
+bool word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
                       const std::vector<unsigned long long>& ors,
                       const std::vector<unsigned long long>& ands, u32* flags, Scratch& scratch,
                       hipStream_t stream) {
   u32* head = flags;
-  run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, T, 1, head);
-  check_launch("run_heads");
+  if (key_heads(k0, T, flags, scratch, stream)) return true;
   const u32* list = nullptr;  // current list of positions (null: all of [0, T))
   std::size_t n_cur = T;
   u32* tied = scratch.take<u32>(T);
@@ -970,6 +1223,7 @@ void word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
   full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, src, static_cast<int>(KW), T,
                                                         flags);
   check_launch("full_heads");
+  return false;
 }
 
 }  // namespace
@@ -1047,6 +1301,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   // Segments of equal keys in sorted order, as positions into idx; the
   // word-0 keys (one-word keys: the whole key) in sorted order.
   const bool packed = W == 0 || (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0);
+  bool distinct;
   if (packed) {
     // One-word keys: x << 32 | z (or 0 for zero-qubit operators).
     const u64 kor = W ? (ors[0] << 32) | ors[1] : 0;
@@ -1054,8 +1309,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     build_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, 1, keys_a, idx_a);
     check_launch("build_records");
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
-    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, 1, flags);
-    check_launch("run_heads");
+    distinct = key_heads(keys_a, T, flags, scratch, stream);
   } else {
     // Word 0 (x0) for every item, then the tied items word by word.
     if (!fused) {
@@ -1064,21 +1318,21 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     }
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
-    word_major_heads(ArrayTerms{x, z, W}, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch,
-                     stream);
+    distinct = word_major_heads(ArrayTerms{x, z, W}, 2 * W, T, keys_a, idx_a, ors, ands, flags,
+                                scratch, stream);
   }
 
-  // Segments.
-  u32* seg_pos = scratch.take<u32>(T);
-  u32* seg_start = scratch.take<u32>(T + 1);
+  // Segments (each key its own when every key is distinct: no table).
   u32* d_count = scratch.take<u32>(2);
-  exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
-  segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
-  check_launch("segment_starts");
-  u32 S = 0;
-  check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
-            "copy device buffer to host");
-  sync(stream);
+  u32* seg_start = nullptr;
+  u32 S = static_cast<u32>(T);
+  if (distinct) {
+    clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+    check_launch("clear_count");
+  } else {
+    seg_start = scratch.take<u32>(T + 1);
+    S = segment_table(flags, T, seg_start, d_count, scratch, stream);
+  }
 
   u32* keep = scratch.take<u32>(S);
   u32* out_pos = scratch.take<u32>(S);
@@ -1252,31 +1506,29 @@ wg_pauli_sum* simplify_product_keys(const ProductTerms& P, std::size_t n, std::s
     hash_seed = seed_mode != 0;
   }
 
+  bool distinct;
   if (packed) {
     const u64 kor = (ors[0] << 32) | ors[1];
     const u64 kand = (ands[0] << 32) | ands[1];
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
-    run_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys_a, T, 1, flags);
-    check_launch("run_heads");
+    distinct = key_heads(keys_a, T, flags, scratch, stream);
   } else {
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
-    word_major_heads(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch, stream);
+    distinct = word_major_heads(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch, stream);
   }
 
-  // Segments.
-  u32* seg_pos = scratch.take<u32>(T);
-  u32* seg_start = scratch.take<u32>(T + 1);
+  // Segments (each key its own when every key is distinct: no table).
   u32* d_count = scratch.take<u32>(2);
-  exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
-  segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
-  check_launch("segment_starts");
-  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
-  check_launch("clear_count");
-  u32 S = 0;
-  check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
-            "copy device buffer to host");
-  sync(stream);
+  u32* seg_start = nullptr;
+  u32 S = static_cast<u32>(T);
+  if (distinct) {
+    clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+    check_launch("clear_count");
+  } else {
+    seg_start = scratch.take<u32>(T + 1);
+    S = segment_table(flags, T, seg_start, d_count, scratch, stream);
+  }
 
   // Sums and output at each segment's position; compacted only when some
   // segment is dropped.
