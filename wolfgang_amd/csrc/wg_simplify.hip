@@ -402,39 +402,164 @@ __global__ __launch_bounds__(256) void clear_u32_kernel(u32* __restrict__ p, std
 
 // ---------------------------------------------------------------- segments
 
-// Bounds of segment s in sorted order. seg_start == nullptr means every key
-// is distinct: segment s is position s alone.
-__device__ inline void segment_bounds(const u32* __restrict__ seg_start, std::size_t s, u32& b,
-                                      u32& e) {
-  if (seg_start) {
-    b = seg_start[s];
-    e = seg_start[s + 1];
-  } else {
-    b = static_cast<u32>(s);
-    e = static_cast<u32>(s + 1);
+// Where the segments of equal keys lie in sorted order. With `start` the
+// table: segment s is [start[s], start[s + 1]). Without it, the keys of the
+// sorted order that repeat their predecessor are the m positions `nonheads`
+// (ascending; m = 0: every key distinct): the head of segment s is the
+// position s + c, c the number of non-heads before it, i.e. the number of j
+// with nonheads[j] - j <= s, and its run goes on through the non-heads right
+// after it.
+struct SegMap {
+  const u32* start;
+  const u32* nonheads;
+  u32 m;
+};
+
+__device__ inline void segment_bounds(const SegMap& map, std::size_t s, u32& b, u32& e) {
+  if (map.start) {
+    b = map.start[s];
+    e = map.start[s + 1];
+    return;
+  }
+  u32 lo = 0, hi = map.m;  // first j with nonheads[j] - j > s
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (map.nonheads[mid] - mid <= s) lo = mid + 1; else hi = mid;
+  }
+  b = static_cast<u32>(s) + lo;
+  e = b + 1;
+  for (u32 j = lo; j < map.m && map.nonheads[j] == e; ++j) ++e;
+}
+
+// Contiguous ranges of sorted one-word keys, one per block: head flags
+// (head[i] = 1 where a run of equal keys starts), and per block the number of
+// non-heads and of tied items (items whose run has two or more), for an
+// ordered list of the tied items (tied_list_kernel).
+__device__ inline bool key_tied(const u64* __restrict__ keys, std::size_t n, std::size_t i) {
+  const u64 k = keys[i];
+  return (i > 0 && keys[i - 1] == k) || (i + 1 < n && keys[i + 1] == k);
+}
+
+__global__ __launch_bounds__(256) void key_heads_kernel(const u64* __restrict__ keys,
+                                                        std::size_t n, std::size_t chunk,
+                                                        u32* __restrict__ head,
+                                                        u32* __restrict__ block_nonheads,
+                                                        u32* __restrict__ block_tied) {
+  __shared__ u32 lds[2][4];
+  u32 nonheads = 0, tied = 0;
+  const std::size_t begin = static_cast<std::size_t>(blockIdx.x) * chunk;
+  const std::size_t end = std::min(n, begin + chunk);
+  for (std::size_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    const u64 k = keys[i];
+    const bool h = i == 0 || keys[i - 1] != k;
+    head[i] = h ? 1U : 0U;
+    nonheads += h ? 0U : 1U;
+    tied += (!h || (i + 1 < n && keys[i + 1] == k)) ? 1U : 0U;
+  }
+  for (int d = 32; d > 0; d >>= 1) {
+    nonheads += static_cast<u32>(__shfl_xor(static_cast<int>(nonheads), d, 64));
+    tied += static_cast<u32>(__shfl_xor(static_cast<int>(tied), d, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    lds[0][threadIdx.x >> 6] = nonheads;
+    lds[1][threadIdx.x >> 6] = tied;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    block_nonheads[blockIdx.x] = lds[0][0] + lds[0][1] + lds[0][2] + lds[0][3];
+    block_tied[blockIdx.x] = lds[1][0] + lds[1][1] + lds[1][2] + lds[1][3];
   }
 }
 
-// head[i] = 1 where a run of equal one-word keys starts; *ties gains the
-// number of non-heads (one atomic per block that has any, so distinct keys
-// cost no atomics).
-__global__ __launch_bounds__(256) void key_heads_kernel(const u64* __restrict__ keys,
-                                                        std::size_t n, u32* __restrict__ head,
-                                                        u32* __restrict__ ties) {
-  __shared__ u32 lds[4];
-  u32 tied = 0;
-  for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    const bool h = i == 0 || keys[i] != keys[i - 1];
-    head[i] = h ? 1U : 0U;
-    tied += h ? 0U : 1U;
+// The tied items' positions in ascending order: block b writes its range's
+// tied positions from offset[b] on, in order (ballots within each 256-item
+// step, the waves' counts through LDS).
+__global__ __launch_bounds__(256) void tied_list_kernel(const u64* __restrict__ keys,
+                                                        std::size_t n, std::size_t chunk,
+                                                        const u32* __restrict__ block_tied,
+                                                        const u32* __restrict__ offset,
+                                                        u32* __restrict__ list) {
+  __shared__ u32 wcount[4];
+  if (block_tied[blockIdx.x] == 0) return;  // (uniform: the whole block leaves)
+  const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+  const u64 lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  u32 out = offset[blockIdx.x];
+  const std::size_t begin = static_cast<std::size_t>(blockIdx.x) * chunk;
+  const std::size_t end = std::min(n, begin + chunk);
+  for (std::size_t base = begin; base < end; base += blockDim.x) {
+    const std::size_t i = base + threadIdx.x;
+    const bool t = i < end && key_tied(keys, n, i);
+    const u64 ball = __ballot(t);
+    if (lane == 0) wcount[wave] = static_cast<u32>(__popcll(ball));
+    __syncthreads();
+    u32 before = 0;
+    for (u32 w = 0; w < wave; ++w) before += wcount[w];
+    if (t) list[out + before + static_cast<u32>(__popcll(ball & lt))] = static_cast<u32>(i);
+    out += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
   }
-  for (int d = 32; d > 0; d >>= 1) tied += static_cast<u32>(__shfl_xor(static_cast<int>(tied), d, 64));
-  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = tied;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const u32 total = lds[0] + lds[1] + lds[2] + lds[3];
-    if (total) atomicAdd(ties, total);
+}
+
+// First refinement level from the list of tied items: records [run rank, key
+// word w] and the items' term indices (run rank = heads before the item in
+// the list - 1; runs are whole in the list).
+template <class Src>
+__global__ __launch_bounds__(256) void list_records_kernel(
+    const u32* __restrict__ list, std::size_t A, const u32* __restrict__ head,
+    const u32* __restrict__ lhead_scan, const u32* __restrict__ idx, Src src, int w,
+    u64* __restrict__ keys2, u32* __restrict__ idx2) {
+  for (std::size_t j = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < A;
+       j += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const u32 pos = list[j];
+    const u32 t = idx[pos];
+    keys2[2 * j] = lhead_scan[j] + head[pos] - 1U;
+    keys2[2 * j + 1] = src.word(t, w);
+    idx2[j] = t;
+  }
+}
+
+// Head flags of the listed items: list_head[j] = head[list[j]].
+__global__ __launch_bounds__(256) void gather_heads_kernel(const u32* __restrict__ list,
+                                                           std::size_t A,
+                                                           const u32* __restrict__ head,
+                                                           u32* __restrict__ out) {
+  for (std::size_t j = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < A;
+       j += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    out[j] = head[list[j]];
+  }
+}
+
+// Full-key head flags of the listed (tied) items, after the refinement; the
+// other positions' flags (heads) are final already. notes[j] = 1 when the
+// item repeats its predecessor.
+template <class Src>
+__global__ __launch_bounds__(256) void list_full_heads_kernel(
+    const u32* __restrict__ list, std::size_t A, const u64* __restrict__ k0,
+    const u32* __restrict__ idx, Src src, int KW, u32* __restrict__ flags,
+    u32* __restrict__ repeats) {
+  for (std::size_t j = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < A;
+       j += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    const u32 i = list[j];
+    bool h = i == 0 || k0[i] != k0[i - 1];
+    if (!h) {
+      const u32 a = idx[i], b = idx[i - 1];
+      for (int w = 1; w < KW && !h; ++w) h = src.word(a, w) != src.word(b, w);
+    }
+    flags[i] = h ? 1U : 0U;
+    repeats[j] = h ? 0U : 1U;
+  }
+}
+
+// nonheads[pos[j]] = list[j] for the listed items that repeat their
+// predecessor (pos: exclusive scan of repeats).
+__global__ __launch_bounds__(256) void compact_list_kernel(const u32* __restrict__ list,
+                                                           std::size_t A,
+                                                           const u32* __restrict__ repeats,
+                                                           const u32* __restrict__ pos,
+                                                           u32* __restrict__ nonheads) {
+  for (std::size_t j = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < A;
+       j += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    if (repeats[j]) nonheads[pos[j]] = list[j];
   }
 }
 
@@ -450,13 +575,13 @@ __global__ __launch_bounds__(256) void segment_starts_kernel(const u32* __restri
 }
 
 __global__ __launch_bounds__(256) void segment_sum_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx,
+    SegMap map, std::size_t S, const u32* __restrict__ idx,
     const double2* __restrict__ c, int zero_seed, double threshold, double2* __restrict__ sums,
     u32* __restrict__ keep) {
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     u32 b, e;
-    segment_bounds(seg_start, s, b, e);
+    segment_bounds(map, s, b, e);
     double2 acc = c[idx[b]];
     for (u32 j = b + 1; j < e; ++j) {
       const double2 v = c[idx[j]];
@@ -473,7 +598,7 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(
 }
 
 __global__ __launch_bounds__(256) void compact_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    SegMap map, std::size_t S, const u32* __restrict__ keep,
     const u32* __restrict__ out_pos, const u64* __restrict__ keys, int KW, std::size_t W,
     const double2* __restrict__ sums, u64* __restrict__ ox, u64* __restrict__ oz,
     double2* __restrict__ oc) {
@@ -482,7 +607,7 @@ __global__ __launch_bounds__(256) void compact_kernel(
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
     u32 kb, ke;
-    segment_bounds(seg_start, s, kb, ke);
+    segment_bounds(map, s, kb, ke);
     const std::size_t k = kb;
     if (W == 0) {
       // Zero-qubit operators have no mask words (their buffers are null).
@@ -606,7 +731,7 @@ __global__ __launch_bounds__(256) void full_heads_kernel(const u64* __restrict__
 }
 
 __global__ __launch_bounds__(256) void compact_gather_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    SegMap map, std::size_t S, const u32* __restrict__ keep,
     const u32* __restrict__ out_pos, const u32* __restrict__ idx, const u64* __restrict__ x,
     const u64* __restrict__ z, std::size_t W, const double2* __restrict__ sums,
     u64* __restrict__ ox, u64* __restrict__ oz, double2* __restrict__ oc) {
@@ -615,7 +740,7 @@ __global__ __launch_bounds__(256) void compact_gather_kernel(
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
     u32 b, e;
-    segment_bounds(seg_start, s, b, e);
+    segment_bounds(map, s, b, e);
     const std::size_t t = idx[b];
     for (std::size_t w = 0; w < W; ++w) {
       ox[o * W + w] = x[t * W + w];
@@ -733,7 +858,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(
 // the output is final; otherwise the survivors are compacted afterwards.
 template <int W>
 __global__ __launch_bounds__(256) void segment_emit_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx,
+    SegMap map, std::size_t S, const u32* __restrict__ idx,
     const u64* __restrict__ rec, int zero_seed, double threshold, u64* __restrict__ ox,
     u64* __restrict__ oz, double2* __restrict__ oc, u32* __restrict__ keep,
     u32* __restrict__ survivors) {
@@ -744,7 +869,7 @@ __global__ __launch_bounds__(256) void segment_emit_kernel(
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     u32 b, e;
-    segment_bounds(seg_start, s, b, e);
+    segment_bounds(map, s, b, e);
     const u64* head = rec + static_cast<std::size_t>(idx[b]) * RW;
     u64 r[2 * W + 2];
 #pragma unroll
@@ -806,7 +931,7 @@ __global__ __launch_bounds__(256) void compact_emitted_kernel(
 
 template <int W>
 __global__ __launch_bounds__(256) void compact_records_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ keep,
+    SegMap map, std::size_t S, const u32* __restrict__ keep,
     const u32* __restrict__ out_pos, const u32* __restrict__ idx, const u64* __restrict__ rec,
     const double2* __restrict__ sums, u64* __restrict__ ox, u64* __restrict__ oz,
     double2* __restrict__ oc) {
@@ -816,7 +941,7 @@ __global__ __launch_bounds__(256) void compact_records_kernel(
     if (!keep[s]) continue;
     const std::size_t o = out_pos[s];
     u32 b, e;
-    segment_bounds(seg_start, s, b, e);
+    segment_bounds(map, s, b, e);
     const std::size_t t = idx[b];
     u64 r[RW];
 #pragma unroll
@@ -922,7 +1047,7 @@ __global__ __launch_bounds__(256) void product_sample_kernel(ProductTerms P, u32
 // are counted (one atomic per block).
 template <int W>
 __global__ __launch_bounds__(256) void product_emit_kernel(
-    const u32* __restrict__ seg_start, std::size_t S, const u32* __restrict__ idx, ProductTerms P,
+    SegMap map, std::size_t S, const u32* __restrict__ idx, ProductTerms P,
     int zero_seed, double threshold, u64* __restrict__ ox, u64* __restrict__ oz,
     double2* __restrict__ oc, u32* __restrict__ keep, u32* __restrict__ survivors) {
   __shared__ u32 lds[4];
@@ -930,7 +1055,7 @@ __global__ __launch_bounds__(256) void product_emit_kernel(
   for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
        s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
     u32 b, e;
-    segment_bounds(seg_start, s, b, e);
+    segment_bounds(map, s, b, e);
     u64 mx[W], mz[W];
     double2 acc = P.term<W>(idx[b], mx, mz);
 #pragma unroll
@@ -1132,31 +1257,48 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
 }
 
 // Head flags of sorted one-word keys (flags[i] = 1 where a run of equal keys
-// starts); true when every key is distinct.
-bool key_heads(const u64* keys, std::size_t T, u32* flags, Scratch& scratch, hipStream_t stream) {
-  u32* ties = scratch.take<u32>(1);
-  clear_count_kernel<<<1, 1, 0, stream>>>(ties);
-  check_launch("clear_count");
-  key_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(keys, T, flags, ties);
+// starts) and the counts behind them; each block takes a contiguous range.
+struct HeadInfo {
+  u32 nonheads = 0;  // items equal to their predecessor
+  u32 tied = 0;      // items whose run has two or more
+  unsigned grid = 0;
+  std::size_t chunk = 0;
+  u32* block_tied = nullptr;   // per block: its tied items
+  u32* tied_offset = nullptr;  // per block: its first slot in the ordered tied list
+};
+
+HeadInfo key_heads(const u64* keys, std::size_t T, u32* flags, Scratch& scratch,
+                   hipStream_t stream) {
+  HeadInfo h;
+  h.grid = static_cast<unsigned>(std::min<std::size_t>((T + 255) / 256, 2048));
+  h.chunk = ((T + h.grid - 1) / h.grid + 255) / 256 * 256;
+  u32* nonheads = scratch.take<u32>(h.grid);
+  u32* tied = scratch.take<u32>(h.grid);
+  h.block_tied = tied;
+  u32* scanned = scratch.take<u32>(h.grid);
+  u32* totals = scratch.take<u32>(2);
+  h.tied_offset = scratch.take<u32>(h.grid);
+  key_heads_kernel<<<h.grid, 256, 0, stream>>>(keys, T, h.chunk, flags, nonheads, tied);
   check_launch("key_heads");
-  u32 tied = 0;
-  check_hip(hipMemcpyAsync(&tied, ties, sizeof(u32), hipMemcpyDeviceToHost, stream),
+  exclusive_scan_u32(nonheads, scanned, h.grid, totals, scratch, stream);
+  exclusive_scan_u32(tied, h.tied_offset, h.grid, totals + 1, scratch, stream);
+  u32 host[2] = {0, 0};
+  check_hip(hipMemcpyAsync(host, totals, sizeof(host), hipMemcpyDeviceToHost, stream),
             "copy device buffer to host");
   sync(stream);
-  return tied == 0;
+  h.nonheads = host[0];
+  h.tied = host[1];
+  return h;
 }
 
 // Segment starts of the sorted order from its head flags (seg_start[s], s <=
-// S; seg_start[S] = T); returns S and leaves d_count[1] cleared for the
-// emit's survivor count.
+// S; seg_start[S] = T); returns S.
 u32 segment_table(const u32* flags, std::size_t T, u32* seg_start, u32* d_count, Scratch& scratch,
                   hipStream_t stream) {
   u32* seg_pos = scratch.take<u32>(T);
   exclusive_scan_u32(flags, seg_pos, T, d_count, scratch, stream);
   segment_starts_kernel<<<stride_grid(T), 256, 0, stream>>>(flags, seg_pos, T, seg_start);
   check_launch("segment_starts");
-  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
-  check_launch("clear_count");
   u32 S = 0;
   check_hip(hipMemcpyAsync(&S, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
             "copy device buffer to host");
@@ -1164,27 +1306,22 @@ u32 segment_table(const u32* flags, std::size_t T, u32* seg_start, u32* d_count,
   return S;
 }
 
-// Segment heads of the full key order (flags[i] = 1 where a run of equal keys
-// starts), given the items sorted stably by key word 0 (k0, idx): the items
-// whose word-0 values tie are re-sorted by (run rank, next word), word by word,
-// and written back into their positions, so equal keys end up adjacent in
-// construction order. src reads the KW key words of a term; ors / ands hold
-// each word's bitwise OR / AND over all terms (their XOR: the varying bits).
-// Returns true (and skips the refinement) when every word-0 key is distinct.
+// Refinement levels w_begin .. KW - 1 over the tied items of the current list
+// (list == null: all of [0, n_cur)) whose run heads are `head`: the items
+// whose previous words tie are re-sorted by (run rank, word w) and written
+// back into their positions of idx, word by word.
 template <class Src>
-bool word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
-                      const std::vector<unsigned long long>& ors,
-                      const std::vector<unsigned long long>& ands, u32* flags, Scratch& scratch,
-                      hipStream_t stream) {
-  u32* head = flags;
-  if (key_heads(k0, T, flags, scratch, stream)) return true;
-  const u32* list = nullptr;  // current list of positions (null: all of [0, T))
-  std::size_t n_cur = T;
-  u32* tied = scratch.take<u32>(T);
-  u32* tpos = scratch.take<u32>(T);
-  u32* hscan = scratch.take<u32>(T);
+void refine_levels(Src src, std::size_t w_begin, std::size_t KW, const u32* list,
+                   std::size_t n_cur, u32* head, u32* idx,
+                   const std::vector<unsigned long long>& ors,
+                   const std::vector<unsigned long long>& ands, Scratch& scratch,
+                   hipStream_t stream) {
+  if (w_begin >= KW || n_cur == 0) return;
+  u32* tied = scratch.take<u32>(n_cur);
+  u32* tpos = scratch.take<u32>(n_cur);
+  u32* hscan = scratch.take<u32>(n_cur);
   u32* d_tied = scratch.take<u32>(1);
-  for (std::size_t w = 1; w < KW; ++w) {
+  for (std::size_t w = w_begin; w < KW; ++w) {
     tied_kernel<<<stride_grid(n_cur), 256, 0, stream>>>(head, n_cur, tied);
     check_launch("tied");
     exclusive_scan_u32(tied, tpos, n_cur, d_tied, scratch, stream);
@@ -1220,10 +1357,94 @@ bool word_major_heads(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
     list = plist;
     n_cur = A;
   }
-  full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, src, static_cast<int>(KW), T,
-                                                        flags);
-  check_launch("full_heads");
-  return false;
+}
+
+// At most this many tied items take the list path of sorted_segments: its
+// segment map is searched per output term (log2 of the non-head count steps).
+constexpr u32 kTiedListMax = 8192;
+
+// Segments of equal keys, given the items sorted stably by key word 0 (k0,
+// idx; KW = 1 when k0 is the whole key): the items whose word-0 values tie are
+// re-sorted word by word (refine_levels), so equal keys end up adjacent in
+// construction order, and flags[i] = 1 where a segment starts. src reads the
+// KW key words of a term; ors / ands hold each word's bitwise OR / AND over all
+// terms (their XOR: the varying bits). Three outcomes, by the number of ties:
+//  * none: every key its own segment (no map, S = T);
+//  * a few (at most kTiedListMax tied items): the ordered list of tied items
+//    is refined alone, and the map holds the positions that repeat their
+//    predecessor;
+//  * more: every level and the head flags run over all positions, and the
+//    map is the segment start table.
+// *S receives the segment count.
+template <class Src>
+SegMap sorted_segments(Src src, std::size_t KW, std::size_t T, u64* k0, u32* idx,
+                       const std::vector<unsigned long long>& ors,
+                       const std::vector<unsigned long long>& ands, u32* flags, u32* d_count,
+                       u32* S, Scratch& scratch, hipStream_t stream) {
+  const HeadInfo h = key_heads(k0, T, flags, scratch, stream);
+  if (h.nonheads == 0) {
+    *S = static_cast<u32>(T);
+    return SegMap{nullptr, nullptr, 0};
+  }
+  if (h.tied <= kTiedListMax) {
+    const std::size_t A = h.tied;
+    u32* plist = scratch.take<u32>(A);
+    tied_list_kernel<<<h.grid, 256, 0, stream>>>(k0, T, h.chunk, h.block_tied, h.tied_offset,
+                                                 plist);
+    check_launch("tied_list");
+    if (KW > 1) {
+      // Level 1 over the listed items, then the later levels as usual.
+      u32* lhead = scratch.take<u32>(A);
+      u32* lscan = scratch.take<u32>(A);
+      gather_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, A, flags, lhead);
+      check_launch("gather_heads");
+      exclusive_scan_u32(lhead, lscan, A, nullptr, scratch, stream);
+      u64* r_keys_a = scratch.take<u64>(2 * A);
+      u64* r_keys_b = scratch.take<u64>(2 * A);
+      u32* r_idx_a = scratch.take<u32>(A);
+      u32* r_idx_b = scratch.take<u32>(A);
+      list_records_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, A, flags, lscan, idx, src, 1,
+                                                             r_keys_a, r_idx_a);
+      check_launch("list_records");
+      int run_bits = 0;
+      while (run_bits < 64 && (u64{1} << run_bits) < A) ++run_bits;
+      const u64 run_varying = run_bits >= 64 ? ~0ULL : ((u64{1} << run_bits) - 1);
+      radix_sort(r_keys_a, r_idx_a, r_keys_b, r_idx_b, A, 2,
+                 digit_passes({run_varying, ors[1] ^ ands[1]}), scratch, stream, "radix_refine");
+      refine_scatter_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, r_idx_a, A, idx);
+      check_launch("refine_scatter");
+      u32* head2 = scratch.take<u32>(A);
+      run_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(r_keys_a, A, 2, head2);
+      check_launch("run_heads");
+      refine_levels(src, 2, KW, plist, A, head2, idx, ors, ands, scratch, stream);
+    }
+    u32* repeats = scratch.take<u32>(A);
+    u32* rpos = scratch.take<u32>(A);
+    list_full_heads_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, A, k0, idx, src,
+                                                               static_cast<int>(KW), flags, repeats);
+    check_launch("list_full_heads");
+    exclusive_scan_u32(repeats, rpos, A, d_count, scratch, stream);
+    u32 m = 0;
+    check_hip(hipMemcpyAsync(&m, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
+              "copy device buffer to host");
+    sync(stream);
+    u32* nonheads = scratch.take<u32>(std::max<u32>(m, 1));
+    if (m) {
+      compact_list_kernel<<<stride_grid(A), 256, 0, stream>>>(plist, A, repeats, rpos, nonheads);
+      check_launch("compact_list");
+    }
+    *S = static_cast<u32>(T - m);
+    return SegMap{nullptr, m ? nonheads : nullptr, m};
+  }
+  refine_levels(src, 1, KW, nullptr, T, flags, idx, ors, ands, scratch, stream);
+  if (KW > 1) {
+    full_heads_kernel<<<stride_grid(T), 256, 0, stream>>>(k0, idx, src, static_cast<int>(KW), T,
+                                                          flags);
+    check_launch("full_heads");
+  }
+  u32* seg_start = scratch.take<u32>(T + 1);
+  *S = segment_table(flags, T, seg_start, d_count, scratch, stream);
+  return SegMap{seg_start, nullptr, 0};
 }
 
 }  // namespace
@@ -1301,7 +1522,9 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
   // Segments of equal keys in sorted order, as positions into idx; the
   // word-0 keys (one-word keys: the whole key) in sorted order.
   const bool packed = W == 0 || (W == 1 && (ors[0] >> 32) == 0 && (ors[1] >> 32) == 0);
-  bool distinct;
+  u32* d_count = scratch.take<u32>(2);
+  u32 S = 0;
+  SegMap map;
   if (packed) {
     // One-word keys: x << 32 | z (or 0 for zero-qubit operators).
     const u64 kor = W ? (ors[0] << 32) | ors[1] : 0;
@@ -1309,7 +1532,8 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     build_records_kernel<<<stride_grid(T), 256, 0, stream>>>(x, z, T, W, 1, keys_a, idx_a);
     check_launch("build_records");
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
-    distinct = key_heads(keys_a, T, flags, scratch, stream);
+    map = sorted_segments(ArrayTerms{x, z, W}, 1, T, keys_a, idx_a, ors, ands, flags, d_count, &S,
+                          scratch, stream);
   } else {
     // Word 0 (x0) for every item, then the tied items word by word.
     if (!fused) {
@@ -1318,21 +1542,11 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     }
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
-    distinct = word_major_heads(ArrayTerms{x, z, W}, 2 * W, T, keys_a, idx_a, ors, ands, flags,
-                                scratch, stream);
+    map = sorted_segments(ArrayTerms{x, z, W}, 2 * W, T, keys_a, idx_a, ors, ands, flags, d_count,
+                          &S, scratch, stream);
   }
-
-  // Segments (each key its own when every key is distinct: no table).
-  u32* d_count = scratch.take<u32>(2);
-  u32* seg_start = nullptr;
-  u32 S = static_cast<u32>(T);
-  if (distinct) {
-    clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
-    check_launch("clear_count");
-  } else {
-    seg_start = scratch.take<u32>(T + 1);
-    S = segment_table(flags, T, seg_start, d_count, scratch, stream);
-  }
+  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+  check_launch("clear_count");
 
   u32* keep = scratch.take<u32>(S);
   u32* out_pos = scratch.take<u32>(S);
@@ -1343,7 +1557,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
     check_launch("clear_count");
     auto emit = [&](auto kernel) {
-      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, rec, hash_seed ? 0 : 1,
+      kernel<<<stride_grid(S), 256, 0, stream>>>(map, S, idx_a, rec, hash_seed ? 0 : 1,
                                                  threshold, staged->x, staged->z, staged->coeffs,
                                                  keep, d_count + 1);
     };
@@ -1365,7 +1579,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     return out.release();
   }
   double2* sums = scratch.take<double2>(S);
-  segment_sum_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, c,
+  segment_sum_kernel<<<stride_grid(S), 256, 0, stream>>>(map, S, idx_a, c,
                                                          hash_seed ? 0 : 1, threshold, sums, keep);
   check_launch("segment_sum");
   exclusive_scan_u32(keep, out_pos, S, d_count + 1, scratch, stream);
@@ -1376,7 +1590,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
 
   std::unique_ptr<wg_pauli_sum> out(new_pauli_sum(n, U, device));
   if (U && packed) {
-    compact_kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, keys_a, 1, W,
+    compact_kernel<<<stride_grid(S), 256, 0, stream>>>(map, S, keep, out_pos, keys_a, 1, W,
                                                        sums, out->x, out->z, out->coeffs);
     check_launch("compact");
   } else if (U && W <= 4) {
@@ -1388,7 +1602,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
       check_launch("mask_records");
     }
     auto gather = [&](auto kernel) {
-      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, keep, out_pos, idx_a, rec, sums,
+      kernel<<<stride_grid(S), 256, 0, stream>>>(map, S, keep, out_pos, idx_a, rec, sums,
                                                  out->x, out->z, out->coeffs);
     };
     // (Two and three words took segment_emit_kernel above.)
@@ -1396,7 +1610,7 @@ wg_pauli_sum* simplify_device(const u64* x, const u64* z, const double2* c, std:
     check_launch("compact");
   } else if (U) {
     compact_gather_kernel<<<stride_grid(S), 256, 0, stream>>>(
-        seg_start, S, keep, out_pos, idx_a, x, z, W, sums, out->x, out->z, out->coeffs);
+        map, S, keep, out_pos, idx_a, x, z, W, sums, out->x, out->z, out->coeffs);
     check_launch("compact");
   }
   sync(stream);
@@ -1506,29 +1720,22 @@ wg_pauli_sum* simplify_product_keys(const ProductTerms& P, std::size_t n, std::s
     hash_seed = seed_mode != 0;
   }
 
-  bool distinct;
+  u32* d_count = scratch.take<u32>(2);
+  u32 S = 0;
+  SegMap map;
   if (packed) {
     const u64 kor = (ors[0] << 32) | ors[1];
     const u64 kand = (ands[0] << 32) | ands[1];
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
-    distinct = key_heads(keys_a, T, flags, scratch, stream);
+    map = sorted_segments(P, 1, T, keys_a, idx_a, ors, ands, flags, d_count, &S, scratch, stream);
   } else {
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
                stream);
-    distinct = word_major_heads(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, scratch, stream);
+    map = sorted_segments(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, d_count, &S, scratch,
+                          stream);
   }
-
-  // Segments (each key its own when every key is distinct: no table).
-  u32* d_count = scratch.take<u32>(2);
-  u32* seg_start = nullptr;
-  u32 S = static_cast<u32>(T);
-  if (distinct) {
-    clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
-    check_launch("clear_count");
-  } else {
-    seg_start = scratch.take<u32>(T + 1);
-    S = segment_table(flags, T, seg_start, d_count, scratch, stream);
-  }
+  clear_count_kernel<<<1, 1, 0, stream>>>(d_count + 1);
+  check_launch("clear_count");
 
   // Sums and output at each segment's position; compacted only when some
   // segment is dropped.
@@ -1537,7 +1744,7 @@ wg_pauli_sum* simplify_product_keys(const ProductTerms& P, std::size_t n, std::s
   {
     ProfileScope prof("product_emit", stream);
     auto emit = [&](auto kernel) {
-      kernel<<<stride_grid(S), 256, 0, stream>>>(seg_start, S, idx_a, P, hash_seed ? 0 : 1,
+      kernel<<<stride_grid(S), 256, 0, stream>>>(map, S, idx_a, P, hash_seed ? 0 : 1,
                                                  threshold, staged->x, staged->z, staged->coeffs,
                                                  keep, d_count + 1);
     };
