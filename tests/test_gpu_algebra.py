@@ -216,6 +216,31 @@ def test_simplify_word_ties_bit_exact(gpu, orc, n, terms):
     assert_bit_exact(op2.to_device().simplify().to_host(), orc.simplify(*packed(op2)))
 
 
+@pytest.mark.parametrize("n", [48, 128, 200])
+@pytest.mark.parametrize("terms", [100_000, 1_200_000])
+def test_simplify_few_ties_bit_exact(gpu, orc, n, terms):
+    # Mostly distinct keys with a handful of ties: exact duplicates (their
+    # coefficients summed in construction order, some cancelling below the
+    # threshold) and keys equal in word 0 only. The tied items are refined
+    # on their own list and the sums find their segments through the
+    # positions that repeat their predecessor; 1.2 M terms also take the
+    # one-sweep radix passes.
+    rng = np.random.default_rng(n + terms)
+    pool = random_pauli_masks(n, terms, 0.25, 31 * n + terms)
+    xs, zs = pool.x_words().copy(), pool.z_words().copy()
+    c = rng.normal(size=terms) + 1j * rng.normal(size=terms)
+    dst = rng.choice(terms, size=120, replace=False)
+    src, dst = dst[:60], dst[60:]
+    xs[dst[:40]] = xs[src[:40]]          # exact duplicates
+    zs[dst[:40]] = zs[src[:40]]
+    c[dst[:10]] = -c[src[:10]]           # ... ten of which cancel
+    xs[dst[40:], 0] = xs[src[40:], 0]    # equal word 0 only
+    if zs.shape[1] == 1:
+        zs[dst[40:], 0] ^= np.uint64(1)
+    op = PauliSum.from_packed(n, xs, zs, c)
+    assert_bit_exact(op.to_device().simplify().to_host(), orc.simplify(*packed(op)))
+
+
 def test_simplify_signed_zero_seed_matches_oracle(gpu, orc):
     for n in (1, 65):
         pool = random_pauli_masks(n, 4, 0.6, 3)
