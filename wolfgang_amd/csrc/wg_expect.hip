@@ -1767,6 +1767,283 @@ void assign_step_classes(ExpItem& e, int min_d) {
   e.cls_log = static_cast<unsigned char>(d);
 }
 
+// ------------------------------------------------------------ owner slots
+
+constexpr u32 kSegMask = (1U << kSegBits) - 1;
+
+// One slot of the owner sweep: the keys of an item whose |x & z| has the given
+// parity (a slot computes one component of the pair product), 2 per slot, or
+// 4 when the parity class holds more than 2 keys ("wide").
+struct OwnerCand {
+  u32 item;  // the item (relative to the pass in plan_owner_bundles)
+  u32 xl;    // x in pass-local coordinates
+  int odd;
+  int ncols;
+  int cols[kOwnerMaxKeys];
+  int wide;
+  int lanex;  // x inside the contiguous low bits: lane-pivot bundle
+};
+
+void owner_cands(const ExpItem& e, u32 item, u32 xl, std::vector<OwnerCand>& out) {
+  for (int odd = 0; odd < 2; ++odd) {
+    int cols[kZ];
+    int nc = 0;
+    for (int j = 0; j < e.nz; ++j) {
+      if ((e.y[j] & 1) == odd) cols[nc++] = j;
+    }
+    const int wide = nc > 2 ? 1 : 0;
+    const int per = wide ? 4 : 2;
+    for (int s = 0; s < nc; s += per) {
+      OwnerCand c{};
+      c.item = item;
+      c.xl = xl;
+      c.odd = odd;
+      c.ncols = std::min(per, nc - s);
+      for (int q = 0; q < c.ncols; ++q) c.cols[q] = cols[s + q];
+      c.wide = wide;
+      c.lanex = ((xl >> kTileLowLog) & kSegMask) == 0 ? 1 : 0;
+      out.push_back(c);
+    }
+  }
+}
+
+// Register-pivot slots of one pool grouped by bundle layout, the functional
+// lam with lam . s = 1 for the segment part s of every member's x: repeatedly
+// the lam covering the most remaining slots (the lowest on ties); while it
+// covers a full bundle only whole bundles are taken (its other slots may fill
+// bundles of other layouts), and the last groups take everything their lam
+// covers. The counts come from a Walsh-Hadamard transform of the pool's
+// histogram of segment parts: #{i : lam . s_i = 1} = (k - W(lam)) / 2.
+struct OwnerGroup {
+  u32 lam;
+  std::vector<u32> members;  // indices into the pool, in pool order
+};
+
+std::vector<OwnerGroup> owner_layout_groups(const std::vector<u32>& segs, std::size_t full) {
+  std::vector<OwnerGroup> out;
+  std::vector<u32> pool(segs.size());
+  for (u32 i = 0; i < pool.size(); ++i) pool[i] = i;
+  std::vector<int> w(std::size_t{1} << kSegBits);
+  while (!pool.empty()) {
+    std::fill(w.begin(), w.end(), 0);
+    for (u32 i : pool) ++w[segs[i] & kSegMask];
+    for (std::size_t len = 1; len < w.size(); len <<= 1) {
+      for (std::size_t b = 0; b < w.size(); b += 2 * len) {
+        for (std::size_t i = b; i < b + len; ++i) {
+          const int u = w[i], v = w[i + len];
+          w[i] = u + v;
+          w[i + len] = u - v;
+        }
+      }
+    }
+    const int k = static_cast<int>(pool.size());
+    u32 best = 1;
+    std::size_t best_n = 0;
+    for (u32 lam = 1; lam <= kSegMask; ++lam) {
+      const std::size_t cnt = static_cast<std::size_t>((k - w[lam]) / 2);
+      if (cnt > best_n) {
+        best_n = cnt;
+        best = lam;
+      }
+    }
+    if (best_n == 0) fail_runtime("owner bundle slot has no register pivot");
+    const std::size_t quota = best_n >= full ? best_n / full * full : best_n;
+    OwnerGroup g{best, {}};
+    std::vector<u32> rest;
+    for (u32 i : pool) {
+      const bool covered = host_parity(best & segs[i]);
+      (covered && g.members.size() < quota ? g.members : rest).push_back(i);
+    }
+    out.push_back(std::move(g));
+    pool = std::move(rest);
+  }
+  return out;
+}
+
+// Shape of each bundle of a group of `size` slots: bundles of the largest
+// shape, then the smallest shape that holds the rest.
+template <class F>
+void owner_group_shapes(std::size_t size, int first, int last, F&& f) {
+  for (std::size_t s = 0; s < size;) {
+    const std::size_t left = size - s;
+    int shape = first;
+    for (int q = last; q >= first; --q) {
+      if (static_cast<std::size_t>(kOwnerShapeSlots[q]) >= left) {
+        shape = q;
+        break;
+      }
+    }
+    f(shape, s);
+    s += static_cast<std::size_t>(kOwnerShapeSlots[shape]);
+  }
+}
+
+// The owner slots of a pass in bundle pools (width, parity, pivot kind) and
+// layout groups: calls f(pool, group lam (0 for lane pivots), member indices
+// into cands, wide, base kind) in the order the bundles are emitted.
+template <class F>
+void owner_pass_groups(const std::vector<OwnerCand>& cands, F&& f) {
+  for (int wide = 0; wide < 2; ++wide) {
+    for (int base_kind = 0; base_kind < 4; ++base_kind) {
+      std::vector<u32> pool;
+      for (u32 i = 0; i < cands.size(); ++i) {
+        const OwnerCand& c = cands[i];
+        if (c.odd == (base_kind & 1) && c.lanex == ((base_kind >> 1) & 1) && c.wide == wide) {
+          pool.push_back(i);
+        }
+      }
+      if (pool.empty()) continue;
+      if ((base_kind >> 1) & 1) {
+        f(0U, pool, wide, base_kind);
+        continue;
+      }
+      std::vector<u32> segs(pool.size());
+      for (std::size_t i = 0; i < pool.size(); ++i) {
+        segs[i] = (cands[pool[i]].xl >> kTileLowLog) & kSegMask;
+      }
+      const std::size_t full =
+          static_cast<std::size_t>(kOwnerShapeSlots[owner_first_shape(wide != 0, false)]);
+      for (const OwnerGroup& g : owner_layout_groups(segs, full)) {
+        std::vector<u32> members(g.members.size());
+        for (std::size_t i = 0; i < members.size(); ++i) members[i] = pool[g.members[i]];
+        f(g.lam, members, wide, base_kind);
+      }
+    }
+  }
+}
+
+// Bundles the owner sweep needs for these slots, and the sum over layout
+// groups of (slots in the group's partial bundle)^2, which grows as partial
+// bundles concentrate.
+struct OwnerCost {
+  std::size_t bundles = 0;
+  std::size_t spread = 0;
+};
+
+OwnerCost owner_cost(const std::vector<OwnerCand>& cands) {
+  OwnerCost c;
+  owner_pass_groups(cands, [&](u32, const std::vector<u32>& members, int wide, int base_kind) {
+    const bool lanex = (base_kind >> 1) & 1;
+    const int first = owner_first_shape(wide != 0, lanex);
+    owner_group_shapes(members.size(), first, owner_last_shape(wide != 0, lanex),
+                       [&](int, std::size_t) { ++c.bundles; });
+    const std::size_t r = members.size() % static_cast<std::size_t>(kOwnerShapeSlots[first]);
+    c.spread += r * r;
+  });
+  return c;
+}
+
+// Segment part (pass-local bits kTileLowLog.. of x) of x in U.
+u32 span_seg(const Span& sp, u64 x, int l) {
+  u32 sel = 0;
+  const u64 r = sp.reduce(x, &sel);
+  u32 out = 0;
+  int a = 0;
+  for (int q = l; q < 64; ++q) {
+    if (!((sp.coord >> q) & 1ULL)) continue;
+    if ((r >> q) & 1ULL) out |= 1U << a;
+    ++a;
+  }
+  return out | (sel << a);
+}
+
+// Moves x-groups between the passes whose subspace holds them so that the
+// owner sweep needs fewer bundles (each bundle reads every tile of its pass):
+// local search over single moves, taking a move when it lowers the bundle
+// count, or keeps it and concentrates the partial bundles. Passes left
+// without an x-group are dropped (pass 0 stays: it carries the diagonal
+// items). Items keep their input order within a pass.
+void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>& xs,
+                       std::vector<Span>& spans, std::vector<std::vector<u64>>& pass_xs, int l) {
+  const std::size_t P = spans.size();
+  if (P < 2) return;
+  std::unordered_map<u64, u32> group_of;
+  for (u32 g = 0; g < xs.size(); ++g) group_of.emplace(xs[g], g);
+  std::vector<int> at(xs.size(), -1);
+  for (std::size_t p = 0; p < P; ++p) {
+    for (u64 x : pass_xs[p]) at[group_of.at(x)] = static_cast<int>(p);
+  }
+  // Off-diagonal items in input order with their x-group; each x-group's
+  // segment part in every pass that holds it.
+  std::vector<u32> item_idx, item_group;
+  for (u32 i = 0; i < items.size(); ++i) {
+    if (items[i].x == 0) continue;
+    item_idx.push_back(i);
+    item_group.push_back(group_of.at(items[i].x));
+  }
+  std::vector<std::vector<std::pair<int, u32>>> cover(xs.size());
+  for (u32 g = 0; g < xs.size(); ++g) {
+    for (std::size_t p = 0; p < P; ++p) {
+      if (spans[p].contains(xs[g])) {
+        cover[g].push_back({static_cast<int>(p), span_seg(spans[p], xs[g], l) << kTileLowLog});
+      }
+    }
+  }
+  auto seg_in = [&](u32 g, int p) -> u32 {
+    for (const auto& [q, xl] : cover[g]) {
+      if (q == p) return xl;
+    }
+    fail_runtime("owner pass packing lost an x-group");
+  };
+  // Cost of pass p with x-group `add` moved in and `drop` moved out.
+  auto cost = [&](int p, u32 add, u32 drop) {
+    std::vector<OwnerCand> cands;
+    for (std::size_t k = 0; k < item_idx.size(); ++k) {
+      const u32 g = item_group[k];
+      if (g == drop || (at[g] != p && g != add)) continue;
+      owner_cands(items[item_idx[k]], 0, seg_in(g, p), cands);
+    }
+    return owner_cost(cands);
+  };
+  constexpr u32 kNone = 0xFFFFFFFFU;
+  std::vector<OwnerCost> pc(P);
+  for (std::size_t p = 0; p < P; ++p) pc[p] = cost(static_cast<int>(p), kNone, kNone);
+  for (int sweep = 0; sweep < 4; ++sweep) {
+    bool moved = false;
+    for (u32 g = 0; g < xs.size(); ++g) {
+      if (cover[g].size() < 2) continue;
+      const int p = at[g];
+      const OwnerCost cp = cost(p, kNone, g);
+      int best_q = -1;
+      long best_db = 0, best_ds = 0;
+      OwnerCost best_cq{};
+      for (const auto& [q, xl] : cover[g]) {
+        if (q == p) continue;
+        const OwnerCost cq = cost(q, g, kNone);
+        const long db = static_cast<long>(cp.bundles + cq.bundles) -
+                        static_cast<long>(pc[p].bundles + pc[q].bundles);
+        const long ds = static_cast<long>(cp.spread + cq.spread) -
+                        static_cast<long>(pc[p].spread + pc[q].spread);
+        if ((db < best_db) || (db == best_db && ds > best_ds)) {
+          best_q = q;
+          best_db = db;
+          best_ds = ds;
+          best_cq = cq;
+        }
+      }
+      if (best_q < 0) continue;
+      at[g] = best_q;
+      pc[p] = cp;
+      pc[best_q] = best_cq;
+      moved = true;
+    }
+    if (!moved) break;
+  }
+  std::vector<std::vector<u64>> out(P);
+  for (u32 g = 0; g < xs.size(); ++g) {
+    if (at[g] >= 0) out[at[g]].push_back(xs[g]);
+  }
+  std::vector<Span> kept_spans;
+  std::vector<std::vector<u64>> kept_xs;
+  for (std::size_t p = 0; p < P; ++p) {
+    if (p > 0 && out[p].empty()) continue;
+    kept_spans.push_back(spans[p]);
+    kept_xs.push_back(std::move(out[p]));
+  }
+  spans = std::move(kept_spans);
+  pass_xs = std::move(kept_xs);
+}
+
 }  // namespace
 
 void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
@@ -1931,6 +2208,8 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
       return;
     }
   }
+
+  if (kOwnerSweep && structured) pack_owner_passes(items, xs, spans, pass_xs, l);
 
   // Emit passes: local coordinates, segment offsets, free positions; items are
   // reordered pass by pass. Diagonal items either lead the item list (swept
@@ -2100,8 +2379,6 @@ struct SegLayout {
   int pi[kSegBits - 1];
 };
 
-constexpr u32 kSegMask = (1U << kSegBits) - 1;
-
 SegLayout seg_layout(u32 lam) {
   SegLayout L{};
   L.lam = lam;
@@ -2158,15 +2435,6 @@ u64 host_compress(u64 v, u64 mask) {
 
 void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunch>& passes,
                         std::vector<OwnerBundle>& bundles) {
-  struct Cand {
-    u32 item;  // relative to the pass
-    u32 xl;
-    int odd;
-    int ncols;
-    int cols[kOwnerMaxKeys];
-    int wide;
-    int lanex;
-  };
   for (PassLaunch& pl : passes) {
     const PassDesc& d = pl.desc;
     // The diagonal items keep their sweep variants; the off-diagonal ones move
@@ -2182,50 +2450,20 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
     pl.subs = keep;
     pl.owner_first = static_cast<u32>(bundles.size());
     pl.owner_count = 0;
-    std::vector<Cand> cands;
+    std::vector<OwnerCand> cands;
     for (u32 r = 0; r < pl.num_items; ++r) {
       const ExpItem& e = items[pl.first_item + r];
-      if (e.x == 0) continue;
-      for (int odd = 0; odd < 2; ++odd) {
-        std::vector<int> cols;
-        for (int j = 0; j < e.nz; ++j) {
-          if ((e.y[j] & 1) == odd) cols.push_back(j);
-        }
-        const int wide = cols.size() > 2 ? 1 : 0;
-        const std::size_t per = wide ? 4 : 2;
-        for (std::size_t s = 0; s < cols.size(); s += per) {
-          Cand c{};
-          c.item = r;
-          c.xl = static_cast<u32>(e.xl);
-          c.odd = odd;
-          c.ncols = static_cast<int>(std::min(per, cols.size() - s));
-          for (int q = 0; q < c.ncols; ++q) c.cols[q] = cols[s + q];
-          c.wide = wide;
-          c.lanex = ((c.xl >> kTileLowLog) & kSegMask) == 0 ? 1 : 0;
-          cands.push_back(c);
-        }
-      }
+      if (e.x != 0) owner_cands(e, r, static_cast<u32>(e.xl), cands);
     }
-    // Bundles of a group: full bundles of the largest shape, then the
-    // smallest shape that holds the rest.
-    auto emit = [&](const std::vector<Cand>& group, const SegLayout& L, int base_kind, bool wide) {
-      const int first_shape = owner_first_shape(wide, (base_kind >> 1) & 1);
-      const int last_shape = owner_last_shape(wide, (base_kind >> 1) & 1);
-      for (std::size_t s = 0; s < group.size();) {
-        const std::size_t left = group.size() - s;
-        int shape = first_shape;
-        for (int q = last_shape; q >= first_shape; --q) {
-          if (static_cast<std::size_t>(kOwnerShapeSlots[q]) >= left) {
-            shape = q;
-            break;
-          }
-        }
+    owner_pass_groups(cands, [&](u32 lam, const std::vector<u32>& group, int wide, int base_kind) {
+      const bool lanex = (base_kind >> 1) & 1;
+      const SegLayout L = seg_layout(lanex ? 0U : lam);
+      owner_group_shapes(group.size(), owner_first_shape(wide != 0, lanex),
+                         owner_last_shape(wide != 0, lanex), [&](int shape, std::size_t s) {
         const int M = kOwnerShapeSlots[shape];
-        const int kind = shape * 4 + base_kind;
-        const bool lanex = (base_kind >> 1) & 1;
         OwnerBundle bd{};
         for (int i = 0; i < kSegBits; ++i) bd.seg_basis[i] = d.seg_off[L.img[i]];
-        bd.kind = static_cast<u32>(kind);
+        bd.kind = static_cast<u32>(shape * 4 + base_kind);
         for (int m = 0; m < M; ++m) {
           OwnerSlot& S = bd.slot[m];
           if (s + m >= group.size()) {
@@ -2233,7 +2471,7 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
             S.addr0 = lanex ? 0U : (4U << 13);
             continue;
           }
-          const Cand& c = group[s + m];
+          const OwnerCand& c = cands[group[s + m]];
           const ExpItem& e = items[pl.first_item + c.item];
           const u32 xl = layout_x(L, c.xl);
           S.item = c.item;
@@ -2259,50 +2497,8 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
         }
         bundles.push_back(bd);
         ++pl.owner_count;
-        s += static_cast<std::size_t>(M);
-      }
-    };
-    for (int wide = 0; wide < 2; ++wide) {
-      for (int base_kind = 0; base_kind < 4; ++base_kind) {
-        std::vector<Cand> pool;
-        for (const Cand& c : cands) {
-          if (c.odd == (base_kind & 1) && c.lanex == ((base_kind >> 1) & 1) && c.wide == wide) {
-            pool.push_back(c);
-          }
-        }
-        if (pool.empty()) continue;
-        if ((base_kind >> 1) & 1) {
-          emit(pool, seg_layout(0), base_kind, wide);
-          continue;
-        }
-        // Greedy: the functional covering the most remaining items, repeatedly.
-        // While it covers a full bundle, only whole bundles are taken (the
-        // rest of its items may fill bundles of other layouts); the final
-        // partial bundles take everything their functional covers.
-        const std::size_t full = static_cast<std::size_t>(
-            kOwnerShapeSlots[owner_first_shape(wide, false)]);
-        while (!pool.empty()) {
-          u32 best = 1;
-          std::size_t best_n = 0;
-          for (u32 lam = 1; lam <= kSegMask; ++lam) {
-            std::size_t cnt = 0;
-            for (const Cand& c : pool) cnt += host_parity(lam & ((c.xl >> kTileLowLog) & kSegMask));
-            if (cnt > best_n) {
-              best_n = cnt;
-              best = lam;
-            }
-          }
-          const std::size_t quota = best_n >= full ? best_n / full * full : best_n;
-          std::vector<Cand> take, rest;
-          for (const Cand& c : pool) {
-            const bool covered = host_parity(best & ((c.xl >> kTileLowLog) & kSegMask));
-            (covered && take.size() < quota ? take : rest).push_back(c);
-          }
-          emit(take, seg_layout(best), base_kind, wide);
-          pool = std::move(rest);
-        }
-      }
-    }
+      });
+    });
     groups += pl.owner_count;
     pl.item_groups = std::max<u32>(1, groups);
     if (pl.owner_count) {
