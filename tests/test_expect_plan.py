@@ -31,6 +31,8 @@ def distinct_keys(op):
 
 @pytest.mark.parametrize("gen,n,terms", [
     ("chemistry", 18, 3000), ("chemistry", 20, 5000), ("random", 18, 300),
+    # 25 qubits and up: the 11-slot owner bundles (partner addresses formed per tile).
+    ("chemistry", 25, 120),
     ("random", 19, 40), ("diag", 18, 100), ("dense", 18, 50),
     # More than 1,024 distinct x-masks: the greedy coset planner.
     ("random", 18, 3000),

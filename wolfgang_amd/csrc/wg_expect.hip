@@ -648,25 +648,30 @@ const std::array<TiledKernel, kVariants> kTiledKernels =
 // shape (slots, keys per slot).
 constexpr int kOwnerBlock = kTileBlock;
 constexpr int kOwnerWaves = kOwnerBlock / 64;
-// Shapes 0..5 are register-pivot bundles (half-tile sweep), 6..9 lane-pivot
+// Shapes 0..6 are register-pivot bundles (half-tile sweep), 7..10 lane-pivot
 // bundles (full-tile sweep: their partners may sit in any register, and they
 // also hold all 8 points and the representatives' selects, so they are smaller).
-constexpr int kOwnerShapes = 10;
-constexpr int kOwnerShapeSlots[kOwnerShapes] = {9, 4, 2, 4, 2, 1, 4, 2, 2, 1};
-constexpr int kOwnerShapeKeys[kOwnerShapes] = {2, 2, 2, 4, 4, 4, 2, 2, 4, 4};
+// The 11-slot shape forms its partner addresses at every tile to fit in 128
+// VGPRs: it pays where each tile read costs most, states beyond the
+// Infinity Cache (kOwnerWideMinQubits and up); smaller states start from the
+// 9-slot shape, which keeps its addresses in registers.
+constexpr int kOwnerShapes = 11;
+constexpr int kOwnerShapeSlots[kOwnerShapes] = {11, 9, 4, 2, 4, 2, 1, 4, 2, 2, 1};
+constexpr int kOwnerShapeKeys[kOwnerShapes] = {2, 2, 2, 2, 4, 4, 4, 2, 2, 4, 4};
 constexpr int kOwnerKinds = 4 * kOwnerShapes;
-constexpr int kOwnerMaxAcc = 18;
+constexpr int kOwnerMaxAcc = 22;
+constexpr int kOwnerWideMinQubits = 25;
 // Shapes available to a group of slots, largest first.
-constexpr int owner_first_shape(bool wide, bool lanex) {
-  return lanex ? (wide ? 8 : 6) : (wide ? 3 : 0);
+constexpr int owner_first_shape(bool wide, bool lanex, bool big) {
+  return lanex ? (wide ? 9 : 7) : (wide ? 4 : (big ? 0 : 1));
 }
 constexpr int owner_last_shape(bool wide, bool lanex) {
-  return lanex ? (wide ? 9 : 7) : (wide ? 5 : 2);
+  return lanex ? (wide ? 10 : 8) : (wide ? 6 : 3);
 }
 constexpr bool owner_kind_used(int kind) {
   const int shape = kind >> 2;
   const bool lanex = (kind >> 1) & 1;
-  return lanex ? shape >= 6 : shape < 6;
+  return lanex ? shape >= 7 : shape < 7;
 }
 // Owner passes give each block about kOwnerTilesPerBlock tiles: the blocks of
 // the bundles that share a tile start together and stay close, so all but the
@@ -679,7 +684,7 @@ constexpr u64 kOwnerMinBlocks = 512;
 constexpr u64 kOwnerMaxPassRows = u64{1} << 20;
 static_assert(kTilePoints / kOwnerBlock == 8 && kOwnerWaves == 8,
               "owner staging is written for 8 points per thread and 8 waves");
-static_assert(kOwnerMaxSlots == 9, "bundle slots match the largest shape");
+static_assert(kOwnerMaxSlots == 11, "bundle slots match the largest shape");
 
 __device__ __forceinline__ double unit_sign(u32 hi) { return __hiloint2double(static_cast<int>(hi), 0); }
 
@@ -866,7 +871,11 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
   const u32 lane_lo = lane & ((1U << kTileLowLog) - 1U);
   const u32 tid16 = tid << 4;
   // Partner of representative j: byte (tid << 4) ^ haddr[m] ^ (j << 13) of the
-  // half-buffer (register j ^ x[9..11] minus 4; x bit 11 is set).
+  // half-buffer (register j ^ x[9..11] minus 4; x bit 11 is set). Above 9
+  // slots the 4 M partner addresses are formed again at every tile (5 VALU per
+  // slot): held across the tile loop they would take 4 M VGPRs, and 11 slots
+  // fit in 128 VGPRs only without them (each slot a bundle holds saves a tile
+  // read).
   u32 haddr[M], kw[M][NZ];
   double acc[M][NZ];
 #pragma unroll
@@ -909,9 +918,13 @@ __device__ __forceinline__ void owner_half_sweep(const double2* __restrict__ psi
       issue(rep, next);
     }
     const u32 t32 = tb | (static_cast<u32>(step ^ (step >> 1)) << log_tb);
+    u32 t16 = tid16;
+    if constexpr (M > 9) {
+      asm volatile("" : "+v"(t16));  // opaque per tile: keeps the addresses out of the loop invariants
+    }
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const u32 base = tid16 ^ haddr[m];
+      const u32 base = t16 ^ haddr[m];
       double2 q[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) q[j] = lds_at(cur, base ^ (static_cast<u32>(j) << 13));
@@ -1882,7 +1895,7 @@ void owner_group_shapes(std::size_t size, int first, int last, F&& f) {
 // layout groups: calls f(pool, group lam (0 for lane pivots), member indices
 // into cands, wide, base kind) in the order the bundles are emitted.
 template <class F>
-void owner_pass_groups(const std::vector<OwnerCand>& cands, F&& f) {
+void owner_pass_groups(const std::vector<OwnerCand>& cands, bool big, F&& f) {
   for (int wide = 0; wide < 2; ++wide) {
     for (int base_kind = 0; base_kind < 4; ++base_kind) {
       std::vector<u32> pool;
@@ -1902,7 +1915,7 @@ void owner_pass_groups(const std::vector<OwnerCand>& cands, F&& f) {
         segs[i] = (cands[pool[i]].xl >> kTileLowLog) & kSegMask;
       }
       const std::size_t full =
-          static_cast<std::size_t>(kOwnerShapeSlots[owner_first_shape(wide != 0, false)]);
+          static_cast<std::size_t>(kOwnerShapeSlots[owner_first_shape(wide != 0, false, big)]);
       for (const OwnerGroup& g : owner_layout_groups(segs, full)) {
         std::vector<u32> members(g.members.size());
         for (std::size_t i = 0; i < members.size(); ++i) members[i] = pool[g.members[i]];
@@ -1920,11 +1933,11 @@ struct OwnerCost {
   std::size_t spread = 0;
 };
 
-OwnerCost owner_cost(const std::vector<OwnerCand>& cands) {
+OwnerCost owner_cost(const std::vector<OwnerCand>& cands, bool big) {
   OwnerCost c;
-  owner_pass_groups(cands, [&](u32, const std::vector<u32>& members, int wide, int base_kind) {
+  owner_pass_groups(cands, big, [&](u32, const std::vector<u32>& members, int wide, int base_kind) {
     const bool lanex = (base_kind >> 1) & 1;
-    const int first = owner_first_shape(wide != 0, lanex);
+    const int first = owner_first_shape(wide != 0, lanex, big);
     owner_group_shapes(members.size(), first, owner_last_shape(wide != 0, lanex),
                        [&](int, std::size_t) { ++c.bundles; });
     const std::size_t r = members.size() % static_cast<std::size_t>(kOwnerShapeSlots[first]);
@@ -1954,7 +1967,8 @@ u32 span_seg(const Span& sp, u64 x, int l) {
 // without an x-group are dropped (pass 0 stays: it carries the diagonal
 // items). Items keep their input order within a pass.
 void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>& xs,
-                       std::vector<Span>& spans, std::vector<std::vector<u64>>& pass_xs, int l) {
+                       std::vector<Span>& spans, std::vector<std::vector<u64>>& pass_xs, int l,
+                       bool big) {
   const std::size_t P = spans.size();
   if (P < 2) return;
   std::unordered_map<u64, u32> group_of;
@@ -1993,7 +2007,7 @@ void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>
       if (g == drop || (at[g] != p && g != add)) continue;
       owner_cands(items[item_idx[k]], 0, seg_in(g, p), cands);
     }
-    return owner_cost(cands);
+    return owner_cost(cands, big);
   };
   constexpr u32 kNone = 0xFFFFFFFFU;
   std::vector<OwnerCost> pc(P);
@@ -2209,7 +2223,9 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
     }
   }
 
-  if (kOwnerSweep && structured) pack_owner_passes(items, xs, spans, pass_xs, l);
+  if (kOwnerSweep && structured) {
+    pack_owner_passes(items, xs, spans, pass_xs, l, static_cast<int>(n) >= kOwnerWideMinQubits);
+  }
 
   // Emit passes: local coordinates, segment offsets, free positions; items are
   // reordered pass by pass. Diagonal items either lead the item list (swept
@@ -2455,10 +2471,11 @@ void plan_owner_bundles(const std::vector<ExpItem>& items, std::vector<PassLaunc
       const ExpItem& e = items[pl.first_item + r];
       if (e.x != 0) owner_cands(e, r, static_cast<u32>(e.xl), cands);
     }
-    owner_pass_groups(cands, [&](u32 lam, const std::vector<u32>& group, int wide, int base_kind) {
+    const bool big = d.n >= kOwnerWideMinQubits;
+    owner_pass_groups(cands, big, [&](u32 lam, const std::vector<u32>& group, int wide, int base_kind) {
       const bool lanex = (base_kind >> 1) & 1;
       const SegLayout L = seg_layout(lanex ? 0U : lam);
-      owner_group_shapes(group.size(), owner_first_shape(wide != 0, lanex),
+      owner_group_shapes(group.size(), owner_first_shape(wide != 0, lanex, big),
                          owner_last_shape(wide != 0, lanex), [&](int shape, std::size_t s) {
         const int M = kOwnerShapeSlots[shape];
         OwnerBundle bd{};

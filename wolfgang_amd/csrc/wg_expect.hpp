@@ -68,7 +68,7 @@ struct PassDesc {
 // Each bundle may change the basis of the 9 segment coordinates (local bits
 // 3..11) so that every item of a register-pivot bundle has local bit 11 set;
 // items inside the low 3 bits use a lane pivot.
-constexpr int kOwnerMaxSlots = 9;
+constexpr int kOwnerMaxSlots = 11;
 constexpr int kOwnerMaxKeys = 4;
 
 struct OwnerKey {
