@@ -80,9 +80,17 @@ struct ProductTerms {
   template <int W_>
   __device__ double2 term_ij(u32 i, u32 j, u64* ox, u64* oz) const {
     constexpr u32 RW = product_record_words(W_);
+    return combine<W_>(lrec + static_cast<std::size_t>(i) * RW,
+                       rrec + static_cast<std::size_t>(j) * RW, ox, oz);
+  }
+  // The same from the two operand records wherever they are (global memory or
+  // a copy in LDS).
+  template <int W_>
+  __device__ static double2 combine(const u64* __restrict__ ra, const u64* __restrict__ rb,
+                                    u64* ox, u64* oz) {
     u64 a[2 * W_ + 2], b[2 * W_ + 2];
-    load<W_>(lrec + static_cast<std::size_t>(i) * RW, a);
-    load<W_>(rrec + static_cast<std::size_t>(j) * RW, b);
+    load<W_>(ra, a);
+    load<W_>(rb, b);
     int k = 0;
 #pragma unroll
     for (int w = 0; w < W_; ++w) {

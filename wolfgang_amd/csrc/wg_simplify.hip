@@ -1045,37 +1045,75 @@ __global__ __launch_bounds__(256) void product_sample_kernel(ProductTerms P, u32
 // order with the reference's +0.0 seed when it applies, and writes masks and
 // sum at output position s; keep[s] flags |sum| > threshold and the survivors
 // are counted (one atomic per block).
+typedef __attribute__((address_space(3))) void lds_void;
+
 template <int W>
 __global__ __launch_bounds__(256) void product_emit_kernel(
     SegMap map, std::size_t S, const u32* __restrict__ idx, ProductTerms P,
     int zero_seed, double threshold, u64* __restrict__ ox, u64* __restrict__ oz,
     double2* __restrict__ oc, u32* __restrict__ keep, u32* __restrict__ survivors) {
+  constexpr u32 RW = product_record_words(W);
+  constexpr u32 kChunks = RW / 2;         // 16-byte pieces of a record
+  constexpr u32 kPerLoad = 64 / kChunks;  // records one wave-wide load moves
+  // Per wave: the operand records of the first product of its 64 segments.
+  __shared__ u64 stage[4][2][64 * RW];
   __shared__ u32 lds[4];
+  const u32 lane = threadIdx.x & 63U, wave = threadIdx.x >> 6;
+  u64* const lst = stage[wave][0];
+  u64* const rst = stage[wave][1];
   u32 kept = 0;
-  for (std::size_t s = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < S;
-       s += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
-    u32 b, e;
-    segment_bounds(map, s, b, e);
-    u64 mx[W], mz[W];
-    double2 acc = P.term<W>(idx[b], mx, mz);
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+  for (std::size_t base = static_cast<std::size_t>(blockIdx.x) * blockDim.x + 64U * wave; base < S;
+       base += stride) {
+    const std::size_t s = base + lane;
+    const bool valid = s < S;
+    u32 b = 0, e = 0, t = 0;
+    if (valid) {
+      segment_bounds(map, s, b, e);
+      t = idx[b];
+    }
+    // Records in sorted order are scattered lines: loaded by the lane that
+    // needs them they cost 2 (W + 1) 16-byte requests per segment. Here each
+    // wave-wide load (gfx950's 16-byte LDS DMA) gives kChunks consecutive
+    // lanes one record, a single line, and each lane then reads its segment's
+    // records from LDS.
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      ox[s * W + w] = mx[w];
-      oz[s * W + w] = mz[w];
+    for (u32 g = 0; g < kChunks; ++g) {
+      const u32 src = g * kPerLoad + lane / kChunks;
+      const u32 ts = static_cast<u32>(__shfl(static_cast<int>(t), static_cast<int>(src), 64));
+      if (base + src < S) {
+        u32 i, j;
+        P.split(ts, i, j);
+        const u32 c = 2U * (lane % kChunks);
+        __builtin_amdgcn_global_load_lds(P.lrec + static_cast<std::size_t>(i) * RW + c,
+                                         (lds_void*)(lst + g * kPerLoad * RW), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(P.rrec + static_cast<std::size_t>(j) * RW + c,
+                                         (lds_void*)(rst + g * kPerLoad * RW), 16, 0, 0);
+      }
     }
-    for (u32 q = b + 1; q < e; ++q) {
-      const double2 v = P.coeff<W>(idx[q]);
-      acc.x += v.x;
-      acc.y += v.y;
+    __builtin_amdgcn_s_waitcnt(0);
+    if (valid) {
+      u64 mx[W], mz[W];
+      double2 acc = ProductTerms::combine<W>(lst + lane * RW, rst + lane * RW, mx, mz);
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        ox[s * W + w] = mx[w];
+        oz[s * W + w] = mz[w];
+      }
+      for (u32 q = b + 1; q < e; ++q) {
+        const double2 v = P.coeff<W>(idx[q]);
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      if (zero_seed) {
+        acc.x += 0.0;
+        acc.y += 0.0;
+      }
+      oc[s] = acc;
+      const u32 k = hypot(acc.x, acc.y) > threshold ? 1U : 0U;
+      keep[s] = k;
+      kept += k;
     }
-    if (zero_seed) {
-      acc.x += 0.0;
-      acc.y += 0.0;
-    }
-    oc[s] = acc;
-    const u32 k = hypot(acc.x, acc.y) > threshold ? 1U : 0U;
-    keep[s] = k;
-    kept += k;
   }
   for (int d = 32; d > 0; d >>= 1) kept += static_cast<u32>(__shfl_xor(static_cast<int>(kept), d, 64));
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = kept;
