@@ -1834,11 +1834,11 @@ struct OwnerGroup {
 
 std::vector<OwnerGroup> owner_layout_groups(const std::vector<u32>& segs, std::size_t full) {
   std::vector<OwnerGroup> out;
-  std::vector<u32> pool(segs.size());
+  std::vector<u32> pool(segs.size()), rest;
   for (u32 i = 0; i < pool.size(); ++i) pool[i] = i;
-  std::vector<int> w(std::size_t{1} << kSegBits);
+  std::array<int, std::size_t{1} << kSegBits> w;
   while (!pool.empty()) {
-    std::fill(w.begin(), w.end(), 0);
+    w.fill(0);
     for (u32 i : pool) ++w[segs[i] & kSegMask];
     for (std::size_t len = 1; len < w.size(); len <<= 1) {
       for (std::size_t b = 0; b < w.size(); b += 2 * len) {
@@ -1862,13 +1862,13 @@ std::vector<OwnerGroup> owner_layout_groups(const std::vector<u32>& segs, std::s
     if (best_n == 0) fail_runtime("owner bundle slot has no register pivot");
     const std::size_t quota = best_n >= full ? best_n / full * full : best_n;
     OwnerGroup g{best, {}};
-    std::vector<u32> rest;
+    rest.clear();
     for (u32 i : pool) {
       const bool covered = host_parity(best & segs[i]);
       (covered && g.members.size() < quota ? g.members : rest).push_back(i);
     }
     out.push_back(std::move(g));
-    pool = std::move(rest);
+    pool.swap(rest);
   }
   return out;
 }
@@ -1972,26 +1972,52 @@ void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>
   const std::size_t P = spans.size();
   if (P < 2) return;
   std::unordered_map<u64, u32> group_of;
-  for (u32 g = 0; g < xs.size(); ++g) group_of.emplace(xs[g], g);
-  std::vector<int> at(xs.size(), -1);
-  for (std::size_t p = 0; p < P; ++p) {
-    for (u64 x : pass_xs[p]) at[group_of.at(x)] = static_cast<int>(p);
+  std::unordered_map<u64, std::vector<u32>> by_high;  // x >> l -> x-groups
+  for (u32 g = 0; g < xs.size(); ++g) {
+    group_of.emplace(xs[g], g);
+    by_high[xs[g] >> l].push_back(g);
   }
-  // Off-diagonal items in input order with their x-group; each x-group's
-  // segment part in every pass that holds it.
-  std::vector<u32> item_idx, item_group;
+  // Each x-group's items (consecutive in the input) and the passes holding it
+  // with its pass-local x. Every pass's subspace holds the low l bits, so x
+  // lies in it exactly when x >> l lies in the span of its coordinate bits
+  // and generators above them: 2^(k - l) lookups per pass.
+  std::vector<u32> first_item(xs.size(), 0), num_items(xs.size(), 0);
   for (u32 i = 0; i < items.size(); ++i) {
     if (items[i].x == 0) continue;
-    item_idx.push_back(i);
-    item_group.push_back(group_of.at(items[i].x));
+    const u32 g = group_of.at(items[i].x);
+    if (num_items[g]++ == 0) first_item[g] = i;
   }
   std::vector<std::vector<std::pair<int, u32>>> cover(xs.size());
-  for (u32 g = 0; g < xs.size(); ++g) {
-    for (std::size_t p = 0; p < P; ++p) {
-      if (spans[p].contains(xs[g])) {
-        cover[g].push_back({static_cast<int>(p), span_seg(spans[p], xs[g], l) << kTileLowLog});
+  for (std::size_t p = 0; p < P; ++p) {
+    const Span& sp = spans[p];
+    std::vector<u64> basis;
+    for (int q = l; q < 64; ++q) {
+      if ((sp.coord >> q) & 1ULL) basis.push_back(1ULL << (q - l));
+    }
+    for (u64 gv : sp.gens) basis.push_back(gv >> l);
+    std::vector<u64> elems{0};
+    for (u64 b : basis) {
+      const std::size_t n0 = elems.size();
+      for (std::size_t e = 0; e < n0; ++e) elems.push_back(elems[e] ^ b);
+    }
+    for (u64 h : elems) {
+      const auto it = by_high.find(h);
+      if (it == by_high.end()) continue;
+      for (u32 g : it->second) {
+        if (!sp.contains(xs[g])) fail_runtime("owner pass packing: span membership differs");
+        cover[g].push_back({static_cast<int>(p), span_seg(sp, xs[g], l) << kTileLowLog});
       }
     }
+  }
+  std::vector<int> at(xs.size(), -1);
+  std::vector<std::vector<u32>> members(P);  // ascending x-group indices (= item order)
+  for (std::size_t p = 0; p < P; ++p) {
+    for (u64 x : pass_xs[p]) {
+      const u32 g = group_of.at(x);
+      at[g] = static_cast<int>(p);
+      members[p].push_back(g);
+    }
+    std::sort(members[p].begin(), members[p].end());
   }
   auto seg_in = [&](u32 g, int p) -> u32 {
     for (const auto& [q, xl] : cover[g]) {
@@ -2000,22 +2026,33 @@ void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>
     fail_runtime("owner pass packing lost an x-group");
   };
   // Cost of pass p with x-group `add` moved in and `drop` moved out.
+  constexpr u32 kNone = 0xFFFFFFFFU;
+  std::vector<OwnerCand> cands;
   auto cost = [&](int p, u32 add, u32 drop) {
-    std::vector<OwnerCand> cands;
-    for (std::size_t k = 0; k < item_idx.size(); ++k) {
-      const u32 g = item_group[k];
-      if (g == drop || (at[g] != p && g != add)) continue;
-      owner_cands(items[item_idx[k]], 0, seg_in(g, p), cands);
+    cands.clear();
+    auto put = [&](u32 g) {
+      for (u32 i = 0; i < num_items[g]; ++i) owner_cands(items[first_item[g] + i], 0, seg_in(g, p), cands);
+    };
+    bool added = add == kNone;
+    for (u32 g : members[p]) {
+      if (!added && add < g) {
+        put(add);
+        added = true;
+      }
+      if (g != drop) put(g);
     }
+    if (!added) put(add);
     return owner_cost(cands, big);
   };
-  constexpr u32 kNone = 0xFFFFFFFFU;
   std::vector<OwnerCost> pc(P);
   for (std::size_t p = 0; p < P; ++p) pc[p] = cost(static_cast<int>(p), kNone, kNone);
-  for (int sweep = 0; sweep < 4; ++sweep) {
+  // Most moves happen in the first sweep; large unstructured sets (each
+  // evaluation is a layout grouping of two passes) stop after it.
+  const int sweeps = xs.size() > 4096 ? 1 : 4;
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
     bool moved = false;
     for (u32 g = 0; g < xs.size(); ++g) {
-      if (cover[g].size() < 2) continue;
+      if (cover[g].size() < 2 || at[g] < 0) continue;
       const int p = at[g];
       const OwnerCost cp = cost(p, kNone, g);
       int best_q = -1;
@@ -2037,22 +2074,22 @@ void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>
       }
       if (best_q < 0) continue;
       at[g] = best_q;
+      members[p].erase(std::lower_bound(members[p].begin(), members[p].end(), g));
+      members[best_q].insert(std::lower_bound(members[best_q].begin(), members[best_q].end(), g), g);
       pc[p] = cp;
       pc[best_q] = best_cq;
       moved = true;
     }
     if (!moved) break;
   }
-  std::vector<std::vector<u64>> out(P);
-  for (u32 g = 0; g < xs.size(); ++g) {
-    if (at[g] >= 0) out[at[g]].push_back(xs[g]);
-  }
   std::vector<Span> kept_spans;
   std::vector<std::vector<u64>> kept_xs;
   for (std::size_t p = 0; p < P; ++p) {
-    if (p > 0 && out[p].empty()) continue;
+    if (p > 0 && members[p].empty()) continue;
     kept_spans.push_back(spans[p]);
-    kept_xs.push_back(std::move(out[p]));
+    std::vector<u64> xp;
+    for (u32 g : members[p]) xp.push_back(xs[g]);
+    kept_xs.push_back(std::move(xp));
   }
   spans = std::move(kept_spans);
   pass_xs = std::move(kept_xs);
@@ -2223,7 +2260,7 @@ void plan_tiled_passes(std::size_t n, std::vector<ExpItem>& items, std::vector<P
     }
   }
 
-  if (kOwnerSweep && structured) {
+  if (kOwnerSweep) {
     pack_owner_passes(items, xs, spans, pass_xs, l, static_cast<int>(n) >= kOwnerWideMinQubits);
   }
 
