@@ -393,6 +393,43 @@ __global__ __launch_bounds__(256) void digit_starts_kernel(const u32* __restrict
   starts[blockIdx.x * 256 + tid] = before + incl - c;
 }
 
+// Digit counts of the products' one-word keys without reading them: a
+// product's key word is the XOR of its operands' key words (x0, or
+// x0 << 32 | z0 when packed), so the count of products whose digit at shift
+// s is b is sum_a L_s[a] R_s[a ^ b], a XOR convolution of the operands' digit
+// counts (L * R < 2^32 products, so the sums fit).
+__global__ __launch_bounds__(256) void operand_digit_hist_kernel(const u64* __restrict__ rec,
+                                                                 u32 count, u32 RW, u32 W,
+                                                                 int packed, DigitShifts shifts,
+                                                                 int passes,
+                                                                 u32* __restrict__ hist) {
+  __shared__ u32 h[8][256];
+  for (int q = 0; q < 8; ++q) h[q][threadIdx.x] = 0;
+  __syncthreads();
+  for (u32 r = blockIdx.x * blockDim.x + threadIdx.x; r < count; r += gridDim.x * blockDim.x) {
+    const u64* e = rec + static_cast<std::size_t>(r) * RW;
+    const u64 key = packed ? ((e[0] << 32) | e[W]) : e[0];
+    for (int q = 0; q < passes; ++q) atomicAdd(&h[q][static_cast<u32>(key >> shifts.s[q]) & 0xFFU], 1U);
+  }
+  __syncthreads();
+  for (int q = 0; q < passes; ++q) {
+    if (h[q][threadIdx.x]) atomicAdd(&hist[q * 256 + threadIdx.x], h[q][threadIdx.x]);
+  }
+}
+
+__global__ __launch_bounds__(256) void xor_convolve_kernel(const u32* __restrict__ lh,
+                                                           const u32* __restrict__ rh,
+                                                           u32* __restrict__ hist) {
+  __shared__ u32 l[256], r[256];
+  const u32 q = blockIdx.x, b = threadIdx.x;
+  l[b] = lh[q * 256 + b];
+  r[b] = rh[q * 256 + b];
+  __syncthreads();
+  u64 c = 0;
+  for (u32 a = 0; a < 256; ++a) c += static_cast<u64>(l[a]) * r[a ^ b];
+  hist[q * 256 + b] = static_cast<u32>(c);
+}
+
 __global__ __launch_bounds__(256) void clear_u32_kernel(u32* __restrict__ p, std::size_t n) {
   for (std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
@@ -1227,9 +1264,13 @@ constexpr std::size_t kOneSweepMin = std::size_t{1} << 20;
 // end up in (keys_a, idx_a). Large one-word sorts take one histogram of every
 // pass's digits up front and then one kernel per pass (look-back offsets);
 // the others run histogram -> scan -> scatter per pass.
+//
+// With `products` (the keys are the one-word keys of those products, packed
+// when `packed`), the one-sweep digit counts come from the operands.
 void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_t n, int KW,
                 const std::vector<DigitPass>& passes, Scratch& scratch, hipStream_t stream,
-                const char* prof_name = "radix_scatter") {
+                const char* prof_name = "radix_scatter", const ProductTerms* products = nullptr,
+                bool packed = false) {
   if (passes.empty() || n < 2) return;
   const std::size_t nblocks = (n + kTile - 1) / kTile;
   if (KW == 1 && n >= kOneSweepMin && n < (std::size_t{1} << 30) && passes.size() <= 8) {
@@ -1248,9 +1289,25 @@ void radix_sort(u64*& keys_a, u32*& idx_a, u64*& keys_b, u32*& idx_b, std::size_
     check_launch("clear");
     DigitShifts sh{};
     for (std::size_t p = 0; p < P; ++p) sh.s[p] = passes[p].shift;
-    digit_hist_all_kernel<<<std::min<unsigned>(stride_grid(n), 2048), 256, 0, stream>>>(
-        keys_a, n, sh, static_cast<int>(P), hist);
-    check_launch("digit_hist_all");
+    if (products) {
+      const ProductTerms& PT = *products;
+      const u32 L = static_cast<u32>(n / PT.R), RW = product_record_words(PT.W);
+      u32* oh = scratch.take<u32>(2 * P * 256);
+      clear_u32_kernel<<<stride_grid(2 * P * 256), 256, 0, stream>>>(oh, 2 * P * 256);
+      check_launch("clear");
+      operand_digit_hist_kernel<<<std::min<unsigned>(stride_grid(L), 64), 256, 0, stream>>>(
+          PT.lrec, L, RW, PT.W, packed ? 1 : 0, sh, static_cast<int>(P), oh);
+      check_launch("operand_digit_hist");
+      operand_digit_hist_kernel<<<std::min<unsigned>(stride_grid(PT.R), 64), 256, 0, stream>>>(
+          PT.rrec, PT.R, RW, PT.W, packed ? 1 : 0, sh, static_cast<int>(P), oh + P * 256);
+      check_launch("operand_digit_hist");
+      xor_convolve_kernel<<<static_cast<unsigned>(P), 256, 0, stream>>>(oh, oh + P * 256, hist);
+      check_launch("xor_convolve");
+    } else {
+      digit_hist_all_kernel<<<std::min<unsigned>(stride_grid(n), 2048), 256, 0, stream>>>(
+          keys_a, n, sh, static_cast<int>(P), hist);
+      check_launch("digit_hist_all");
+    }
     digit_starts_kernel<<<static_cast<unsigned>(P), 256, 0, stream>>>(hist, starts);
     check_launch("digit_starts");
     for (std::size_t p = 0; p < P; ++p) {
@@ -1764,11 +1821,12 @@ wg_pauli_sum* simplify_product_keys(const ProductTerms& P, std::size_t n, std::s
   if (packed) {
     const u64 kor = (ors[0] << 32) | ors[1];
     const u64 kand = (ands[0] << 32) | ands[1];
-    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream);
+    radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({kor ^ kand}), scratch, stream,
+               "radix_scatter", &P, true);
     map = sorted_segments(P, 1, T, keys_a, idx_a, ors, ands, flags, d_count, &S, scratch, stream);
   } else {
     radix_sort(keys_a, idx_a, keys_b, idx_b, T, 1, digit_passes({ors[0] ^ ands[0]}), scratch,
-               stream);
+               stream, "radix_scatter", &P, false);
     map = sorted_segments(P, 2 * W, T, keys_a, idx_a, ors, ands, flags, d_count, &S, scratch,
                           stream);
   }
