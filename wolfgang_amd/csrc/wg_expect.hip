@@ -1966,6 +1966,12 @@ u32 span_seg(const Span& sp, u64 x, int l) {
 // count, or keeps it and concentrates the partial bundles. Passes left
 // without an x-group are dropped (pass 0 stays: it carries the diagonal
 // items). Items keep their input order within a pass.
+// Perturbation rounds of the packing search, for states of kOwnerWideMinQubits
+// and up (where a bundle reads gigabytes) and up to this many x-groups
+// (structured operators; each round is a full local search).
+constexpr int kPackRounds = 48;
+constexpr std::size_t kPackRoundsMaxGroups = 1024;
+
 void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>& xs,
                        std::vector<Span>& spans, std::vector<std::vector<u64>>& pass_xs, int l,
                        bool big) {
@@ -2048,39 +2054,95 @@ void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>
   for (std::size_t p = 0; p < P; ++p) pc[p] = cost(static_cast<int>(p), kNone, kNone);
   // Most moves happen in the first sweep; large unstructured sets (each
   // evaluation is a layout grouping of two passes) stop after it.
-  const int sweeps = xs.size() > 4096 ? 1 : 4;
-  for (int sweep = 0; sweep < sweeps; ++sweep) {
-    bool moved = false;
-    for (u32 g = 0; g < xs.size(); ++g) {
-      if (cover[g].size() < 2 || at[g] < 0) continue;
-      const int p = at[g];
-      const OwnerCost cp = cost(p, kNone, g);
-      int best_q = -1;
-      long best_db = 0, best_ds = 0;
-      OwnerCost best_cq{};
-      for (const auto& [q, xl] : cover[g]) {
-        if (q == p) continue;
-        const OwnerCost cq = cost(q, g, kNone);
-        const long db = static_cast<long>(cp.bundles + cq.bundles) -
-                        static_cast<long>(pc[p].bundles + pc[q].bundles);
-        const long ds = static_cast<long>(cp.spread + cq.spread) -
-                        static_cast<long>(pc[p].spread + pc[q].spread);
-        if ((db < best_db) || (db == best_db && ds > best_ds)) {
-          best_q = q;
-          best_db = db;
-          best_ds = ds;
-          best_cq = cq;
+  auto relocate = [&](u32 g, int q, const OwnerCost& cp, const OwnerCost& cq) {
+    const int p = at[g];
+    at[g] = q;
+    members[p].erase(std::lower_bound(members[p].begin(), members[p].end(), g));
+    members[q].insert(std::lower_bound(members[q].begin(), members[q].end(), g), g);
+    pc[p] = cp;
+    pc[q] = cq;
+  };
+  auto local_search = [&](int sweeps) {
+    for (int sweep = 0; sweep < sweeps; ++sweep) {
+      bool moved = false;
+      for (u32 g = 0; g < xs.size(); ++g) {
+        if (cover[g].size() < 2 || at[g] < 0) continue;
+        const int p = at[g];
+        const OwnerCost cp = cost(p, kNone, g);
+        int best_q = -1;
+        long best_db = 0, best_ds = 0;
+        OwnerCost best_cq{};
+        for (const auto& [q, xl] : cover[g]) {
+          if (q == p) continue;
+          const OwnerCost cq = cost(q, g, kNone);
+          const long db = static_cast<long>(cp.bundles + cq.bundles) -
+                          static_cast<long>(pc[p].bundles + pc[q].bundles);
+          const long ds = static_cast<long>(cp.spread + cq.spread) -
+                          static_cast<long>(pc[p].spread + pc[q].spread);
+          if ((db < best_db) || (db == best_db && ds > best_ds)) {
+            best_q = q;
+            best_db = db;
+            best_ds = ds;
+            best_cq = cq;
+          }
         }
+        if (best_q < 0) continue;
+        relocate(g, best_q, cp, best_cq);
+        moved = true;
       }
-      if (best_q < 0) continue;
-      at[g] = best_q;
-      members[p].erase(std::lower_bound(members[p].begin(), members[p].end(), g));
-      members[best_q].insert(std::lower_bound(members[best_q].begin(), members[best_q].end(), g), g);
-      pc[p] = cp;
-      pc[best_q] = best_cq;
-      moved = true;
+      if (!moved) break;
     }
-    if (!moved) break;
+  };
+  auto total = [&]() {
+    std::size_t b = 0;
+    for (const OwnerCost& c : pc) b += c.bundles;
+    return b;
+  };
+  // Most moves happen in the first sweep; large unstructured sets (each
+  // evaluation is a layout grouping of two passes) stop after it.
+  local_search(xs.size() > 4096 ? 1 : 4);
+  if (big && xs.size() <= kPackRoundsMaxGroups) {
+    // Perturbation rounds: a few x-groups move to another pass that holds
+    // them at random (a fixed-seed generator, so plans are reproducible),
+    // the local search runs again, and the best assignment is kept.
+    std::vector<u32> movable;
+    for (u32 g = 0; g < xs.size(); ++g) {
+      if (cover[g].size() >= 2 && at[g] >= 0) movable.push_back(g);
+    }
+    u64 rng = 0x9E3779B97F4A7C15ULL ^ xs.size();
+    auto next = [&rng]() {
+      rng += 0x9E3779B97F4A7C15ULL;
+      u64 z = rng;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+      return z ^ (z >> 31);
+    };
+    std::vector<int> best_at = at;
+    std::vector<std::vector<u32>> best_members = members;
+    std::vector<OwnerCost> best_pc = pc;
+    std::size_t best_total = total();
+    for (int round = 0; round < kPackRounds && !movable.empty(); ++round) {
+      const std::size_t kicks = 1 + movable.size() / 16;
+      for (std::size_t k = 0; k < kicks; ++k) {
+        const u32 g = movable[next() % movable.size()];
+        const auto& cv = cover[g];
+        const int q = cv[next() % cv.size()].first;
+        if (q == at[g]) continue;
+        relocate(g, q, cost(at[g], kNone, g), cost(q, g, kNone));
+      }
+      local_search(4);
+      const std::size_t t = total();
+      if (t < best_total) {
+        best_total = t;
+        best_at = at;
+        best_members = members;
+        best_pc = pc;
+      } else {
+        at = best_at;
+        members = best_members;
+        pc = best_pc;
+      }
+    }
   }
   std::vector<Span> kept_spans;
   std::vector<std::vector<u64>> kept_xs;
