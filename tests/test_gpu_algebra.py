@@ -113,6 +113,17 @@ def test_matmul_bit_exact(gpu, orc, n, simplify):
     assert_bit_exact(got, want)
 
 
+@pytest.mark.parametrize("n", [20, 50, 150, 256])
+def test_matmul_one_sweep_sort_bit_exact(gpu, orc, n):
+    # 1,024 x 1,024 = 2^20 products: the smallest product sort that runs the
+    # one-sweep radix passes, whose digit counts come from the operands' key
+    # words (packed x << 32 | z at n = 20, x0 at 50; two and four mask words
+    # at 150 and 256).
+    a = random_pauli_masks(n, 1024, 0.3, 11 * n + 1)
+    b = random_pauli_masks(n, 1024, 0.3, 11 * n + 2)
+    assert_bit_exact(a @ b, orc.matmul(n, packed(a)[1:], packed(b)[1:]))
+
+
 @pytest.mark.parametrize("n", [1, 65])
 def test_duplicate_heavy_matmul_hash_path_bit_exact(gpu, orc, n):
     # reference tests/test_phase5_arithmetic.py:148-173: 128 x 128 products of
