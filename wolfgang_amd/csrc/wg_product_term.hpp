@@ -126,7 +126,7 @@ struct ProductTerms {
 // bits order like the values; only with kMax) — at block_stats[k * stride +
 // unit] for k < 4W + 1.
 constexpr u32 kProductKeyBlock = 256;
-constexpr u32 kProductPerThread = 4;
+constexpr u32 kProductPerThread = 8;
 constexpr u32 kProductChunk = kProductKeyBlock * kProductPerThread;
 constexpr int kProductMaxStats = 4 * 4 + 1;
 

@@ -2,7 +2,8 @@
 // qubit counts, every unit of every pair served from one device work queue by
 // one persistent kernel, then each pair's sort-reduce simplify.
 //
-// A unit of work is (pair, lhs term, block of 1,024 rhs terms). With simplify
+// A unit of work is (pair, lhs term, block of rhs terms: 2,048 when the unit
+// writes sort keys, 1,024 when it writes products). With simplify
 // (the matmul path) a unit writes its products' word-0 sort keys, indices and
 // key statistics, as matmul's key pass does, and the products themselves are
 // never materialized: each pair's sort, segment heads and segment sums read
