@@ -64,3 +64,30 @@ def test_tiled_plan_replay_matches_oracle(orc, gen, n, terms):
         # Random masks: the greedy generators find coincidences x_a ^ x_b =
         # x_c, so a pass holds far more than its 9 generators.
         assert passes * 20 < groups
+
+
+def plan_rows(op):
+    x = np.ascontiguousarray(op.x_words()[:, 0])
+    z = np.ascontiguousarray(op.z_words()[:, 0])
+    rows = np.zeros((4096, 8), dtype=np.int32)
+    npass = C.c_size_t()
+    _native.call("wg_debug_plan_passes", op.num_qubits, op.num_terms, x.ctypes.data, z.ctypes.data,
+                 rows.ctypes.data, 4096, C.byref(npass))
+    return rows[: npass.value].copy()
+
+
+@pytest.mark.parametrize("n,terms", [(20, 3000), (30, 20_000)])
+def test_owner_packing_is_reproducible_and_fills_bundles(n, terms):
+    # The packing search (with its perturbation rounds at 25 qubits and up)
+    # draws from a fixed-seed generator: the same operator always gets the
+    # same plan, and therefore the same summation order and result bits.
+    op = chemistry_masks(n, terms, 3001)
+    a, b = plan_rows(op), plan_rows(op)
+    np.testing.assert_array_equal(a, b)
+    # Every off-diagonal item sits in a bundle slot, and the bundles are
+    # nearly full: at most one partial bundle of slack per pass.
+    items, bundles, slots = int(a[:, 5].sum()), int(a[:, 6].sum()), int(a[:, 7].sum())
+    assert slots >= items
+    full = 11 if n >= 25 else 9
+    assert slots - items < full * a.shape[0]
+    assert bundles * full >= items
