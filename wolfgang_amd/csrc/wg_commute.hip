@@ -241,39 +241,50 @@ void check_entries(std::size_t L, std::size_t R, std::size_t max_entries) {
   }
 }
 
-void launch_commute(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, u64* out,
-                    std::size_t row_words, hipStream_t stream) {
-  const std::size_t L = lhs->num_terms, R = rhs->num_terms, W = lhs->words;
+}  // namespace
+
+// Packed commutation bits of (lx, lz) [L x W] against (rx, rz) [R x W] into
+// out [L x row_words]: bit c of row r is 1 when the terms commute.
+void launch_commute_raw(const u64* lx, const u64* lz, std::size_t L, const u64* rx, const u64* rz,
+                        std::size_t R, std::size_t W, u64* out, std::size_t row_words,
+                        hipStream_t stream) {
   if (L == 0 || R == 0) return;
   const std::size_t word_blocks = (row_words + kWordsPerBlock - 1) / kWordsPerBlock;
   const std::size_t row_blocks = (L + 63) / 64;
   const unsigned grid = grid_for(checked_mul(word_blocks, row_blocks, "commutation grid"), 1);
   ProfileScope prof("commute_packed", stream);
   switch (W) {
-    case 0: {
+    case 0:
       // Zero-qubit operators: every pair commutes.
-      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
-                                                            0, row_words, word_blocks, out);
+      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lx, lz, rx, rz, L, R, 0, row_words,
+                                                            word_blocks, out);
       break;
-    }
     case 1:
-      commute_packed_kernel<1><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
-                                                            1, row_words, word_blocks, out);
+      commute_packed_kernel<1><<<grid, kBlock, 0, stream>>>(lx, lz, rx, rz, L, R, 1, row_words,
+                                                            word_blocks, out);
       break;
     case 2:
-      commute_packed_kernel<2><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
-                                                            2, row_words, word_blocks, out);
+      commute_packed_kernel<2><<<grid, kBlock, 0, stream>>>(lx, lz, rx, rz, L, R, 2, row_words,
+                                                            word_blocks, out);
       break;
     case 4:
-      commute_packed_kernel<4><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
-                                                            4, row_words, word_blocks, out);
+      commute_packed_kernel<4><<<grid, kBlock, 0, stream>>>(lx, lz, rx, rz, L, R, 4, row_words,
+                                                            word_blocks, out);
       break;
     default:
-      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lhs->x, lhs->z, rhs->x, rhs->z, L, R,
-                                                            W, row_words, word_blocks, out);
+      commute_packed_kernel<0><<<grid, kBlock, 0, stream>>>(lx, lz, rx, rz, L, R, W, row_words,
+                                                            word_blocks, out);
       break;
   }
   check_launch("commute_packed");
+}
+
+namespace {
+
+void launch_commute(const wg_pauli_sum* lhs, const wg_pauli_sum* rhs, u64* out,
+                    std::size_t row_words, hipStream_t stream) {
+  launch_commute_raw(lhs->x, lhs->z, lhs->num_terms, rhs->x, rhs->z, rhs->num_terms, lhs->words,
+                     out, row_words, stream);
 }
 
 void validate_operands(const char* operation, const wg_pauli_sum* lhs, const wg_pauli_sum* rhs) {

@@ -2,27 +2,48 @@
 // src/grouping.cpp:197-290, strategy "largest_first").
 //
 // The visiting order (descending weight |x | z|, then canonical key order,
-// then index; reference src/grouping.cpp:36-62) is built on the host. The
-// placement itself is sequential by definition (each term takes the first
-// group that accepts it, given every earlier placement), so one 1024-thread
-// block walks the order and, per term, tests all open groups in parallel:
-//   * full: group g keeps a conflict bit row over all terms (OR of the
-//     anticommutation rows of its members, taken from the packed device
-//     commutation matrix); term t is accepted iff bit t of the row is clear,
-//     and joining ORs t's anticommutation row into it;
-//   * qubit-wise: group g keeps Xb, Zb (ORs of its members' masks); t is
-//     accepted iff (x_t & Zb) ^ (z_t & Xb) = 0 on every word (reference
-//     src/grouping.cpp:64-85).
-// The first accepting group is a block-wide minimum. The result equals the
-// reference's graph and streaming paths, which differ only in how they test
-// acceptance.
+// then index; reference src/grouping.cpp:36-62) is built on the host, and the
+// masks are gathered into that order on the device, so a term is named by its
+// visiting position s. Each term takes the first group that accepts it given
+// every earlier placement:
+//   * full: a group accepts s when s commutes with every member; group g keeps
+//     a conflict bit row over positions (the OR of its members'
+//     anticommutation rows, from the packed commutation matrix of the
+//     reordered operator);
+//   * qubit-wise: group g keeps Xb, Zb (ORs of its members' masks) and accepts
+//     s iff (x_s & Zb) ^ (z_s & Xb) = 0 on every word (reference
+//     src/grouping.cpp:64-85). Members of a group never disagree on a qubit,
+//     so this equals "compatible with every member".
+// In both modes acceptance is "no member conflicts with s", and adding
+// members only removes acceptances.
+//
+// The positions are resolved in batches of kBatch, with the same result as one
+// position at a time:
+//   1. group_candidates_kernel (one block per batch position, whole GPU): the
+//      first kCands groups that accept s as the groups stood before the batch,
+//      the next accepting group after them (where to resume a scan), and the
+//      conflict bits of s against the earlier positions of its batch;
+//   2. group_resolve_kernel (one block): walks the batch in order. Position i
+//      stamps the groups of its conflicting earlier batch positions (the only
+//      members the candidate kernel did not see), then takes the first
+//      candidate without its stamp; if every candidate is stamped, it scans
+//      the remaining pre-batch groups from the resume point; then the groups
+//      opened earlier in this batch (no pre-batch members: accepted unless
+//      stamped); else it opens a new group;
+//   3. group_commit_*_kernel: folds the batch's placements into the group
+//      state for the next batch.
+// A group rejects s exactly when a pre-batch member conflicts (step 1) or a
+// member placed earlier in the batch conflicts (the stamps), so each position
+// gets the group the one-at-a-time walk would give it.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
+#include <cstdint>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "wg_internal.hpp"
@@ -30,82 +51,348 @@
 namespace wg {
 namespace {
 
-constexpr int kGroupBlock = 1024;
+constexpr u32 kBatch = 1024;         // visiting positions resolved per batch
+constexpr int kCandBlock = 256;      // candidate kernel block (one batch position)
+constexpr int kResolveBlock = 256;   // resolve kernel: 4 waves, 4 batch positions per lane
+constexpr int kCands = 15;           // pre-batch candidate groups kept per position
+constexpr int kCandWords = 16;       // the candidates and the resume group
+constexpr u32 kNone = 0xFFFFFFFFu;
+constexpr u32 kLdsGroups = 32768;    // stamps live in LDS when every group id is below this
 
-// Block-wide minimum of v; every thread gets the result.
-__device__ u32 block_min(u32 v, u32* scratch) {
-  for (int d = 32; d > 0; d >>= 1) v = min(v, static_cast<u32>(__shfl_xor(static_cast<int>(v), d, 64)));
-  const u32 wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 0) scratch[wave] = v;
-  __syncthreads();
-  u32 r = UINT_MAX;
-  for (u32 w = 0; w < kGroupBlock / 64; ++w) r = min(r, scratch[w]);
-  __syncthreads();
-  return r;
+static_assert(kBatch == 4 * kResolveBlock, "four batch positions per resolve thread");
+
+enum GroupMode : int { kQwc = 0, kFull = 1 };
+
+struct GroupState {
+  const u64* px;        // masks in visiting order [T x W]
+  const u64* pz;
+  u64 W;
+  const u64* commute;   // full: packed commutation rows in visiting order [T x row_words]
+  u64 row_words;
+  u64* conflict;        // full: per group, OR of the members' anticommutation rows
+  u64* xb;              // qubit-wise: per group, OR of the members' masks [T x W]
+  u64* zb;
+};
+
+// Does group g (its members so far) accept position s?
+template <int kMode>
+__device__ inline bool group_accepts(const GroupState& st, u32 g, u64 s) {
+  if constexpr (kMode == kFull) {
+    return ((st.conflict[static_cast<u64>(g) * st.row_words + (s >> 6)] >> (s & 63)) & 1) == 0;
+  } else {
+    const u64* xs = st.px + s * st.W;
+    const u64* zs = st.pz + s * st.W;
+    const u64* xg = st.xb + static_cast<u64>(g) * st.W;
+    const u64* zg = st.zb + static_cast<u64>(g) * st.W;
+    for (u64 k = 0; k < st.W; ++k) {
+      if ((xs[k] & zg[k]) ^ (zs[k] & xg[k])) return false;
+    }
+    return true;
+  }
 }
 
-__global__ __launch_bounds__(kGroupBlock) void greedy_full_kernel(
-    const u32* __restrict__ order, u32 T, const u64* __restrict__ commute_bits, u64 row_words,
-    u64* __restrict__ conflict, u32* __restrict__ group_of, u32* __restrict__ num_groups) {
-  __shared__ u32 scratch[kGroupBlock / 64];
-  u32 G = 0;  // identical in every thread
-  for (u32 s = 0; s < T; ++s) {
-    const u32 t = order[s];
-    const u64 w = t >> 6;
-    const u64 bit = 1ULL << (t & 63);
-    u32 mine = UINT_MAX;
-    for (u32 g = threadIdx.x; g < G; g += kGroupBlock) {
-      if (!(conflict[g * row_words + w] & bit)) {
-        mine = g;
-        break;
-      }
+// May positions a and b not share a group?
+template <int kMode>
+__device__ inline bool terms_conflict(const GroupState& st, u64 a, u64 b) {
+  if constexpr (kMode == kFull) {
+    return ((st.commute[a * st.row_words + (b >> 6)] >> (b & 63)) & 1) == 0;
+  } else {
+    const u64* xa = st.px + a * st.W;
+    const u64* za = st.pz + a * st.W;
+    const u64* xc = st.px + b * st.W;
+    const u64* zc = st.pz + b * st.W;
+    for (u64 k = 0; k < st.W; ++k) {
+      if ((xa[k] & zc[k]) ^ (za[k] & xc[k])) return true;
     }
-    u32 g = block_min(mine, scratch);
-    if (g == UINT_MAX) g = G++;
-    const u64* crow = commute_bits + static_cast<u64>(t) * row_words;
-    u64* row = conflict + static_cast<u64>(g) * row_words;
-    for (u64 k = threadIdx.x; k < row_words; k += kGroupBlock) row[k] |= ~crow[k];
-    if (threadIdx.x == 0) group_of[t] = g;
+    return false;
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_masks_kernel(const u32* __restrict__ order,
+                                                           const u64* __restrict__ x,
+                                                           const u64* __restrict__ z, u64 T, u64 W,
+                                                           u64* __restrict__ px,
+                                                           u64* __restrict__ pz) {
+  const u64 n = T * W;
+  for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 s = i / W, k = i % W;
+    const u64 src = static_cast<u64>(order[s]) * W + k;
+    px[i] = x[src];
+    pz[i] = z[src];
+  }
+}
+
+// Block i: position s = s0 + i of the batch.
+//   cmask[i][t], bit m: s conflicts with batch position 256 (t >> 6) + 64 m + (t & 63),
+//     for earlier positions only (the resolve kernel's register layout);
+//   cand[i][0..kCands): the first kCands groups (index order) that accept s given
+//     the groups before this batch, kNone-padded; cand[i][kCands]: the next
+//     accepting group after them, kNone when there is none.
+template <int kMode>
+__global__ __launch_bounds__(kCandBlock) void group_candidates_kernel(
+    GroupState st, u64 s0, const u32* __restrict__ num_groups, u32* __restrict__ cand,
+    std::uint8_t* __restrict__ cmask) {
+  __shared__ u64 ballots[4][kCandBlock / 64];
+  __shared__ u32 list[kCandWords];
+  __shared__ u32 found_s, resume_s;
+  const u32 i = blockIdx.x;
+  const u64 s = s0 + i;
+  const u32 t = threadIdx.x, wave = t >> 6, lane = t & 63;
+
+  u32 mask = 0;
+#pragma unroll
+  for (u32 m = 0; m < 4; ++m) {
+    const u32 j = 256 * wave + 64 * m + lane;
+    if (j < i && terms_conflict<kMode>(st, s, s0 + j)) mask |= 1U << m;
+  }
+  cmask[static_cast<u64>(i) * kResolveBlock + t] = static_cast<std::uint8_t>(mask);
+
+  const u32 G = *num_groups;
+  u32 found = 0, resume = kNone;
+  for (u32 base = 0; base < G && resume == kNone; base += 4 * kCandBlock) {
+    bool acc[4];
+#pragma unroll
+    for (u32 q = 0; q < 4; ++q) {
+      const u32 g = base + q * kCandBlock + t;
+      acc[q] = g < G && group_accepts<kMode>(st, g < G ? g : 0, s);
+    }
+#pragma unroll
+    for (u32 q = 0; q < 4; ++q) {
+      const u64 b = __ballot(acc[q]);
+      if (lane == 0) ballots[q][wave] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+      u32 f = found, r = kNone;
+      for (u32 q = 0; q < 4 && r == kNone; ++q) {
+        for (u32 w = 0; w < kCandBlock / 64 && r == kNone; ++w) {
+          u64 b = ballots[q][w];
+          while (b != 0) {
+            const u32 g = base + q * kCandBlock + w * 64 + static_cast<u32>(__builtin_ctzll(b));
+            if (f < kCands) {
+              list[f++] = g;
+            } else {
+              r = g;
+              break;
+            }
+            b &= b - 1;
+          }
+        }
+      }
+      found_s = f;
+      resume_s = r;
+    }
+    __syncthreads();
+    found = found_s;
+    resume = resume_s;
+  }
+  if (t < kCandWords) {
+    cand[static_cast<u64>(i) * kCandWords + t] = t < kCands ? (t < found ? list[t] : kNone) : resume;
+  }
+}
+
+// First pre-batch group at or after `from` that accepts s and is not stamped.
+template <int kMode, typename Stamp>
+__device__ u32 scan_old_groups(const GroupState& st, u64 s, u32 from, u32 G_old,
+                               const Stamp* cur, Stamp stamp, u32 lane) {
+  for (u32 base = from; base < G_old; base += 64) {
+    const u32 g = base + lane;
+    const u32 gs = g < G_old ? g : 0;
+    const bool ok = g < G_old && group_accepts<kMode>(st, gs, s) && cur[gs] != stamp;
+    const u64 b = __ballot(ok);
+    if (b != 0) return base + static_cast<u32>(__builtin_ctzll(b));
+  }
+  return kNone;
+}
+
+// One block walks the batch. Thread t holds the groups of batch positions
+// 256 (t >> 6) + 64 q + (t & 63), q < 4. Stamps alternate between two arrays
+// (position i uses array i & 1), so one barrier per position suffices: a
+// wave stamps for position i + 2 only after every wave has passed position
+// i + 1's barrier, i.e. finished reading position i's stamps. With kLds the
+// stamps are 16-bit batch-local (i + 1) and zeroed per launch; otherwise
+// 32-bit position stamps (s + 1) in global memory, zeroed once.
+// The candidate lists and conflict masks are staged through LDS kChunk
+// positions at a time: the next chunk's are loaded into registers at the start
+// of a chunk and written to LDS at its end, so their latency is hidden.
+constexpr u32 kChunk = 32;
+
+template <int kMode, bool kLds>
+__global__ __launch_bounds__(kResolveBlock) void group_resolve_kernel(
+    GroupState st, u64 s0, u32 nb, const u32* __restrict__ cand,
+    const std::uint8_t* __restrict__ cmask, u32* __restrict__ num_groups, u32* __restrict__ gpos,
+    u32* __restrict__ gstamps, u64 gstamp_len) {
+  using Stamp = std::conditional_t<kLds, std::uint16_t, u32>;
+  constexpr u32 kLdsLen = kLdsGroups + kResolveBlock;
+  __shared__ Stamp lds_stamps[kLds ? 2 * kLdsLen : 1];
+  __shared__ uint2 s_cand[kChunk * kCandWords / 2];
+  __shared__ uint4 s_cmask[kChunk * kResolveBlock / 16];
+  __shared__ u32 s_gpos[kBatch];
+  static_assert(kChunk * kCandWords / 2 == kResolveBlock && kChunk * kResolveBlock / 16 == 2 * kResolveBlock,
+                "one uint2 and two uint4 of a chunk per thread");
+  const u32 t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const u32 G_old = *num_groups;
+  u32 G_cur = G_old;
+  Stamp* st0;
+  Stamp* st1;
+  u32 dummy;  // a private slot for threads with nothing to stamp
+  if constexpr (kLds) {
+    st0 = lds_stamps;
+    st1 = lds_stamps + kLdsLen;
+    dummy = kLdsGroups + t;
+    for (u32 k = t; k < G_old + nb; k += kResolveBlock) {
+      st0[k] = 0;
+      st1[k] = 0;
+    }
+  } else {
+    st0 = gstamps;
+    st1 = gstamps + gstamp_len;
+    dummy = static_cast<u32>(gstamp_len) - kResolveBlock + t;
+  }
+  // The buffers hold kBatch positions, so chunk loads past nb stay in bounds.
+  const uint2* cand2 = reinterpret_cast<const uint2*>(cand);
+  const uint4* cmask4 = reinterpret_cast<const uint4*>(cmask);
+  uint2 nc = cand2[t];
+  uint4 nm0 = cmask4[t], nm1 = cmask4[kResolveBlock + t];
+  s_cand[t] = nc;
+  s_cmask[t] = nm0;
+  s_cmask[kResolveBlock + t] = nm1;
+  __syncthreads();
+  const std::uint8_t* s_mask8 = reinterpret_cast<const std::uint8_t*>(s_cmask);
+  const u32* s_cand32 = reinterpret_cast<const u32*>(s_cand);
+  const u32 chunks = (nb + kChunk - 1) / kChunk;
+
+  u32 pl[4] = {kNone, kNone, kNone, kNone};
+  for (u32 ch = 0; ch < chunks; ++ch) {
+    {
+      const u32 next = min(ch + 1, kBatch / kChunk - 1);
+      nc = cand2[next * kResolveBlock + t];
+      nm0 = cmask4[next * 2 * kResolveBlock + t];
+      nm1 = cmask4[next * 2 * kResolveBlock + kResolveBlock + t];
+    }
+    const u32 n_here = min(kChunk, nb - ch * kChunk);
+    // Inputs of the next position are read one position ahead.
+    u32 c_next = s_cand32[lane & 15], m_next = s_mask8[t];
+    for (u32 ii = 0; ii < n_here; ++ii) {
+      const u32 i = ch * kChunk + ii;
+      const u32 c = c_next, m = m_next;
+      const u32 ahead = min(ii + 1, kChunk - 1);
+      c_next = s_cand32[ahead * kCandWords + (lane & 15)];
+      m_next = s_mask8[ahead * kResolveBlock + t];
+      Stamp* cur = (i & 1) ? st1 : st0;
+      const Stamp stamp = kLds ? static_cast<Stamp>(i + 1) : static_cast<Stamp>(s0 + i + 1);
+#pragma unroll
+      for (u32 q = 0; q < 4; ++q) cur[((m >> q) & 1) ? pl[q] : dummy] = stamp;
+      if constexpr (kLds) {
+        // LDS stamps: the barrier needs only the LDS writes done. A full
+        // __syncthreads() would also drain the next chunk's global loads.
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      } else {
+        __syncthreads();
+      }
+
+      const u32 cc = lane < kCands ? c : kNone;
+      const bool ok = cc != kNone && cur[cc != kNone ? cc : dummy] != stamp;
+      const u64 b = __ballot(ok);
+      u32 g = kNone;
+      if (b != 0) {
+        g = __builtin_amdgcn_readlane(cc, static_cast<int>(__builtin_ctzll(b)));
+      } else {
+        const u32 resume = __builtin_amdgcn_readlane(c, kCands);
+        if (resume != kNone) g = scan_old_groups<kMode, Stamp>(st, s0 + i, resume, G_old, cur, stamp, lane);
+        // Groups opened earlier in this batch, 256 stamps per round trip.
+        for (u32 base = G_old; g == kNone && base < G_cur; base += 256) {
+          bool open[4];
+#pragma unroll
+          for (u32 q = 0; q < 4; ++q) {
+            const u32 gg = base + 64 * q + lane;
+            open[q] = gg < G_cur && cur[gg < G_cur ? gg : dummy] != stamp;
+          }
+#pragma unroll
+          for (u32 q = 0; q < 4; ++q) {
+            const u64 fb = __ballot(open[q]);
+            if (g == kNone && fb != 0) g = base + 64 * q + static_cast<u32>(__builtin_ctzll(fb));
+          }
+        }
+        if (g == kNone) g = G_cur++;
+      }
+      if (wave == (i >> 8) && lane == (i & 63)) {
+        const u32 q = (i >> 6) & 3;
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) pl[k] = k == q ? g : pl[k];
+      }
+      if (t == 0) s_gpos[i] = g;
+    }
+    // Every wave is done with this chunk's staged inputs: replace them.
+    __syncthreads();
+    s_cand[t] = nc;
+    s_cmask[t] = nm0;
+    s_cmask[kResolveBlock + t] = nm1;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *num_groups = G;
+  for (u32 k = t; k < nb; k += kResolveBlock) gpos[s0 + k] = s_gpos[k];
+  if (t == 0) *num_groups = G_cur;
 }
 
-__global__ __launch_bounds__(kGroupBlock) void greedy_qwc_kernel(
-    const u32* __restrict__ order, u32 T, const u64* __restrict__ x, const u64* __restrict__ z,
-    u64 W, u64* __restrict__ xb, u64* __restrict__ zb, u32* __restrict__ group_of,
-    u32* __restrict__ num_groups) {
-  __shared__ u32 scratch[kGroupBlock / 64];
-  u32 G = 0;
-  for (u32 s = 0; s < T; ++s) {
-    const u32 t = order[s];
-    const u64* xt = x + static_cast<u64>(t) * W;
-    const u64* zt = z + static_cast<u64>(t) * W;
-    u32 mine = UINT_MAX;
-    for (u32 g = threadIdx.x; g < G; g += kGroupBlock) {
-      bool ok = true;
-      for (u64 k = 0; k < W && ok; ++k) {
-        ok = ((xt[k] & zb[g * W + k]) ^ (zt[k] & xb[g * W + k])) == 0;
-      }
-      if (ok) {
-        mine = g;
-        break;
-      }
-    }
-    u32 g = block_min(mine, scratch);
-    if (g == UINT_MAX) g = G++;
-    for (u64 k = threadIdx.x; k < W; k += kGroupBlock) {
-      xb[g * W + k] |= xt[k];
-      zb[g * W + k] |= zt[k];
-    }
-    if (threadIdx.x == 0) group_of[t] = g;
-    __syncthreads();
+// Full mode: conflict[g] |= anticommutation row of each batch position placed
+// in g, over the words from w0 on (later positions only). Block (x, i).
+__global__ __launch_bounds__(256) void group_commit_full_kernel(GroupState st, u64 s0, u64 w0,
+                                                                const u32* __restrict__ gpos) {
+  const u64 s = s0 + blockIdx.y;
+  const u64 k = w0 + static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= st.row_words) return;
+  const u64 v = ~st.commute[s * st.row_words + k];
+  if (v != 0) {
+    atomicOr(reinterpret_cast<unsigned long long*>(st.conflict + static_cast<u64>(gpos[s]) * st.row_words + k),
+             static_cast<unsigned long long>(v));
   }
-  if (threadIdx.x == 0) *num_groups = G;
 }
 
-// Largest term count for the full mode (packed matrix and conflict rows are
-// each T^2 / 8 bytes).
+// Qubit-wise mode: Xb[g] |= x_s, Zb[g] |= z_s for each batch position s in g.
+__global__ __launch_bounds__(256) void group_commit_qwc_kernel(GroupState st, u64 s0, u32 nb,
+                                                               const u32* __restrict__ gpos) {
+  const u64 idx = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<u64>(nb) * st.W) return;
+  const u64 s = s0 + idx / st.W, k = idx % st.W;
+  const u64 dst = static_cast<u64>(gpos[s]) * st.W + k;
+  const u64 xv = st.px[s * st.W + k], zv = st.pz[s * st.W + k];
+  if (xv != 0) atomicOr(reinterpret_cast<unsigned long long*>(st.xb + dst), static_cast<unsigned long long>(xv));
+  if (zv != 0) atomicOr(reinterpret_cast<unsigned long long*>(st.zb + dst), static_cast<unsigned long long>(zv));
+}
+
+template <int kMode>
+void run_batches(const GroupState& st, u64 T, u32* count, u32* cand, std::uint8_t* cmask,
+                 u32* gpos, u32* gstamps, u64 gstamp_len, hipStream_t stream) {
+  const bool lds = T <= kLdsGroups;
+  for (u64 s0 = 0; s0 < T; s0 += kBatch) {
+    const u32 nb = static_cast<u32>(std::min<u64>(kBatch, T - s0));
+    group_candidates_kernel<kMode><<<nb, kCandBlock, 0, stream>>>(st, s0, count, cand, cmask);
+    check_launch("group_candidates");
+    if (lds) {
+      group_resolve_kernel<kMode, true><<<1, kResolveBlock, 0, stream>>>(st, s0, nb, cand, cmask, count,
+                                                                         gpos, gstamps, gstamp_len);
+    } else {
+      group_resolve_kernel<kMode, false><<<1, kResolveBlock, 0, stream>>>(st, s0, nb, cand, cmask, count,
+                                                                          gpos, gstamps, gstamp_len);
+    }
+    check_launch("group_resolve");
+    if (s0 + nb >= T) break;
+    if constexpr (kMode == kFull) {
+      const u64 w0 = (s0 + nb) >> 6;
+      const u64 words = st.row_words - w0;
+      const dim3 grid(static_cast<unsigned>((words + 255) / 256), nb);
+      group_commit_full_kernel<<<grid, 256, 0, stream>>>(st, s0, w0, gpos);
+      check_launch("group_commit_full");
+    } else if (st.W != 0) {
+      group_commit_qwc_kernel<<<grid_for(static_cast<u64>(nb) * st.W, 256), 256, 0, stream>>>(st, s0, nb, gpos);
+      check_launch("group_commit_qwc");
+    }
+  }
+}
+
+// Largest term count for the full mode (the packed matrix and the conflict
+// rows are each T^2 / 8 bytes).
 constexpr std::size_t kMaxFullGroupingTerms = std::size_t{1} << 18;
 
 }  // namespace
@@ -150,40 +437,45 @@ int wg_group_commuting(const wg_pauli_sum* op, int full, void* stream_ptr, size_
       }
       return a < b;
     });
-    // The packed commutation matrix first (its entry points take the
-    // device's scratch arena themselves, so it must not be held here yet).
-    wg_comm_matrix* matrix = nullptr;
-    std::unique_ptr<wg_comm_matrix, void (*)(wg_comm_matrix*)> guard_matrix(nullptr,
-                                                                            wg_comm_matrix_destroy);
-    const uint64_t* bits = nullptr;
-    size_t rows = 0, cols = 0, row_words = 0;
-    if (full) {
-      if (wg_comm_matrix_create(T, T, op->device, &matrix) != WG_OK) fail_runtime(wg_last_error());
-      guard_matrix.reset(matrix);
-      if (wg_commutes_with_device(op, op, matrix, T * T, stream_ptr) != WG_OK) {
-        fail_runtime(wg_last_error());
-      }
-      int dev = 0;
-      if (wg_comm_matrix_device_bits(matrix, &bits) != WG_OK ||
-          wg_comm_matrix_info(matrix, &rows, &cols, &row_words, &dev) != WG_OK) {
-        fail_runtime(wg_last_error());
-      }
-    }
     Scratch scratch(op->device);
     u32* d_order = scratch.take<u32>(T);
-    u32* d_group = scratch.take<u32>(T);
+    u32* gpos = scratch.take<u32>(T);
     u32* d_count = scratch.take<u32>(1);
+    u32* cand = scratch.take<u32>(std::size_t{kBatch} * kCandWords);
+    auto* cmask = scratch.take<std::uint8_t>(std::size_t{kBatch} * kResolveBlock);
     check_hip(hipMemcpyAsync(d_order, order.data(), T * sizeof(u32), hipMemcpyHostToDevice, stream),
               "copy host buffer to device");
+    check_hip(hipMemsetAsync(d_count, 0, sizeof(u32), stream), "memset");
+    GroupState st{};
+    st.W = W;
+    u64* px = scratch.take<u64>(std::max<std::size_t>(1, T * W));
+    u64* pz = scratch.take<u64>(std::max<std::size_t>(1, T * W));
+    st.px = px;
+    st.pz = pz;
+    if (W) {
+      gather_masks_kernel<<<grid_for(std::min<std::size_t>(T * W, std::size_t{1} << 22), 256), 256, 0,
+                            stream>>>(d_order, op->x, op->z, T, W, px, pz);
+      check_launch("gather_masks");
+    }
+    // Stamps of the resolve kernel when some group id may not fit its LDS.
+    u32* gstamps = nullptr;
+    const std::size_t gstamp_len = T + kResolveBlock;
+    if (T > kLdsGroups) {
+      gstamps = scratch.take<u32>(2 * gstamp_len);
+      check_hip(hipMemsetAsync(gstamps, 0, 2 * gstamp_len * sizeof(u32), stream), "memset");
+    }
     if (full) {
-      u64* conflict = scratch.take<u64>(checked_mul(T, row_words, "group conflict rows"));
-      check_hip(hipMemsetAsync(conflict, 0, T * row_words * sizeof(u64), stream), "memset");
+      const std::size_t row_words = (T + 63) / 64;
+      const std::size_t words = checked_mul(T, row_words, "group conflict rows");
+      u64* commute = scratch.take<u64>(words);
+      u64* conflict = scratch.take<u64>(words);
+      launch_commute_raw(px, pz, T, px, pz, T, W, commute, row_words, stream);
+      check_hip(hipMemsetAsync(conflict, 0, words * sizeof(u64), stream), "memset");
+      st.commute = commute;
+      st.row_words = row_words;
+      st.conflict = conflict;
       ProfileScope prof("group_full", stream);
-      greedy_full_kernel<<<1, kGroupBlock, 0, stream>>>(
-          d_order, static_cast<u32>(T), reinterpret_cast<const u64*>(bits), row_words, conflict, d_group,
-          d_count);
-      check_launch("greedy_full");
-      sync(stream);
+      run_batches<kFull>(st, T, d_count, cand, cmask, gpos, gstamps, gstamp_len, stream);
     } else {
       u64* xb = scratch.take<u64>(std::max<std::size_t>(1, T * W));
       u64* zb = scratch.take<u64>(std::max<std::size_t>(1, T * W));
@@ -191,18 +483,20 @@ int wg_group_commuting(const wg_pauli_sum* op, int full, void* stream_ptr, size_
         check_hip(hipMemsetAsync(xb, 0, T * W * sizeof(u64), stream), "memset");
         check_hip(hipMemsetAsync(zb, 0, T * W * sizeof(u64), stream), "memset");
       }
+      st.xb = xb;
+      st.zb = zb;
       ProfileScope prof("group_qwc", stream);
-      greedy_qwc_kernel<<<1, kGroupBlock, 0, stream>>>(d_order, static_cast<u32>(T), op->x, op->z, W,
-                                                       xb, zb, d_group, d_count);
-      check_launch("greedy_qwc");
+      run_batches<kQwc>(st, T, d_count, cand, cmask, gpos, gstamps, gstamp_len, stream);
     }
     std::vector<u32> group_of(T);
     u32 G = 0;
-    check_hip(hipMemcpyAsync(group_of.data(), d_group, T * sizeof(u32), hipMemcpyDeviceToHost, stream),
+    std::vector<u32> gof_pos(T);
+    check_hip(hipMemcpyAsync(gof_pos.data(), gpos, T * sizeof(u32), hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
     check_hip(hipMemcpyAsync(&G, d_count, sizeof(u32), hipMemcpyDeviceToHost, stream),
               "copy device buffer to host");
     sync(stream);
+    for (std::size_t p = 0; p < T; ++p) group_of[order[p]] = gof_pos[p];
     // Groups in creation order, members in placement (visiting) order.
     std::vector<u32> count(G, 0), start(G + 1, 0);
     for (std::size_t t = 0; t < T; ++t) {

@@ -237,4 +237,11 @@ class ProfileScope {
 void exclusive_scan_u32(const u32* in, u32* out, std::size_t n, u32* total_dev, Scratch& scratch,
                         hipStream_t stream);
 
+// Packed commutation bits (wg_commute.hip): bit c of row r of out
+// [L x row_words] is 1 when term r of (lx, lz) commutes with term c of (rx, rz);
+// both operands hold W mask words per term.
+void launch_commute_raw(const u64* lx, const u64* lz, std::size_t L, const u64* rx, const u64* rz,
+                        std::size_t R, std::size_t W, u64* out, std::size_t row_words,
+                        hipStream_t stream);
+
 }  // namespace wg

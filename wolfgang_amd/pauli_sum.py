@@ -547,12 +547,18 @@ class DevicePauliSum:
         _native.call("wg_group_commuting", self._handle(), int(mode == "full"), _stream_arg(stream),
                      C.byref(count), sizes.ctypes.data, members.ctypes.data)
         host = self.to_host()
-        x, z, c = host.x_words(), host.z_words(), host.coeffs()
-        out, k = [], 0
-        for g in range(count.value):
-            idx = members[k:k + sizes[g]].astype(np.int64)
-            out.append(PauliSum._from_arrays(self._n, x[idx], z[idx], c[idx]))
-            k += int(sizes[g])
+        # One gather into group order; each group is a disjoint slice of it.
+        idx = members.astype(np.int64)
+        x = np.ascontiguousarray(host._x[idx])
+        z = np.ascontiguousarray(host._z[idx])
+        c = np.ascontiguousarray(host._c[idx])
+        ends = np.cumsum(sizes[:count.value], dtype=np.int64).tolist()
+        out, a, n, new = [], 0, self._n, PauliSum.__new__
+        for b in ends:
+            g = new(PauliSum)
+            g._n, g._x, g._z, g._c = n, x[a:b], z[a:b], c[a:b]
+            out.append(g)
+            a = b
         return out
 
     def commutes_with_into(
