@@ -272,14 +272,10 @@ __global__ __launch_bounds__(kResolveBlock) void group_resolve_kernel(
       nm1 = cmask4[next * 2 * kResolveBlock + kResolveBlock + t];
     }
     const u32 n_here = min(kChunk, nb - ch * kChunk);
-    // Inputs of the next position are read one position ahead.
-    u32 c_next = s_cand32[lane & 15], m_next = s_mask8[t];
     for (u32 ii = 0; ii < n_here; ++ii) {
       const u32 i = ch * kChunk + ii;
-      const u32 c = c_next, m = m_next;
-      const u32 ahead = min(ii + 1, kChunk - 1);
-      c_next = s_cand32[ahead * kCandWords + (lane & 15)];
-      m_next = s_mask8[ahead * kResolveBlock + t];
+      const u32 c = s_cand32[ii * kCandWords + (lane & 15)];
+      const u32 m = s_mask8[ii * kResolveBlock + t];
       Stamp* cur = (i & 1) ? st1 : st0;
       const Stamp stamp = kLds ? static_cast<Stamp>(i + 1) : static_cast<Stamp>(s0 + i + 1);
 #pragma unroll
@@ -292,8 +288,11 @@ __global__ __launch_bounds__(kResolveBlock) void group_resolve_kernel(
         __syncthreads();
       }
 
+      // Stamp reads are unconditional (invalid lanes read their dummy slot),
+      // so they are issued together rather than as masked branches.
       const u32 cc = lane < kCands ? c : kNone;
-      const bool ok = cc != kNone && cur[cc != kNone ? cc : dummy] != stamp;
+      const Stamp cs = cur[cc != kNone ? cc : dummy];
+      const bool ok = cc != kNone && cs != stamp;
       const u64 b = __ballot(ok);
       u32 g = kNone;
       if (b != 0) {
@@ -303,12 +302,15 @@ __global__ __launch_bounds__(kResolveBlock) void group_resolve_kernel(
         if (resume != kNone) g = scan_old_groups<kMode, Stamp>(st, s0 + i, resume, G_old, cur, stamp, lane);
         // Groups opened earlier in this batch, 256 stamps per round trip.
         for (u32 base = G_old; g == kNone && base < G_cur; base += 256) {
-          bool open[4];
+          Stamp v[4];
 #pragma unroll
           for (u32 q = 0; q < 4; ++q) {
             const u32 gg = base + 64 * q + lane;
-            open[q] = gg < G_cur && cur[gg < G_cur ? gg : dummy] != stamp;
+            v[q] = cur[gg < G_cur ? gg : dummy];
           }
+          bool open[4];
+#pragma unroll
+          for (u32 q = 0; q < 4; ++q) open[q] = base + 64 * q + lane < G_cur && v[q] != stamp;
 #pragma unroll
           for (u32 q = 0; q < 4; ++q) {
             const u64 fb = __ballot(open[q]);
