@@ -43,6 +43,7 @@
 #include <cstdint>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -393,6 +394,9 @@ void run_batches(const GroupState& st, u64 T, u32* count, u32* cand, std::uint8_
   }
 }
 
+// Visiting orders of at least this many terms are sorted on four threads.
+constexpr std::size_t kParallelSortTerms = 8192;
+
 // Largest term count for the full mode (the packed matrix and the conflict
 // rows are each T^2 / 8 bytes).
 constexpr std::size_t kMaxFullGroupingTerms = std::size_t{1} << 18;
@@ -431,14 +435,30 @@ int wg_group_commuting(const wg_pauli_sum* op, int full, void* stream_ptr, size_
       order[t] = static_cast<u32>(t);
       for (std::size_t w = 0; w < W; ++w) weight[t] += __builtin_popcountll(hx[t * W + w] | hz[t * W + w]);
     }
-    std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
+    const auto before = [&](u32 a, u32 b) {
       if (weight[a] != weight[b]) return weight[a] > weight[b];
       for (std::size_t w = 0; w < W; ++w) {
         if (hx[a * W + w] != hx[b * W + w]) return hx[a * W + w] < hx[b * W + w];
         if (hz[a * W + w] != hz[b * W + w]) return hz[a * W + w] < hz[b * W + w];
       }
       return a < b;
-    });
+    };
+    // The order is total (ties end on the index), so sorting quarters on
+    // their own threads and merging gives the same permutation.
+    if (T >= kParallelSortTerms) {
+      const std::size_t q[5] = {0, T / 4, T / 2, 3 * T / 4, T};
+      std::vector<std::thread> workers;
+      for (int k = 1; k < 4; ++k) {
+        workers.emplace_back([&, k] { std::sort(order.begin() + q[k], order.begin() + q[k + 1], before); });
+      }
+      std::sort(order.begin(), order.begin() + q[1], before);
+      for (auto& th : workers) th.join();
+      std::inplace_merge(order.begin(), order.begin() + q[1], order.begin() + q[2], before);
+      std::inplace_merge(order.begin() + q[2], order.begin() + q[3], order.end(), before);
+      std::inplace_merge(order.begin(), order.begin() + q[2], order.end(), before);
+    } else {
+      std::sort(order.begin(), order.end(), before);
+    }
     Scratch scratch(op->device);
     u32* d_order = scratch.take<u32>(T);
     u32* gpos = scratch.take<u32>(T);
