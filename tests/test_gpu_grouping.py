@@ -167,3 +167,14 @@ def test_grouping_benchmark_operator(gpu, orc, mode):
     for g, idx in zip(got, want):
         np.testing.assert_array_equal(g.x_words(), op.x_words()[idx])
         np.testing.assert_array_equal(g.z_words(), op.z_words()[idx])
+
+
+@pytest.mark.parametrize("mode", ["qwc", "full"])
+@pytest.mark.parametrize("terms", [3, 9000])
+def test_grouping_zero_qubits(gpu, orc, mode, terms):
+    # No mask words: every term is the identity and all share one group.
+    op = PauliSum.from_labels([""] * terms, np.arange(1, terms + 1, dtype=np.float64))
+    want = orc.group_commuting(0, op.x_words(), op.z_words(), mode)
+    got = op.group_commuting(mode=mode)
+    assert len(want) == 1 and len(got) == 1
+    np.testing.assert_array_equal(got[0].coeffs(), op.coeffs()[want[0]])
