@@ -1969,7 +1969,7 @@ u32 span_seg(const Span& sp, u64 x, int l) {
 // Perturbation rounds of the packing search, for states of kOwnerWideMinQubits
 // and up (where a bundle reads gigabytes) and up to this many x-groups
 // (structured operators; each round is a full local search).
-constexpr int kPackRounds = 48;
+constexpr int kPackRounds = 100;
 constexpr std::size_t kPackRoundsMaxGroups = 1024;
 
 void pack_owner_passes(const std::vector<ExpItem>& items, const std::vector<u64>& xs,
